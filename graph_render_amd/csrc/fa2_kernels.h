@@ -1,0 +1,98 @@
+// Device buffers of one FA2 session and the host-side launchers of its kernels
+// (fa2_kernels.hip). Everything here is resident in HBM for the session's life; nothing is
+// allocated per iteration.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gr_device.h"
+
+namespace gr {
+
+enum Repulsion : int { REP_TREE = 0, REP_DENSE = 1, REP_PM = 2, REP_NONE = 3 };
+enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_KAHAN = 2 };
+
+// Force parts of one evaluation (gr_fa2_session_forces): the reference's iterate() order.
+enum Parts : uint32_t { PART_ATTRACTION = 1, PART_REPULSION = 2, PART_GRAVITY = 4 };
+
+struct FaDev {
+    uint32_t n = 0, dim = 2;
+    int precision = PREC_FP64;
+    int repulsion = REP_TREE;
+    double theta2 = 0.8 * 0.8;
+    double scaling_ratio = 2.0, gravity = 1.0, jitter_tolerance = 1.0;
+    uint32_t seed = 0;
+    int weighted = 0;
+    uint32_t m2 = 0;  // CSR entries (2 per simple edge)
+
+    // state, element type T (double or float) chosen by precision
+    void *pos[3] = {};
+    void *mass = nullptr;
+    void *u[3] = {};
+    void *rep[3] = {};
+    uint32_t *row_off = nullptr, *nbr = nullptr;
+    void *w = nullptr;
+    GrState *st = nullptr;
+
+    // tree
+    uint64_t *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *vals = nullptr, *vals_alt = nullptr;
+    uint8_t *stall = nullptr;
+    uint32_t *order = nullptr;
+    void *sp[3] = {};
+    void *sm = nullptr;
+    uint32_t *flag = nullptr, *gid = nullptr, *dflag = nullptr;
+    uint32_t *gstart = nullptr, *gcells = nullptr, *gbase = nullptr, *ginfo = nullptr;
+    uint32_t *glocal = nullptr;
+    uint64_t *gkey = nullptr;
+    uint32_t cap = 0;  // cell capacity
+    void *wc = nullptr;
+    int32_t *parent = nullptr;
+    uint32_t *cnt = nullptr;
+    double *raw = nullptr;      // unnormalised (mass, m*x...) per cell
+    double *cbounds = nullptr;  // 2*dim per cell when the session exports the tree
+    int export_bounds = 0;
+
+    // deep-group builder pools
+    uint32_t pool_cap = 0, lcell_cap = 0;
+    int32_t *pool = nullptr;
+    int32_t *lc_int = nullptr;
+    double *lc_dbl = nullptr;
+    int32_t *chain_next = nullptr;
+
+    // reductions
+    uint32_t nparts = 0;
+    double *bpart = nullptr;  // bounds/mean partials
+    double *spart = nullptr;  // swing/traction partials
+    double *mpart = nullptr;  // moved partials
+
+    void *scan_tmp = nullptr;
+    void *sort_tmp = nullptr;
+
+    // PM grid (see pm.hip)
+    uint32_t pm_side = 0;
+    void *pm = nullptr;
+
+    int ignore_done = 0;  // set on force evaluations, which run after an early exit too
+
+    hipStream_t stream = nullptr;
+};
+
+size_t walkcell_bytes(int precision, uint32_t dim);
+
+// Positions changed from the host: recompute the bounds/mean partials.
+void launch_bounds(FaDev &d);
+// One full iteration (after launch_bounds or a previous iteration's apply).
+void launch_iteration(FaDev &d);
+// One force evaluation on the current positions: builds the tree when needed and leaves the
+// force in d.u; no state moves (swing/traction are not accumulated, nothing applied).
+void launch_forces(FaDev &d, uint32_t parts);
+// The tree build alone (after launch_bounds): keys, sort, topology, centre of mass.
+void launch_tree(FaDev &d);
+// The walk alone over the current tree, for timing.
+void launch_walk(FaDev &d);
+// Clears the halt flag left by a failed capacity check.
+void launch_clear_halt(FaDev &d);
+
+}  // namespace gr

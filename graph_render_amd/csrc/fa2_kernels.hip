@@ -1,0 +1,1377 @@
+// ForceAtlas2 with Barnes-Hut repulsion on gfx950: tree build, centre of mass, the
+// wave-coherent stackless walk, CSR attraction, gravity, the adaptive speed and the step.
+//
+// Reference semantics (paths relative to crates/graph-core/src/):
+//   tree          layout/force/quadtree.rs:116-195, quadtree/{bounds,build,preorder}.rs
+//                 (3D: layout/force/octree.rs, slot z<<2|y<<1|x)
+//   centre        layout/forceatlas2/state/barnes_hut.rs:62-110
+//   walk, pairs   layout/forceatlas2/state/barnes_hut.rs:112-184
+//   attraction    layout/forceatlas2/state.rs:152-162
+//   dense pairs   layout/forceatlas2/state.rs:169-203
+//   gravity       layout/forceatlas2/state.rs:208-251
+//   speed, step   layout/forceatlas2/state/schedule.rs:16-77
+// FP64 kernels keep the reference's operand order and are built without FMA contraction,
+// so the tree topology, the centres and the per-body force sums match it bit for bit.
+#include <math.h>
+
+#include "fa2_kernels.h"
+#include "sort_scan.h"
+
+namespace gr {
+
+// ------------------------------------------------------------------ common
+
+template <int D>
+struct KeyFmt {
+    // Levels held in one 64-bit key (D bits each); bit 63 marks a non-finite point.
+    static constexpr int LEVELS = D == 2 ? 31 : 21;
+    static constexpr int USED = D * LEVELS;
+};
+
+template <typename T, int D>
+struct WalkCell {
+    T c[D];
+    T mass;
+    T w2;
+    uint32_t skip, start, end;
+};
+
+size_t walkcell_bytes(int precision, uint32_t dim) {
+    if (precision == PREC_FP64)
+        return dim == 2 ? sizeof(WalkCell<double, 2>) : sizeof(WalkCell<double, 3>);
+    return dim == 2 ? sizeof(WalkCell<float, 2>) : sizeof(WalkCell<float, 3>);
+}
+
+template <int D>
+struct Box {
+    double lo[D], hi[D];
+};
+
+// bounds.rs:34-50 (octree/bounds.rs narrow): the slot's bit a is set when p[a] >= mid[a].
+template <int D>
+__device__ __forceinline__ int box_narrow(Box<D> &b, const double *p) {
+    int slot = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double m = (b.lo[a] + b.hi[a]) / 2.0;
+        if (p[a] >= m) {
+            b.lo[a] = m;
+            slot |= 1 << a;
+        } else {
+            b.hi[a] = m;
+        }
+    }
+    return slot;
+}
+
+template <int D>
+__device__ __forceinline__ int box_slot(const Box<D> &b, const double *p) {
+    int slot = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double m = (b.lo[a] + b.hi[a]) / 2.0;
+        if (p[a] >= m) slot |= 1 << a;
+    }
+    return slot;
+}
+
+// bounds.rs:62-86 quadrant / octree/bounds.rs octant.
+template <int D>
+__device__ __forceinline__ void box_quad(Box<D> &b, int slot) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double m = (b.lo[a] + b.hi[a]) / 2.0;
+        if (slot & (1 << a))
+            b.lo[a] = m;
+        else
+            b.hi[a] = m;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ Box<D> root_box(const GrState *st) {
+    Box<D> b;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        b.lo[a] = st->root_lo[a];
+        b.hi[a] = st->root_hi[a];
+    }
+    return b;
+}
+
+template <int D>
+__device__ __forceinline__ int key_slot(uint64_t key, int level) {
+    return (int)((key >> (D * (KeyFmt<D>::LEVELS - level))) & ((1u << D) - 1));
+}
+
+// Levels two finite keys share below the root.
+template <int D>
+__device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b) {
+    uint64_t x = a ^ b;
+    if (x == 0) return KeyFmt<D>::LEVELS;
+    int lz = __clzll((long long)x) - (64 - KeyFmt<D>::USED);
+    return lz / D;
+}
+
+template <int D>
+__device__ __forceinline__ bool key_same(uint64_t a, uint64_t b, int level) {
+    if (level <= 0) return true;
+    return ((a ^ b) >> (D * (KeyFmt<D>::LEVELS - level))) == 0;
+}
+
+// rng.rs:27-84: MurmurHash3 fmix64, the three-round fold and the 53-bit nudge.
+__device__ __forceinline__ uint64_t d_fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+__device__ __noinline__ double d_jiggle(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t i,
+                                        uint32_t j) {
+    uint32_t lo = i <= j ? i : j, hi = i <= j ? j : i;
+    uint64_t a = d_fmix64((uint64_t)seed ^ ((uint64_t)tick << 32));
+    uint64_t b = d_fmix64(a ^ ((uint64_t)pass << 32) ^ (uint64_t)lo);
+    uint64_t w = d_fmix64(b ^ ((uint64_t)hi << 32) ^ a) >> 11;
+    if (w == (1ull << 52)) w = (1ull << 52) + 1;
+    double u = (double)w * (1.0 / (double)(1ull << 53));
+    return (u - 0.5) * 1e-6;
+}
+
+// Iteration kernels stop once the run has exited early or a capacity check failed; a
+// force evaluation (ignore_done) still runs on an exited state.
+__device__ __forceinline__ bool halted(const FaDev &d) {
+    const GrState *st = d.st;
+    return st->halt != 0 || (!d.ignore_done && st->done != 0);
+}
+
+// Compensated (Kahan) accumulation, used by the FP32+Kahan precision.
+template <typename T, bool KAHAN>
+__device__ __forceinline__ void acc_add(T &s, T &c, T v) {
+    if (KAHAN) {
+        T y = v - c;
+        T t = s + y;
+        c = (t - s) - y;
+        s = t;
+    } else {
+        s = s + v;
+    }
+}
+
+constexpr int NB = 10;  // bounds partial: lo[3], hi[3], sum[3], finite count
+
+// Block reduction of NV values (min, max or sum) in a fixed order; thread 0 gets the result.
+template <int NV, int OP>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        if (OP == 0) v[q] = gr_wave_sum(v[q]);
+        if (OP == 1) v[q] = gr_wave_min(v[q]);
+        if (OP == 2) v[q] = gr_wave_max(v[q]);
+        if (lane == 0) lds[wave * NV + q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double r = lds[q];
+            for (int w = 1; w < GR_BLOCK / GR_WAVE; w++) {
+                double o = lds[w * NV + q];
+                if (OP == 0) r = r + o;
+                if (OP == 1) r = fmin(r, o);
+                if (OP == 2) r = fmax(r, o);
+            }
+            v[q] = r;
+        }
+    }
+    __syncthreads();
+}
+
+// One point's contribution to the bounds/centroid partials (quadtree.rs:46-53, the finite
+// extent; state.rs:208-220, the centroid over every point).
+template <int D>
+__device__ __forceinline__ void bounds_take(const double *p, double *lo, double *hi, double *sum,
+                                            double &nfin) {
+    bool fin = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        sum[a] = sum[a] + p[a];
+        if (!isfinite(p[a])) fin = false;
+    }
+    if (fin) {
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            lo[a] = fmin(lo[a], p[a]);
+            hi[a] = fmax(hi[a], p[a]);
+        }
+        nfin += 1.0;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void bounds_write(double *lo, double *hi, double *sum, double nfin,
+                                             double *bpart) {
+    __shared__ double lds[4 * 3];
+    double vmin[3] = {lo[0], lo[1], D == 3 ? lo[2] : 0.0};
+    double vmax[3] = {hi[0], hi[1], D == 3 ? hi[2] : 0.0};
+    double vsum[4] = {sum[0], sum[1], D == 3 ? sum[2] : 0.0, nfin};
+    block_reduce<3, 1>(vmin, lds);
+    block_reduce<3, 2>(vmax, lds);
+    __shared__ double lds4[4 * 4];
+    block_reduce<4, 0>(vsum, lds4);
+    if (threadIdx.x == 0) {
+        double *o = bpart + (size_t)blockIdx.x * NB;
+        for (int a = 0; a < 3; a++) {
+            o[a] = vmin[a];
+            o[3 + a] = vmax[a];
+            o[6 + a] = vsum[a];
+        }
+        o[9] = vsum[3];
+    }
+}
+
+// ------------------------------------------------------------------ bounds / root
+
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_bounds(const FaDev d) {
+    const T *P[3] = {(const T *)d.pos[0], (const T *)d.pos[1],
+                     (const T *)(D == 3 ? d.pos[2] : d.pos[0])};
+    double lo[D], hi[D], sum[D], nfin = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+        sum[a] = 0.0;
+    }
+    for (uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * GR_BLOCK) {
+        double p[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) p[a] = (double)P[a][i];
+        bounds_take<D>(p, lo, hi, sum, nfin);
+    }
+    bounds_write<D>(lo, hi, sum, nfin, d.bpart);
+}
+
+// quadtree.rs:157-195 (octree.rs:220-254): the root square from the floor of the first
+// corner, doubled until it holds the second; a square that stops growing is left as is.
+template <int D>
+__device__ void cover_root(GrState *st, const double *lo, const double *hi) {
+    double blo[D], bhi[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double h = floor(lo[a]) + 1.0;
+        blo[a] = h - 1.0;
+        bhi[a] = h;
+    }
+    double z = (bhi[0] - blo[0] != 0.0) ? bhi[0] - blo[0] : 1.0;
+    for (;;) {
+        bool out = false;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            if (blo[a] > hi[a] || hi[a] >= bhi[a]) out = true;
+        if (!out) break;
+        z *= 2.0;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            if (hi[a] < blo[a])
+                blo[a] = bhi[a] - z;
+            else
+                bhi[a] = blo[a] + z;
+        }
+        if (!(isfinite(z) && isfinite(bhi[0] - blo[0]))) break;
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        st->root_lo[a] = blo[a];
+        st->root_hi[a] = bhi[a];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ double lds[4 * 4];
+    double vmin[3] = {INFINITY, INFINITY, INFINITY};
+    double vmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double vsum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) {
+        const double *o = d.bpart + (size_t)b * NB;
+        for (int a = 0; a < 3; a++) {
+            vmin[a] = fmin(vmin[a], o[a]);
+            vmax[a] = fmax(vmax[a], o[3 + a]);
+            vsum[a] = vsum[a] + o[6 + a];
+        }
+        vsum[3] = vsum[3] + o[9];
+    }
+    block_reduce<3, 1>(vmin, lds);
+    block_reduce<3, 2>(vmax, lds);
+    block_reduce<4, 0>(vsum, lds);
+    if (threadIdx.x == 0) {
+        uint32_t nfin = (uint32_t)vsum[3];
+        st->n_finite = nfin;
+        st->has_root = nfin > 0;
+        for (int a = 0; a < D; a++) st->mean[a] = vsum[a] / (double)d.n;
+        if (nfin > 0) cover_root<D>(st, vmin, vmax);
+    }
+}
+
+// ------------------------------------------------------------------ keys
+
+// Each point replays the reference's own descent (narrow from the root square) for LEVELS
+// levels, so its key bits are exactly the slots d3's insertion would take. A level whose
+// narrowing no longer shrinks the x-span is where the reference stops splitting and chains
+// instead (build.rs:76-85); the key stops there and the level is kept as the point's stall.
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= d.n) return;
+    double p[D];
+    bool fin = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        p[a] = (double)((const T *)d.pos[a])[i];
+        if (!isfinite(p[a])) fin = false;
+    }
+    uint64_t key = 1ull << 63;
+    uint32_t stall = 0xFF;
+    if (fin) {
+        key = 0;
+        Box<D> b = root_box<D>(st);
+        for (int l = 1; l <= KeyFmt<D>::LEVELS; l++) {
+            double before = b.hi[0] - b.lo[0];
+            int slot = box_narrow<D>(b, p);
+            double after = b.hi[0] - b.lo[0];
+            if (!(after < before)) {
+                stall = (uint32_t)(l - 1);
+                break;
+            }
+            key |= (uint64_t)slot << (D * (KeyFmt<D>::LEVELS - l));
+        }
+    }
+    // Reverse input order: the stable sort then leaves equal keys in descending point
+    // index, the order a leaf chains its points in (most recent first, build.rs:112-115).
+    uint32_t j = d.n - 1 - i;
+    d.keys[j] = key;
+    d.vals[j] = i;
+    d.stall[i] = (uint8_t)stall;
+}
+
+// ------------------------------------------------------------------ groups
+
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_gather(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (p >= d.n) return;
+    const uint32_t nf = st->n_finite;
+    // Non-finite points follow the tree's points in ascending index (preorder.rs:54-59);
+    // the sort left them descending, so that tail is read backwards.
+    uint32_t src = p < nf ? p : nf + (d.n - 1 - p);
+    uint32_t i = d.vals[src];
+    uint64_t key = d.keys[src];
+    d.order[p] = i;
+#pragma unroll
+    for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = ((const T *)d.pos[a])[i];
+    ((T *)d.sm)[p] = ((const T *)d.mass)[i];
+    uint32_t newgroup = 0, distinct = 0;
+    if (p < nf) {
+        if (p == 0 || d.keys[p - 1] != key) {
+            newgroup = 1;
+        } else {
+            uint32_t q = d.vals[p - 1];
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                if (((const T *)d.pos[a])[i] != ((const T *)d.pos[a])[q]) distinct = 1;
+        }
+    }
+    d.flag[p] = newgroup;
+    d.dflag[p] = distinct;
+}
+
+template <int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_group_compact(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (p >= st->n_finite || !d.flag[p]) return;
+    uint32_t g = d.gid[p];
+    d.gstart[g] = p;
+    d.gkey[g] = d.keys[p];
+}
+
+enum GroupType : uint32_t { G_SIMPLE = 0, G_STALL = 1, G_DEEP = 2 };
+
+// The subtree under a deep group's level-LEVELS cell: the reference's own insertion
+// (build.rs:26-115) over the group's points in ascending index, then its preorder flatten
+// (preorder.rs:41-99), written to the local-cell pool. Runs on one lane per deep group;
+// deep groups (distinct points sharing LEVELS levels) are rare.
+template <typename T, int D>
+__device__ bool local_build(const FaDev &d, uint32_t s, uint32_t e, Box<D> rootb,
+                            uint32_t *out_lc, uint32_t *out_count) {
+    GrState *st = d.st;
+    constexpr int NS = 1 + (1 << D);
+    int32_t *pool = d.pool;
+    auto coords = [&](uint32_t i, double *p) {
+#pragma unroll
+        for (int a = 0; a < D; a++) p[a] = (double)((const T *)d.pos[a])[i];
+    };
+    auto push = [&](int32_t head) -> int32_t {
+        uint32_t at = atomicAdd(&st->pool_used, (uint32_t)NS);
+        if (at + NS > d.pool_cap) {
+            st->pool_overflow = 1;
+            return -1;
+        }
+        pool[at] = head;
+        for (int c = 0; c < (1 << D); c++) pool[at + 1 + c] = -1;
+        return (int32_t)at;
+    };
+    int32_t root = -1;
+    uint32_t nshapes = 0;
+    bool ok = true;
+    for (uint32_t q = e; q-- > s && ok;) {
+        uint32_t pt = d.order[q];
+        d.chain_next[pt] = -1;
+        double p[D];
+        coords(pt, p);
+        if (root < 0) {
+            root = push((int32_t)pt);
+            if (root < 0) ok = false;
+            nshapes++;
+            continue;
+        }
+        int32_t node = root, parent = -1;
+        int pslot = 0;
+        Box<D> b = rootb;
+        for (;;) {
+            if (pool[node] >= 0) {
+                // insert_leaf (build.rs:60-96)
+                int32_t head = pool[node];
+                double h[D];
+                coords((uint32_t)head, h);
+                bool same = true;
+                for (int a = 0; a < D; a++)
+                    if (p[a] != h[a]) same = false;
+                if (same) {
+                    d.chain_next[pt] = head;
+                    pool[node] = (int32_t)pt;
+                    break;
+                }
+                int32_t at = parent;
+                int slot = parent >= 0 ? pslot : 0;
+                for (;;) {
+                    double before = b.hi[0] - b.lo[0];
+                    Box<D> pre = b;
+                    int i = box_narrow<D>(b, p);
+                    int j = box_slot<D>(pre, h);
+                    double after = b.hi[0] - b.lo[0];
+                    if (!(after < before)) {
+                        if (at >= 0)
+                            pool[at + 1 + slot] = node;
+                        else
+                            root = node;
+                        d.chain_next[pt] = pool[node];
+                        pool[node] = (int32_t)pt;
+                        break;
+                    }
+                    int32_t internal = push(-1);
+                    if (internal < 0) {
+                        ok = false;
+                        break;
+                    }
+                    nshapes++;
+                    if (at >= 0)
+                        pool[at + 1 + slot] = internal;
+                    else
+                        root = internal;
+                    if (i != j) {
+                        pool[internal + 1 + j] = node;
+                        int32_t leaf = push((int32_t)pt);
+                        if (leaf < 0) {
+                            ok = false;
+                            break;
+                        }
+                        nshapes++;
+                        pool[internal + 1 + i] = leaf;
+                        break;
+                    }
+                    at = internal;
+                    slot = i;
+                }
+                break;
+            }
+            int slot = box_narrow<D>(b, p);
+            int32_t child = pool[node + 1 + slot];
+            if (child >= 0) {
+                parent = node;
+                pslot = slot;
+                node = child;
+            } else {
+                int32_t leaf = push((int32_t)pt);
+                if (leaf < 0) {
+                    ok = false;
+                    break;
+                }
+                nshapes++;
+                pool[node + 1 + slot] = leaf;
+                break;
+            }
+        }
+    }
+    if (!ok) return false;
+    // flatten into local cells; stack entries (node or -1, cell/parent, slot)
+    uint32_t lc = atomicAdd(&st->lcell_used, nshapes);
+    uint32_t stk = atomicAdd(&st->pool_used, 3u * (2u * nshapes + 2u));
+    if (lc + nshapes > d.lcell_cap || stk + 3u * (2u * nshapes + 2u) > d.pool_cap) {
+        st->pool_overflow = 1;
+        return false;
+    }
+    int32_t *stack = pool + stk;
+    int top = 0;
+    stack[0] = root;
+    stack[1] = -1;
+    stack[2] = 0;
+    top = 1;
+    uint32_t ncell = 0, cursor = 0;
+    while (top > 0) {
+        top--;
+        int32_t node = stack[3 * top], a1 = stack[3 * top + 1], a2 = stack[3 * top + 2];
+        if (node < 0) {  // close cell a1
+            d.lc_int[(size_t)(lc + a1) * 4 + 0] = (int32_t)ncell;
+            d.lc_int[(size_t)(lc + a1) * 4 + 2] = (int32_t)(s + cursor);
+            continue;
+        }
+        uint32_t k = ncell++;
+        double *bk = d.lc_dbl + (size_t)(lc + k) * 2 * D;
+        Box<D> b;
+        if (a1 < 0) {
+            b = rootb;
+        } else {
+            const double *bp = d.lc_dbl + (size_t)(lc + a1) * 2 * D;
+            for (int a = 0; a < D; a++) {
+                b.lo[a] = bp[a];
+                b.hi[a] = bp[D + a];
+            }
+            box_quad<D>(b, a2);
+        }
+        for (int a = 0; a < D; a++) {
+            bk[a] = b.lo[a];
+            bk[D + a] = b.hi[a];
+        }
+        int32_t *ik = d.lc_int + (size_t)(lc + k) * 4;
+        ik[1] = (int32_t)(s + cursor);
+        ik[3] = a1;
+        if (pool[node] >= 0) {
+            for (int32_t q = pool[node]; q >= 0; q = d.chain_next[q]) {
+                uint32_t at = s + cursor++;
+                d.order[at] = (uint32_t)q;
+#pragma unroll
+                for (int a = 0; a < D; a++) ((T *)d.sp[a])[at] = ((const T *)d.pos[a])[q];
+                ((T *)d.sm)[at] = ((const T *)d.mass)[q];
+            }
+            ik[0] = (int32_t)(k + 1);
+            ik[2] = (int32_t)(s + cursor);
+        } else {
+            stack[3 * top] = -1;
+            stack[3 * top + 1] = (int32_t)k;
+            stack[3 * top + 2] = 0;
+            top++;
+            for (int slot = (1 << D) - 1; slot >= 0; slot--) {
+                int32_t c = pool[node + 1 + slot];
+                if (c >= 0) {
+                    stack[3 * top] = c;
+                    stack[3 * top + 1] = (int32_t)k;
+                    stack[3 * top + 2] = slot;
+                    top++;
+                }
+            }
+        }
+    }
+    *out_lc = lc;
+    *out_count = nshapes;
+    return true;
+}
+
+// Per group: how many cells it owns in preorder. A group owns the internal cells from one
+// level below its shared prefix with the previous group down to its own depth, then its
+// leaf (or, for a deep group, its local subtree).
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_group_info(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
+    const uint32_t G = st->n_groups;
+    if (g >= G) return;
+    constexpr int L = KeyFmt<D>::LEVELS;
+    uint64_t key = d.gkey[g];
+    int left = g > 0 ? key_lcp<D>(d.gkey[g - 1], key) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, d.gkey[g + 1]) : -1;
+    uint32_t s = d.gstart[g];
+    uint32_t e = g + 1 < G ? d.gstart[g + 1] : st->n_finite;
+    bool distinct = false;
+    for (uint32_t q = s + 1; q < e; q++)
+        if (d.dflag[q]) {
+            distinct = true;
+            break;
+        }
+    uint32_t type, level = 0, ncells;
+    if (!distinct) {
+        type = G_SIMPLE;
+        level = (uint32_t)((left > right ? left : right) + 1);
+        ncells = (uint32_t)((right > left ? right - left : 0) + 1);
+    } else {
+        uint32_t stall = d.stall[d.order[s]];
+        if (stall != 0xFF) {
+            type = G_STALL;
+            level = stall;
+            ncells = (uint32_t)((int)stall - left);
+        } else {
+            type = G_DEEP;
+            level = L;
+            Box<D> b = root_box<D>(st);
+            for (int l = 1; l <= L; l++) box_quad<D>(b, key_slot<D>(key, l));
+            uint32_t lc = 0, lcount = 0;
+            if (!local_build<T, D>(d, s, e, b, &lc, &lcount)) {
+                st->halt = 1;
+                lcount = 1;
+            }
+            d.glocal[g] = lc;
+            ncells = (uint32_t)((L - 1) - left) + lcount;
+            st->any_deep = 1;
+        }
+    }
+    d.gcells[g] = ncells;
+    d.ginfo[g] = type | (level << 8);
+}
+
+__global__ void k_cells_ready(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    if (st->need_cells > d.cap) {
+        st->halt = 1;
+        return;
+    }
+    st->n_cells = st->need_cells;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_zero_cnt(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k < st->n_cells) d.cnt[k] = 0;
+}
+
+// The last group h >= g whose key shares `level` levels with group g's.
+template <int D>
+__device__ uint32_t gallop_fwd(const uint64_t *gkey, uint32_t G, uint32_t g, int level) {
+    uint64_t key = gkey[g];
+    uint32_t lo = g, step = 1;
+    while (lo + step < G && key_same<D>(key, gkey[lo + step], level)) {
+        lo += step;
+        step <<= 1;
+    }
+    uint32_t hi = lo + step < G ? lo + step : G;  // first index known (or assumed) not to match
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (key_same<D>(key, gkey[mid], level))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// The first group f <= g whose key shares `level` levels with group g's.
+template <int D>
+__device__ uint32_t gallop_bwd(const uint64_t *gkey, uint32_t g, int level) {
+    uint64_t key = gkey[g];
+    uint32_t hi = g, step = 1;
+    while (hi >= step && key_same<D>(key, gkey[hi - step], level)) {
+        hi -= step;
+        step <<= 1;
+    }
+    int64_t lo = (int64_t)hi - (int64_t)step;  // last index known (or assumed) not to match
+    if (lo < -1) lo = -1;
+    while ((int64_t)hi - lo > 1) {
+        int64_t mid = lo + ((int64_t)hi - lo) / 2;
+        if (key_same<D>(key, gkey[mid], level))
+            hi = (uint32_t)mid;
+        else
+            lo = mid;
+    }
+    return hi;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<D> &b,
+                                          uint32_t skip, uint32_t start, uint32_t end,
+                                          int32_t parent) {
+    WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
+    double w = b.hi[0] - b.lo[0];
+    wc[k].w2 = (T)(w * w);
+    wc[k].skip = skip;
+    wc[k].start = start;
+    wc[k].end = end;
+    d.parent[k] = parent;
+    if (parent >= 0) atomicAdd(&d.cnt[parent], 1u);
+    if (d.export_bounds) {
+        double *o = d.cbounds + (size_t)k * 2 * D;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            o[a] = b.lo[a];
+            o[D + a] = b.hi[a];
+        }
+    }
+}
+
+// Writes group g's cells in preorder (preorder.rs:14-99 layout: skip, start, end per cell).
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
+    const uint32_t G = st->n_groups;
+    if (g >= G) return;
+    constexpr int L = KeyFmt<D>::LEVELS;
+    const uint32_t C = st->n_cells, nf = st->n_finite;
+    const uint64_t *gkey = d.gkey;
+    uint64_t key = gkey[g];
+    int left = g > 0 ? key_lcp<D>(gkey[g - 1], key) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, gkey[g + 1]) : -1;
+    uint32_t info = d.ginfo[g], type = info & 0xFF;
+    int level = (int)(info >> 8);
+    int last_internal = type == G_SIMPLE ? level - 1 : (type == G_STALL ? level - 1 : L - 1);
+    uint32_t start = d.gstart[g];
+    uint32_t gend = g + 1 < G ? d.gstart[g + 1] : nf;
+    uint32_t gskip = g + 1 < G ? d.gbase[g + 1] : C;
+    uint32_t idx = d.gbase[g];
+    int32_t prev = -1;
+    if (left >= 0) {
+        uint32_t f = gallop_bwd<D>(gkey, g, left);
+        int left_f = f > 0 ? key_lcp<D>(gkey[f - 1], gkey[f]) : -1;
+        prev = (int32_t)(d.gbase[f] + (uint32_t)(left - left_f - 1));
+    }
+    Box<D> b = root_box<D>(st);
+    int top = type == G_DEEP ? L - 1 : level;
+    for (int l = 0; l <= top; l++) {
+        if (l > 0) box_quad<D>(b, key_slot<D>(key, l));
+        if (l <= left) continue;
+        if (l <= last_internal) {
+            uint32_t skip = gskip, end = gend;
+            if (l <= right) {
+                uint32_t h = gallop_fwd<D>(gkey, G, g, l);
+                skip = h + 1 < G ? d.gbase[h + 1] : C;
+                end = h + 1 < G ? d.gstart[h + 1] : nf;
+            }
+            emit_cell<T, D>(d, idx, b, skip, start, end, prev);
+        } else {
+            emit_cell<T, D>(d, idx, b, idx + 1, start, gend, prev);
+        }
+        prev = (int32_t)idx;
+        idx++;
+    }
+    if (type == G_DEEP) {
+        uint32_t lc = d.glocal[g];
+        uint32_t lcount = (g + 1 < G ? d.gbase[g + 1] : C) - idx;
+        uint32_t base = idx;
+        for (uint32_t c = 0; c < lcount; c++) {
+            const int32_t *ik = d.lc_int + (size_t)(lc + c) * 4;
+            const double *bk = d.lc_dbl + (size_t)(lc + c) * 2 * D;
+            Box<D> lb;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                lb.lo[a] = bk[a];
+                lb.hi[a] = bk[D + a];
+            }
+            int32_t par = ik[3] >= 0 ? (int32_t)(base + (uint32_t)ik[3]) : prev;
+            emit_cell<T, D>(d, base + c, lb, base + (uint32_t)ik[0], (uint32_t)ik[1],
+                            (uint32_t)ik[2], par);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ centre of mass
+
+template <typename T, int D>
+__device__ __forceinline__ void body_final(const FaDev &d, uint32_t k, const double *s) {
+    WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
+    double *r = d.raw + (size_t)k * (1 + D);
+#pragma unroll
+    for (int a = 0; a <= D; a++) r[a] = s[a];
+    wc[k].mass = (T)s[0];
+#pragma unroll
+    for (int a = 0; a < D; a++) wc[k].c[a] = (T)(s[1 + a] / s[0]);
+}
+
+// barnes_hut.rs:62-110. A leaf sums its chain; the last child to finish sums its parent's
+// children in slot order (k+1, then each child's skip), so every sum is in the reference's
+// order whichever lane gets there last. Sums are FP64 in every precision.
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_com(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k >= st->n_cells) return;
+    const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
+    uint32_t skip = wc[k].skip;
+    if (skip != k + 1) return;
+    double s[1 + D];
+#pragma unroll
+    for (int a = 0; a <= D; a++) s[a] = 0.0;
+    const uint32_t qend = wc[k].end < d.n ? wc[k].end : d.n;
+    for (uint32_t q = wc[k].start; q < qend; q++) {
+        double m = (double)((const T *)d.sm)[q];
+        s[0] += m;
+#pragma unroll
+        for (int a = 0; a < D; a++) s[1 + a] += m * (double)((const T *)d.sp[a])[q];
+    }
+    body_final<T, D>(d, k, s);
+    uint32_t cur = k;
+    for (;;) {
+        int32_t par = d.parent[cur];
+        if (par < 0 || (uint32_t)par >= cur) break;  // parents precede children
+        __threadfence();
+        uint32_t old = __hip_atomic_fetch_add(&d.cnt[par], (uint32_t)-1, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        if (old != 1) break;
+        __threadfence();
+#pragma unroll
+        for (int a = 0; a <= D; a++) s[a] = 0.0;
+        uint32_t pskip = __hip_atomic_load(&wc[par].skip, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t c = (uint32_t)par + 1;
+        // a cell has at most 2^D children and every skip points forward: the loop is
+        // bounded even on a malformed arena
+        for (int q = 0; q < (1 << D) && c < pskip; q++) {
+            const double *r = d.raw + (size_t)c * (1 + D);
+#pragma unroll
+            for (int a = 0; a <= D; a++)
+                s[a] += __hip_atomic_load(&r[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t nc = __hip_atomic_load(&wc[c].skip, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            c = nc > c ? nc : pskip;
+        }
+        body_final<T, D>(d, (uint32_t)par, s);
+        cur = (uint32_t)par;
+    }
+}
+
+// ------------------------------------------------------------------ walk
+
+// barnes_hut.rs:112-184 as one wave-uniform preorder scan per 64 consecutive tree
+// positions. Every lane keeps its own next cell (the reference's `k`); the wave visits
+// the smallest of them, so each cell is loaded once per wave (a scalar load) and each lane
+// still sees its own cells in preorder, accepting or opening them exactly as the reference
+// does. A lane parked on an accepted cell waits until the wave reaches that cell's skip.
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    const uint32_t n = d.n;
+    const uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (__ballot(p < n) == 0) return;
+    const bool live = p < n;
+    const uint32_t pc = live ? p : n - 1;
+    const WalkCell<T, D> *__restrict__ wc = (const WalkCell<T, D> *)d.wc;
+    const T *__restrict__ sm = (const T *)d.sm;
+    const uint32_t *__restrict__ order = d.order;
+    const T *sp[3] = {(const T *)d.sp[0], (const T *)d.sp[1],
+                      (const T *)(D == 3 ? d.sp[2] : d.sp[0])};
+    T xi[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) xi[a] = sp[a][pc];
+    const T mi = sm[pc];
+    const uint32_t i = order[pc];
+    const T theta2 = (T)d.theta2, kr = (T)d.scaling_ratio;
+    const uint32_t seed = d.seed, tick = st->iter_no;
+    const uint32_t ncells = st->n_cells;
+    T f[D], fc[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) f[a] = fc[a] = (T)0;
+    uint32_t next = live ? 0u : 0xFFFFFFFFu;
+    uint32_t k = 0;
+    while (k < ncells) {
+        k = gr_uniform(k);
+        const WalkCell<T, D> cell = wc[k];
+        const bool act = next == k;
+        bool open = false;
+        if (act) {
+            const bool holds = cell.start <= p && p < cell.end;
+            T dx[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) dx[a] = xi[a] - cell.c[a];
+            T d2 = dx[0] * dx[0] + dx[1] * dx[1];
+            if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
+            if (!holds && d2 > (T)0 && cell.w2 < theta2 * d2) {
+                const T ff = mi * cell.mass / d2 * kr;
+#pragma unroll
+                for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], dx[a] * ff);
+                next = cell.skip;
+            } else {
+                next = k + 1;
+                open = true;
+            }
+        }
+        const bool leaf = cell.skip == k + 1;
+        const bool pairs = open && leaf;
+        if (__ballot(pairs) != 0) {
+            T lf[D], lc[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) lf[a] = lc[a] = (T)0;
+            const uint32_t qend = cell.end < n ? cell.end : n;
+            for (uint32_t q = cell.start; q < qend; q++) {
+                const uint32_t j = order[q];
+                T xj[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) xj[a] = sp[a][q];
+                const T mj = sm[q];
+                if (pairs && j != i) {
+                    // repel_delta from the lower index, negated for the higher one
+                    // (barnes_hut.rs:170-180), the coincidence jiggle included.
+                    T dd[D];
+                    bool zero = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        dd[a] = i < j ? xi[a] - xj[a] : -(xj[a] - xi[a]);
+                        if (dd[a] != (T)0) zero = false;
+                    }
+                    if (zero) {
+#pragma unroll
+                        for (int a = 0; a < D; a++) {
+                            T jg = (T)d_jiggle(seed, tick, (uint32_t)a, i, j);
+                            dd[a] = i < j ? jg : -jg;
+                        }
+                    }
+                    T d2 = dd[0] * dd[0] + dd[1] * dd[1];
+                    if (D == 3) d2 = d2 + dd[D - 1] * dd[D - 1];
+                    const T ff = mi * mj / d2 * kr;
+#pragma unroll
+                    for (int a = 0; a < D; a++) acc_add<T, KAHAN>(lf[a], lc[a], dd[a] * ff);
+                }
+            }
+            if (pairs) {
+#pragma unroll
+                for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], lf[a]);
+            }
+        }
+        // skip always points past k; the max keeps the scan finite on any arena
+        k = (__ballot(open) != 0 || cell.skip <= k) ? k + 1 : cell.skip;
+    }
+    if (live) {
+#pragma unroll
+        for (int a = 0; a < D; a++) ((T *)d.rep[a])[i] = f[a];
+    }
+}
+
+// ------------------------------------------------------------------ forces and step
+
+// state.rs:152-162 as a gather: node v sums (p[nbr] - p[v]) over its CSR row, which lists
+// its simple edges in ascending id, the order the reference's scatter reaches v in.
+template <typename T, int D, bool KAHAN>
+__device__ __forceinline__ void attraction_of(const FaDev &d, uint32_t v, const T *pv, T *acc) {
+    T c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) acc[a] = c[a] = (T)0;
+    const uint32_t e0 = d.row_off[v], e1 = d.row_off[v + 1];
+    const T *w = (const T *)d.w;
+    for (uint32_t e = e0; e < e1; e++) {
+        uint32_t j = d.nbr[e];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            T dd = ((const T *)d.pos[a])[j] - pv[a];
+            if (w) dd = dd * w[e];
+            acc_add<T, KAHAN>(acc[a], c[a], dd);
+        }
+    }
+}
+
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK) void k_attract_only(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (v >= d.n) return;
+    T pv[D], acc[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) pv[a] = ((const T *)d.pos[a])[v];
+    attraction_of<T, D, KAHAN>(d, v, pv, acc);
+#pragma unroll
+    for (int a = 0; a < D; a++) ((T *)d.u[a])[v] = acc[a];
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_zero_u(const FaDev d) {
+    uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (v >= d.n) return;
+#pragma unroll
+    for (int a = 0; a < D; a++) ((T *)d.u[a])[v] = (T)0;
+}
+
+// state.rs:169-203 as a gather: node v adds every pair term in ascending partner order, the
+// order the reference's i<j double loop reaches v in, starting from v's attraction.
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK) void k_dense(const FaDev d, uint32_t with_attraction) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ T tp[D][GR_BLOCK];
+    __shared__ T tm[GR_BLOCK];
+    const uint32_t n = d.n;
+    const uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x;
+    const bool live = v < n;
+    const uint32_t vc = live ? v : 0;
+    T pv[D], acc[D], c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        pv[a] = ((const T *)d.pos[a])[vc];
+        acc[a] = with_attraction ? ((const T *)d.u[a])[vc] : (T)0;
+        c[a] = (T)0;
+    }
+    const T mv = ((const T *)d.mass)[vc];
+    const T kr = (T)d.scaling_ratio;
+    const uint32_t seed = d.seed, tick = st->iter_no;
+    for (uint32_t j0 = 0; j0 < n; j0 += GR_BLOCK) {
+        __syncthreads();
+        uint32_t jj = j0 + threadIdx.x;
+        if (jj < n) {
+#pragma unroll
+            for (int a = 0; a < D; a++) tp[a][threadIdx.x] = ((const T *)d.pos[a])[jj];
+            tm[threadIdx.x] = ((const T *)d.mass)[jj];
+        }
+        __syncthreads();
+        uint32_t cnt = n - j0 < GR_BLOCK ? n - j0 : GR_BLOCK;
+        if (!live) continue;
+        for (uint32_t t = 0; t < cnt; t++) {
+            uint32_t j = j0 + t;
+            if (j == v) continue;
+            T dd[D];
+            bool zero = true;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                dd[a] = v < j ? pv[a] - tp[a][t] : -(tp[a][t] - pv[a]);
+                if (dd[a] != (T)0) zero = false;
+            }
+            if (zero) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    T jg = (T)d_jiggle(seed, tick, (uint32_t)a, v, j);
+                    dd[a] = v < j ? jg : -jg;
+                }
+            }
+            T d2 = (T)0;
+#pragma unroll
+            for (int a = 0; a < D; a++) d2 = d2 + dd[a] * dd[a];
+            const T ff = mv * tm[t] / d2 * kr;
+#pragma unroll
+            for (int a = 0; a < D; a++) acc_add<T, KAHAN>(acc[a], c[a], dd[a] * ff);
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int a = 0; a < D; a++) ((T *)d.u[a])[v] = acc[a];
+    }
+}
+
+// Completes u (attraction + repulsion, then gravity, state.rs:130-148 order) and the
+// swing/traction partials (schedule.rs:16-30). mode 0: repulsion from the walk (u = attr +
+// rep); mode 1: u already holds attraction (+ dense pairs).
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mode,
+                                                      uint32_t parts) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    double sw = 0.0, tr = 0.0;
+    const double grav = d.gravity;
+    for (uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x; v < d.n; v += gridDim.x * GR_BLOCK) {
+        T pv[D], uv[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) pv[a] = ((const T *)d.pos[a])[v];
+        if (mode == 0) {
+            if (parts & PART_ATTRACTION) {
+                attraction_of<T, D, KAHAN>(d, v, pv, uv);
+            } else {
+#pragma unroll
+                for (int a = 0; a < D; a++) uv[a] = (T)0;
+            }
+            if (parts & PART_REPULSION) {
+#pragma unroll
+                for (int a = 0; a < D; a++) uv[a] = uv[a] + ((const T *)d.rep[a])[v];
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < D; a++) uv[a] = ((const T *)d.u[a])[v];
+        }
+        const double mv = (double)((const T *)d.mass)[v];
+        if (parts & PART_GRAVITY) {
+            double cen[D], nn = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                cen[a] = (double)pv[a] - st->mean[a];
+                nn += cen[a] * cen[a];
+            }
+            double norm = sqrt(nn);
+            double g = grav * mv;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                double unit = norm > 0.0 ? cen[a] / norm : 0.0;
+                uv[a] = (T)((double)uv[a] - g * unit);
+            }
+        }
+        double nb = 0.0, nf = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            ((T *)d.u[a])[v] = uv[a];
+            double back = (double)pv[a] - (double)uv[a];
+            double fwd = (double)pv[a] + (double)uv[a];
+            nb += back * back;
+            nf += fwd * fwd;
+        }
+        sw += mv * sqrt(nb);
+        tr += 0.5 * mv * sqrt(nf);
+    }
+    __shared__ double lds[4 * 2];
+    double v2[2] = {sw, tr};
+    block_reduce<2, 0>(v2, lds);
+    if (threadIdx.x == 0) {
+        d.spart[2 * blockIdx.x] = v2[0];
+        d.spart[2 * blockIdx.x + 1] = v2[1];
+    }
+}
+
+// schedule.rs:33-60 on the device, after the swing/traction sums are reduced.
+__global__ __launch_bounds__(GR_BLOCK) void k_estimate(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ double lds[4 * 2];
+    double v[2] = {0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) {
+        v[0] = v[0] + d.spart[2 * b];
+        v[1] = v[1] + d.spart[2 * b + 1];
+    }
+    block_reduce<2, 0>(v, lds);
+    if (threadIdx.x != 0) return;
+    st->swing += v[0];
+    st->traction += v[1];
+    const double swing = st->swing, traction = st->traction;
+    const double n = (double)d.n;
+    const double jt = d.jitter_tolerance;
+    const double opt_jitter = 0.05 * sqrt(n);
+    const double min_jitter = sqrt(opt_jitter);
+    const double min_speed_efficiency = 0.05;
+    const double other = fmin(10.0, opt_jitter * traction / (n * n));
+    double jitter = jt * fmax(min_jitter, other);
+    double se = st->speed_efficiency;
+    if (swing / traction > 2.0) {
+        if (se > min_speed_efficiency) se *= 0.5;
+        jitter = fmax(jitter, jt);
+    }
+    const double target_speed = swing == 0.0 ? INFINITY : jitter * se * traction / swing;
+    if (swing > jitter * traction) {
+        if (se > min_speed_efficiency) se *= 0.7;
+    } else if (st->speed < 1000.0) {
+        se *= 1.3;
+    }
+    st->speed_efficiency = se;
+    st->speed += fmin(target_speed - st->speed, 0.5 * st->speed);
+    st->iter_no += 1;
+}
+
+// schedule.rs:62-77, fused with the next iteration's bounds/centroid partials.
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    const double speed = st->speed;
+    double moved = 0.0;
+    double lo[D], hi[D], sum[D], nfin = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+        sum[a] = 0.0;
+    }
+    for (uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x; v < d.n; v += gridDim.x * GR_BLOCK) {
+        double uv[D], nn = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            uv[a] = (double)((const T *)d.u[a])[v];
+            nn += uv[a] * uv[a];
+        }
+        const double norm = sqrt(nn);
+        const double factor =
+            speed / (1.0 + sqrt(speed * (double)((const T *)d.mass)[v] * norm));
+        double step = 0.0, p[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            const double dd = uv[a] * factor;
+            T *P = (T *)d.pos[a];
+            T np = (T)((double)P[v] + dd);
+            P[v] = np;
+            p[a] = (double)np;
+            step += fabs(dd);
+        }
+        moved += step;
+        bounds_take<D>(p, lo, hi, sum, nfin);
+    }
+    bounds_write<D>(lo, hi, sum, nfin, d.bpart);
+    __shared__ double lds[4];
+    double mv[1] = {moved};
+    block_reduce<1, 0>(mv, lds);
+    if (threadIdx.x == 0) d.mpart[blockIdx.x] = mv[0];
+}
+
+// state.rs:122-128: the iteration counts, and one that moved less than 1e-10 ends the run.
+__global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ double lds[4];
+    double v[1] = {0.0};
+    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) v[0] = v[0] + d.mpart[b];
+    block_reduce<1, 0>(v, lds);
+    if (threadIdx.x == 0) {
+        st->moved = v[0];
+        st->iters_done += 1;
+        if (v[0] < 1e-10) st->done = 1;
+    }
+}
+
+__global__ void k_clear_halt(const FaDev d) {
+    d.st->halt = 0;
+    d.st->pool_overflow = 0;
+}
+
+__global__ void k_tree_reset(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    st->pool_used = 0;
+    st->lcell_used = 0;
+    st->any_deep = 0;
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <typename T, int D>
+static void tree_impl(FaDev &d) {
+    const uint32_t n = d.n;
+    const uint32_t nbl = gr_blocks(n, GR_BLOCK);
+    hipStream_t s = d.stream;
+    hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_tree_reset, dim3(1), dim3(1), 0, s, d);
+    hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, n, 64, d.sort_tmp, s);
+    hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    scan_exclusive(d.flag, d.gid, n, nullptr, &d.st->n_groups, d.scan_tmp, s);
+    hipLaunchKernelGGL((k_group_compact<D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_group_info<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    scan_exclusive(d.gcells, d.gbase, n, &d.st->n_groups, &d.st->need_cells, d.scan_tmp, s);
+    hipLaunchKernelGGL(k_cells_ready, dim3(1), dim3(1), 0, s, d);
+    uint32_t cbl = gr_blocks(d.cap, GR_BLOCK);
+    hipLaunchKernelGGL(k_zero_cnt, dim3(cbl), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_com<T, D>), dim3(cbl), dim3(GR_BLOCK), 0, s, d);
+}
+
+template <typename T, int D, bool KAHAN>
+static void walk_impl(FaDev &d) {
+    hipLaunchKernelGGL((k_walk<T, D, KAHAN>), dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       d.stream, d);
+}
+
+void pm_repulsion(FaDev &d);  // pm.hip
+
+template <typename T, int D, bool KAHAN>
+static void forces_impl(FaDev &d, uint32_t parts) {
+    const uint32_t nbl = gr_blocks(d.n, GR_BLOCK);
+    hipStream_t s = d.stream;
+    uint32_t mode = 0;
+    if (d.repulsion == REP_TREE) {
+        tree_impl<T, D>(d);
+        if (parts & PART_REPULSION) walk_impl<T, D, KAHAN>(d);
+    } else if (d.repulsion == REP_DENSE) {
+        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        if (parts & PART_ATTRACTION)
+            hipLaunchKernelGGL((k_attract_only<T, D, KAHAN>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+        if (parts & PART_REPULSION) {
+            hipLaunchKernelGGL((k_dense<T, D, KAHAN>), dim3(nbl), dim3(GR_BLOCK), 0, s, d,
+                               (uint32_t)((parts & PART_ATTRACTION) ? 1 : 0));
+        } else if (!(parts & PART_ATTRACTION)) {
+            hipLaunchKernelGGL((k_zero_u<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+        }
+        mode = 1;
+    } else if (d.repulsion == REP_PM) {
+        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        if (parts & PART_REPULSION) pm_repulsion(d);
+    } else {
+        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        parts &= ~(uint32_t)PART_REPULSION;
+    }
+    hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d, mode,
+                       parts);
+}
+
+template <typename T, int D, bool KAHAN>
+static void iteration_impl(FaDev &d) {
+    hipStream_t s = d.stream;
+    forces_impl<T, D, KAHAN>(d, PART_ATTRACTION | PART_REPULSION | PART_GRAVITY);
+    hipLaunchKernelGGL(k_estimate, dim3(1), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_apply<T, D>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(GR_BLOCK), 0, s, d);
+}
+
+#define GR_DISPATCH(FN, ...)                                                                   \
+    do {                                                                                       \
+        if (d.precision == PREC_FP64) {                                                        \
+            if (d.dim == 2)                                                                    \
+                FN<double, 2, false>(__VA_ARGS__);                                             \
+            else                                                                               \
+                FN<double, 3, false>(__VA_ARGS__);                                             \
+        } else if (d.precision == PREC_FP32) {                                                 \
+            if (d.dim == 2)                                                                    \
+                FN<float, 2, false>(__VA_ARGS__);                                              \
+            else                                                                               \
+                FN<float, 3, false>(__VA_ARGS__);                                              \
+        } else {                                                                               \
+            if (d.dim == 2)                                                                    \
+                FN<float, 2, true>(__VA_ARGS__);                                               \
+            else                                                                               \
+                FN<float, 3, true>(__VA_ARGS__);                                               \
+        }                                                                                      \
+    } while (0)
+
+template <typename T, int D, bool KAHAN>
+static void bounds_impl(FaDev &d) {
+    hipLaunchKernelGGL((k_bounds<T, D>), dim3(d.nparts), dim3(GR_BLOCK), 0, d.stream, d);
+}
+
+template <typename T, int D, bool KAHAN>
+static void tree_only_impl(FaDev &d) {
+    tree_impl<T, D>(d);
+}
+
+void launch_bounds(FaDev &d) { GR_DISPATCH(bounds_impl, d); }
+void launch_iteration(FaDev &d) { GR_DISPATCH(iteration_impl, d); }
+void launch_forces(FaDev &dd, uint32_t parts) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(forces_impl, d, parts);
+}
+void launch_tree(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(tree_only_impl, d);
+}
+void launch_walk(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(walk_impl, d);
+}
+void launch_clear_halt(FaDev &d) {
+    hipLaunchKernelGGL(k_clear_halt, dim3(1), dim3(1), 0, d.stream, d);
+}
+
+}  // namespace gr

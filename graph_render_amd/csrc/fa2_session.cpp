@@ -1,0 +1,754 @@
+// Host side of the C-ABI (include/graph_render_fa2.h): parameter checks, the simple graph
+// and its CSR, start positions, device residency of one Fa2State and the iteration loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/graph_render_fa2.h"
+#include "fa2_kernels.h"
+#include "sort_scan.h"
+
+namespace {
+
+struct LastError {
+    gr_status code = GR_OK;
+    std::string name, rule;
+};
+thread_local LastError g_err;
+
+gr_status fail(gr_status code, const char *name, const char *rule) {
+    g_err.code = code;
+    g_err.name = name ? name : "";
+    g_err.rule = rule ? rule : "";
+    return code;
+}
+
+gr_status ok() {
+    g_err = LastError();
+    return GR_OK;
+}
+
+#define HIP_TRY(x)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(GR_ERR_DEVICE, #x, hipGetErrorString(e_));       \
+    } while (0)
+
+// synthetic.rs:44-50 (Mulberry32), the generator of the reference's start positions.
+double mulberry32_next(uint32_t &st) {
+    st += 0x6D2B79F5u;
+    uint32_t a = st;
+    uint32_t t = (a ^ (a >> 15)) * (1u | a);
+    t = (t + ((t ^ (t >> 7)) * (61u | t))) ^ t;
+    return (double)(t ^ (t >> 14)) / 4294967296.0;
+}
+
+uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+struct SimpleGraph {
+    std::vector<uint32_t> lo, hi;
+    std::vector<double> w;
+    std::vector<uint32_t> row_off, nbr;
+    std::vector<double> roww;
+};
+
+// layout/force/mod.rs:101-120 and :177-180: self-loops and repeated unordered pairs
+// dropped (first raw edge kept), then each node's row lists its simple edges in ascending
+// id (the stable counting sort of csr.rs:57-95), stored as neighbour ids.
+void build_simple(uint32_t n, const uint32_t *src, const uint32_t *dst, size_t m,
+                  const double *weights, SimpleGraph &g) {
+    size_t cap = 16;
+    while (cap < 2 * m + 2) cap <<= 1;
+    std::vector<uint64_t> table(cap, ~0ull);
+    g.lo.reserve(m);
+    g.hi.reserve(m);
+    if (weights) g.w.reserve(m);
+    for (size_t e = 0; e < m; e++) {
+        uint32_t a = src[e], b = dst[e];
+        if (a == b) continue;
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        uint64_t key = ((uint64_t)lo << 32) | hi;
+        size_t h = (size_t)(mix64(key) & (cap - 1));
+        bool dup = false;
+        while (table[h] != ~0ull) {
+            if (table[h] == key) {
+                dup = true;
+                break;
+            }
+            h = (h + 1) & (cap - 1);
+        }
+        if (dup) continue;
+        table[h] = key;
+        g.lo.push_back(lo);
+        g.hi.push_back(hi);
+        if (weights) g.w.push_back(weights[e]);
+    }
+    size_t ms = g.lo.size();
+    g.row_off.assign((size_t)n + 1, 0);
+    for (size_t e = 0; e < ms; e++) {
+        g.row_off[g.lo[e] + 1]++;
+        g.row_off[g.hi[e] + 1]++;
+    }
+    for (size_t v = 0; v < n; v++) g.row_off[v + 1] += g.row_off[v];
+    std::vector<uint32_t> cur(g.row_off.begin(), g.row_off.end() - 1);
+    g.nbr.resize(2 * ms);
+    if (weights) g.roww.resize(2 * ms);
+    for (size_t e = 0; e < ms; e++) {
+        uint32_t a = g.lo[e], b = g.hi[e];
+        uint32_t ia = cur[a]++, ib = cur[b]++;
+        g.nbr[ia] = b;
+        g.nbr[ib] = a;
+        if (weights) {
+            g.roww[ia] = g.w[e];
+            g.roww[ib] = g.w[e];
+        }
+    }
+}
+
+}  // namespace
+
+struct gr_fa2_session {
+    gr::FaDev d;
+    gr_fa2_params params{};
+    uint32_t m_simple = 0;
+    std::vector<uint32_t> lo, hi;
+    std::vector<double> mass;
+    std::vector<void *> allocs;
+    int device = 0;
+    bool tree_valid = false;
+
+    ~gr_fa2_session() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (d.stream) (void)hipStreamDestroy(d.stream);
+    }
+
+    template <typename P>
+    gr_status alloc(P **out, size_t bytes) {
+        void *p = nullptr;
+        if (bytes == 0) bytes = 8;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail(GR_ERR_DEVICE, "hipMalloc", hipGetErrorString(e));
+        allocs.push_back(p);
+        *out = (P *)p;
+        return GR_OK;
+    }
+
+    size_t esize() const { return d.precision == gr::PREC_FP64 ? 8 : 4; }
+
+    gr_status alloc_cells(uint32_t cap) {
+        d.cap = cap;
+        gr_status s;
+        if ((s = alloc(&d.wc, (size_t)cap * gr::walkcell_bytes(d.precision, d.dim)))) return s;
+        if ((s = alloc(&d.parent, (size_t)cap * 4))) return s;
+        if ((s = alloc(&d.cnt, (size_t)cap * 4))) return s;
+        if ((s = alloc(&d.raw, (size_t)cap * (1 + d.dim) * 8))) return s;
+        if (d.export_bounds && (s = alloc(&d.cbounds, (size_t)cap * 2 * d.dim * 8))) return s;
+        return GR_OK;
+    }
+
+    gr_status upload_positions(const double *rowmajor) {
+        const uint32_t n = d.n, dim = d.dim;
+        for (uint32_t a = 0; a < dim; a++) {
+            if (d.precision == gr::PREC_FP64) {
+                std::vector<double> col(n);
+                for (uint32_t i = 0; i < n; i++) col[i] = rowmajor[(size_t)i * dim + a];
+                HIP_TRY(hipMemcpy(d.pos[a], col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> col(n);
+                for (uint32_t i = 0; i < n; i++) col[i] = (float)rowmajor[(size_t)i * dim + a];
+                HIP_TRY(hipMemcpy(d.pos[a], col.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+            }
+        }
+        gr::launch_bounds(d);
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        tree_valid = false;
+        return GR_OK;
+    }
+
+    gr_status read_state(GrState &st) {
+        HIP_TRY(hipMemcpyAsync(&st, d.st, sizeof(GrState), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        return GR_OK;
+    }
+
+    // After a halted iteration: grow what ran out, clear the flag. The halted iteration
+    // applied nothing, so it is simply run again.
+    gr_status recover(const GrState &st) {
+        if (st.pool_overflow) {
+            uint32_t grow = d.pool_cap * 2;
+            void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+            HIP_TRY(hipMalloc(&p1, (size_t)grow * 4));
+            HIP_TRY(hipMalloc(&p2, (size_t)d.lcell_cap * 2 * 16));
+            HIP_TRY(hipMalloc(&p3, (size_t)d.lcell_cap * 2 * 2 * d.dim * 8));
+            allocs.push_back(p1);
+            allocs.push_back(p2);
+            allocs.push_back(p3);
+            d.pool = (int32_t *)p1;
+            d.lc_int = (int32_t *)p2;
+            d.lc_dbl = (double *)p3;
+            d.pool_cap = grow;
+            d.lcell_cap *= 2;
+        }
+        if (st.need_cells > d.cap) {
+            uint32_t want = st.need_cells + st.need_cells / 4 + 64;
+            gr_status s = alloc_cells(want);
+            if (s) return s;
+        }
+        if (!st.pool_overflow && st.need_cells <= d.cap)
+            return fail(GR_ERR_DEVICE, "halt", "iteration halted without a capacity cause");
+        gr::launch_clear_halt(d);
+        return GR_OK;
+    }
+};
+
+extern "C" {
+
+void gr_fa2_params_default(gr_fa2_params *out) {
+    out->max_iter = 100;
+    out->jitter_tolerance = 1.0;
+    out->scaling_ratio = 2.0;
+    out->gravity = 1.0;
+    out->seed = 0;
+    out->dim = 2;
+}
+
+void gr_fa2_ext_default(gr_fa2_ext *out) {
+    std::memset(out, 0, sizeof(*out));
+    out->struct_size = sizeof(gr_fa2_ext);
+    out->theta = -1.0;
+    out->precision = GR_PREC_FP64;
+    out->repulsion = GR_REP_DEFAULT;
+}
+
+// registry/params.rs:144-202 with graph-contract params.rs:147-196's validation.
+gr_status gr_fa2_params_decode(const uint8_t *bytes, size_t len, gr_fa2_params *out) {
+    gr_fa2_params_default(out);
+    if (len == 0) return ok();
+    struct Spec {
+        const char *name;
+        bool integer;
+        double min, max;
+    };
+    static const Spec specs[5] = {{"max_iter", true, 1.0, 10000.0},
+                                  {"jitter_tolerance", false, 0.0, 1e6},
+                                  {"scaling_ratio", false, 0.0, 1e6},
+                                  {"gravity", false, 0.0, 1e6},
+                                  {"seed", true, 0.0, 4294967295.0}};
+    if (!bytes || len != 5 * sizeof(double))
+        return fail(GR_ERR_PARAMS_MALFORMED, "params",
+                    "one little-endian f64 per published parameter, and no other bytes");
+    double v[5];
+    for (int i = 0; i < 5; i++) {
+        uint64_t bits = 0;
+        for (int b = 0; b < 8; b++) bits |= (uint64_t)bytes[i * 8 + b] << (8 * b);
+        std::memcpy(&v[i], &bits, 8);
+        const Spec &s = specs[i];
+        if (!std::isfinite(v[i]) || v[i] < s.min || v[i] > s.max ||
+            (s.integer && v[i] != std::floor(v[i])))
+            return fail(GR_ERR_PARAM_OUT_OF_RANGE, s.name,
+                        "inside the range the schema publishes, never clamped");
+    }
+    out->max_iter = (uint32_t)v[0];
+    out->jitter_tolerance = v[1];
+    out->scaling_ratio = v[2];
+    out->gravity = v[3];
+    out->seed = (uint32_t)v[4];
+    return ok();
+}
+
+gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
+                                      const uint32_t *target, size_t m,
+                                      const gr_fa2_params *params, const gr_fa2_ext *ext) {
+    gr_fa2_params p;
+    if (params)
+        p = *params;
+    else
+        gr_fa2_params_default(&p);
+    gr_fa2_ext x;
+    gr_fa2_ext_default(&x);
+    if (ext) {
+        if (ext->struct_size != sizeof(gr_fa2_ext)) {
+            fail(GR_ERR_PARAM, "ext", "struct_size must equal sizeof(gr_fa2_ext)");
+            return nullptr;
+        }
+        x = *ext;
+    }
+    if (p.dim != 2 && p.dim != 3) {  // layout/forceatlas2.rs:70-76
+        fail(GR_ERR_PARAM, "dim", "2 or 3 coordinates per node");
+        return nullptr;
+    }
+    if (m > 0xFFFFFFFFull / 2) {
+        fail(GR_ERR_CAPACITY, "edges", "edge count fits the u32 index space");
+        return nullptr;
+    }
+    if (m > 0 && (!source || !target)) {
+        fail(GR_ERR_PARAM, "edges", "source and target columns are required");
+        return nullptr;
+    }
+    for (size_t e = 0; e < m; e++)
+        if (source[e] >= n || target[e] >= n) {
+            fail(GR_ERR_PARAM, "edges", "every endpoint is a node index below n");
+            return nullptr;
+        }
+    if (x.precision < GR_PREC_FP64 || x.precision > GR_PREC_FP32_KAHAN) {
+        fail(GR_ERR_PARAM, "precision", "GR_PREC_FP64, GR_PREC_FP32 or GR_PREC_FP32_KAHAN");
+        return nullptr;
+    }
+    int rep = x.repulsion == GR_REP_DEFAULT ? GR_REP_TREE : x.repulsion;
+    if (rep < GR_REP_TREE || rep > GR_REP_PM) {
+        fail(GR_ERR_PARAM, "repulsion", "GR_REP_DEFAULT, TREE, DENSE or PM");
+        return nullptr;
+    }
+    double theta = x.theta < 0.0 ? 0.8 : x.theta;
+    if (!std::isfinite(theta)) {
+        fail(GR_ERR_PARAM, "theta", "a finite opening angle");
+        return nullptr;
+    }
+    if (x.edge_weights)
+        for (size_t e = 0; e < m; e++)
+            if (!std::isfinite(x.edge_weights[e])) {
+                fail(GR_ERR_NON_FINITE, "edge.weight", "finite edge weights");
+                return nullptr;
+            }
+    if (rep == GR_REP_PM) {
+        uint32_t side = x.pm_grid ? x.pm_grid : 2048;
+        if (side < 64 || side > 4096 || (side & (side - 1))) {
+            fail(GR_ERR_PARAM, "pm_grid", "a power of two from 64 to 4096");
+            return nullptr;
+        }
+        if (p.dim != 2) {
+            fail(GR_ERR_UNSUPPORTED, "pm_grid", "the particle mesh runs in 2D");
+            return nullptr;
+        }
+        x.pm_grid = side;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        fail(GR_ERR_DEVICE, "device", "no HIP device is visible");
+        return nullptr;
+    }
+
+    auto *s = new gr_fa2_session();
+    s->params = p;
+    gr::FaDev &d = s->d;
+    d.n = n;
+    d.dim = p.dim;
+    d.precision = x.precision;
+    d.repulsion = rep == GR_REP_TREE ? gr::REP_TREE : rep == GR_REP_DENSE ? gr::REP_DENSE
+                                                                           : gr::REP_PM;
+    d.theta2 = theta * theta;
+    d.scaling_ratio = p.scaling_ratio;
+    d.gravity = p.gravity;
+    d.jitter_tolerance = p.jitter_tolerance;
+    d.seed = p.seed;
+    d.export_bounds = (x.flags & GR_EXT_EXPORT_TREE) ? 1 : 0;
+    d.pm_side = x.pm_grid;
+    (void)hipGetDevice(&s->device);
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) {
+        fail(GR_ERR_DEVICE, "hipStreamCreate", "stream creation failed");
+        delete s;
+        return nullptr;
+    }
+
+    SimpleGraph g;
+    build_simple(n, source, target, m, x.edge_weights, g);
+    s->m_simple = (uint32_t)g.lo.size();
+    s->lo = g.lo;
+    s->hi = g.hi;
+    s->mass.resize(n);
+    for (uint32_t v = 0; v < n; v++) s->mass[v] = (double)(g.row_off[v + 1] - g.row_off[v]) + 1.0;
+    d.m2 = (uint32_t)g.nbr.size();
+    d.weighted = x.edge_weights ? 1 : 0;
+
+    const size_t es = s->esize();
+    const size_t nn = n ? n : 1;
+    gr_status st;
+#define TRY_ALLOC(ptr, bytes)                      \
+    if ((st = s->alloc(&(ptr), (bytes))) != GR_OK) { \
+        delete s;                                  \
+        return nullptr;                            \
+    }
+    for (uint32_t a = 0; a < d.dim; a++) {
+        TRY_ALLOC(d.pos[a], nn * es);
+        TRY_ALLOC(d.u[a], nn * es);
+        TRY_ALLOC(d.rep[a], nn * es);
+        TRY_ALLOC(d.sp[a], nn * es);
+    }
+    TRY_ALLOC(d.mass, nn * es);
+    TRY_ALLOC(d.sm, nn * es);
+    TRY_ALLOC(d.row_off, (nn + 1) * 4);
+    TRY_ALLOC(d.nbr, (g.nbr.size() + 1) * 4);
+    if (d.weighted) TRY_ALLOC(d.w, (g.nbr.size() + 1) * es);
+    TRY_ALLOC(d.st, sizeof(GrState));
+    TRY_ALLOC(d.keys, nn * 8);
+    TRY_ALLOC(d.keys_alt, nn * 8);
+    TRY_ALLOC(d.vals, nn * 4);
+    TRY_ALLOC(d.vals_alt, nn * 4);
+    TRY_ALLOC(d.stall, nn);
+    TRY_ALLOC(d.order, nn * 4);
+    TRY_ALLOC(d.flag, nn * 4);
+    TRY_ALLOC(d.gid, nn * 4);
+    TRY_ALLOC(d.dflag, nn * 4);
+    TRY_ALLOC(d.gstart, nn * 4);
+    TRY_ALLOC(d.gcells, nn * 4);
+    TRY_ALLOC(d.gbase, nn * 4);
+    TRY_ALLOC(d.ginfo, nn * 4);
+    TRY_ALLOC(d.glocal, nn * 4);
+    TRY_ALLOC(d.gkey, nn * 8);
+    TRY_ALLOC(d.chain_next, nn * 4);
+    d.pool_cap = 1u << 22;
+    d.lcell_cap = 1u << 16;
+    TRY_ALLOC(d.pool, (size_t)d.pool_cap * 4);
+    TRY_ALLOC(d.lc_int, (size_t)d.lcell_cap * 16);
+    TRY_ALLOC(d.lc_dbl, (size_t)d.lcell_cap * 2 * d.dim * 8);
+    d.nparts = gr_blocks(nn, GR_BLOCK);
+    if (d.nparts > 2048) d.nparts = 2048;
+    TRY_ALLOC(d.bpart, (size_t)d.nparts * 10 * 8);
+    TRY_ALLOC(d.spart, (size_t)d.nparts * 2 * 8);
+    TRY_ALLOC(d.mpart, (size_t)d.nparts * 8);
+    TRY_ALLOC(d.scan_tmp, gr::scan_tmp_bytes((uint32_t)nn));
+    TRY_ALLOC(d.sort_tmp, gr::radix_tmp_bytes((uint32_t)nn));
+#undef TRY_ALLOC
+    if (s->alloc_cells(2 * (uint32_t)nn + 64) != GR_OK) {
+        delete s;
+        return nullptr;
+    }
+
+    auto upload = [&](void *dst, const void *src, size_t bytes) -> bool {
+        if (bytes == 0) return true;
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool good = upload(d.row_off, g.row_off.data(), ((size_t)n + 1) * 4) &&
+                upload(d.nbr, g.nbr.data(), g.nbr.size() * 4);
+    if (d.precision == gr::PREC_FP64) {
+        good = good && upload(d.mass, s->mass.data(), (size_t)n * 8);
+        if (d.weighted) good = good && upload(d.w, g.roww.data(), g.roww.size() * 8);
+    } else {
+        std::vector<float> mf(s->mass.begin(), s->mass.end());
+        good = good && upload(d.mass, mf.data(), (size_t)n * 4);
+        if (d.weighted) {
+            std::vector<float> wf(g.roww.begin(), g.roww.end());
+            good = good && upload(d.w, wf.data(), wf.size() * 4);
+        }
+    }
+    GrState init;
+    std::memset(&init, 0, sizeof(init));
+    init.swing = init.traction = init.speed = init.speed_efficiency = 1.0;
+    good = good && upload(d.st, &init, sizeof(init));
+    if (!good) {
+        fail(GR_ERR_DEVICE, "hipMemcpy", "upload failed");
+        delete s;
+        return nullptr;
+    }
+    std::vector<double> start((size_t)n * d.dim);
+    if (x.start) {
+        std::memcpy(start.data(), x.start, start.size() * 8);
+    } else {  // state.rs:272-283
+        uint32_t rs = p.seed;
+        for (size_t q = 0; q < start.size(); q++) start[q] = mulberry32_next(rs);
+    }
+    if (n > 0 && s->upload_positions(start.data()) != GR_OK) {
+        delete s;
+        return nullptr;
+    }
+    ok();
+    return s;
+}
+
+void gr_fa2_session_release(gr_fa2_session *s) { delete s; }
+
+gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (done_out) *done_out = 0;
+    if (s->d.n == 0 || iters == 0) return ok();
+    GrState st;
+    gr_status r = s->read_state(st);
+    if (r) return r;
+    const uint32_t start = st.iters_done;
+    uint32_t remaining = iters;
+    const uint32_t batch = 32;
+    int recoveries = 0;
+    while (remaining > 0 && !st.done) {
+        uint32_t k = remaining < batch ? remaining : batch;
+        for (uint32_t q = 0; q < k; q++) gr::launch_iteration(s->d);
+        HIP_TRY(hipGetLastError());
+        if ((r = s->read_state(st))) return r;
+        if (st.halt) {
+            if (++recoveries > 8) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
+            if ((r = s->recover(st))) return r;
+            if ((r = s->read_state(st))) return r;
+        }
+        remaining = iters - (st.iters_done - start);
+    }
+    s->tree_valid = s->d.repulsion == gr::REP_TREE;
+    if (done_out) *done_out = st.iters_done - start;
+    return ok();
+}
+
+static gr_status read_columns(gr_fa2_session *s, void *const *cols, double *out) {
+    const uint32_t n = s->d.n, dim = s->d.dim;
+    for (uint32_t a = 0; a < dim; a++) {
+        if (s->d.precision == gr::PREC_FP64) {
+            std::vector<double> c(n);
+            HIP_TRY(hipMemcpy(c.data(), cols[a], (size_t)n * 8, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; i++) out[(size_t)i * dim + a] = c[i];
+        } else {
+            std::vector<float> c(n);
+            HIP_TRY(hipMemcpy(c.data(), cols[a], (size_t)n * 4, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; i++) out[(size_t)i * dim + a] = (double)c[i];
+        }
+    }
+    return GR_OK;
+}
+
+gr_status gr_fa2_session_positions(gr_fa2_session *s, double *out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
+    gr_status r = read_columns(s, s->d.pos, out);
+    return r ? r : ok();
+}
+
+gr_status gr_fa2_session_set_positions(gr_fa2_session *s, const double *in) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
+    if (s->d.n == 0) return ok();
+    gr_status r = s->upload_positions(in);
+    return r ? r : ok();
+}
+
+gr_status gr_fa2_session_forces(gr_fa2_session *s, uint32_t parts, double *out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (s->d.n == 0) return ok();
+    GrState st;
+    gr_status r;
+    for (int tries = 0; tries < 8; tries++) {
+        // the bounds partials of the current positions are always current: they are
+        // written at upload and by every applied step
+        gr::launch_forces(s->d, parts);
+        HIP_TRY(hipGetLastError());
+        if ((r = s->read_state(st))) return r;
+        if (!st.halt) break;
+        if ((r = s->recover(st))) return r;
+    }
+    if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
+    s->tree_valid = s->d.repulsion == gr::REP_TREE;
+    r = read_columns(s, s->d.u, out);
+    return r ? r : ok();
+}
+
+gr_status gr_fa2_session_scalars(gr_fa2_session *s, double *out7) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    GrState st;
+    gr_status r = s->read_state(st);
+    if (r) return r;
+    out7[0] = st.swing;
+    out7[1] = st.traction;
+    out7[2] = st.speed;
+    out7[3] = st.speed_efficiency;
+    out7[4] = st.iter_no;
+    out7[5] = st.iters_done;
+    out7[6] = st.done;
+    return ok();
+}
+
+gr_status gr_fa2_session_graph(gr_fa2_session *s, uint32_t *m_out, uint32_t *lo, uint32_t *hi,
+                               double *mass) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (m_out) *m_out = s->m_simple;
+    if (lo) std::memcpy(lo, s->lo.data(), s->lo.size() * 4);
+    if (hi) std::memcpy(hi, s->hi.data(), s->hi.size() * 4);
+    if (mass) std::memcpy(mass, s->mass.data(), s->mass.size() * 8);
+    return ok();
+}
+
+gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t cap,
+                              double *bounds, uint32_t *skip, uint32_t *start, uint32_t *end,
+                              double *bodies, uint32_t *order) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (s->d.repulsion != gr::REP_TREE || !s->tree_valid)
+        return fail(GR_ERR_UNSUPPORTED, "tree", "a tree session after forces or a step");
+    GrState st;
+    gr_status r = s->read_state(st);
+    if (r) return r;
+    const uint32_t C = st.n_cells, dim = s->d.dim;
+    if (ncells_out) *ncells_out = C;
+    if (order && s->d.n)
+        HIP_TRY(hipMemcpy(order, s->d.order, (size_t)s->d.n * 4, hipMemcpyDeviceToHost));
+    if (C == 0 || cap < C) return ok();
+    if (bounds) {
+        if (!s->d.export_bounds)
+            return fail(GR_ERR_UNSUPPORTED, "bounds", "create the session with GR_EXT_EXPORT_TREE");
+        HIP_TRY(hipMemcpy(bounds, s->d.cbounds, (size_t)C * 2 * dim * 8, hipMemcpyDeviceToHost));
+    }
+    if (skip || start || end) {
+        size_t wb = gr::walkcell_bytes(s->d.precision, dim);
+        std::vector<uint8_t> buf((size_t)C * wb);
+        HIP_TRY(hipMemcpy(buf.data(), s->d.wc, buf.size(), hipMemcpyDeviceToHost));
+        size_t es = s->esize();
+        size_t off = es * (dim + 2);  // c[dim], mass, w2, then skip, start, end
+        for (uint32_t k = 0; k < C; k++) {
+            const uint8_t *c = buf.data() + (size_t)k * wb + off;
+            uint32_t v[3];
+            std::memcpy(v, c, 12);
+            if (skip) skip[k] = v[0];
+            if (start) start[k] = v[1];
+            if (end) end[k] = v[2];
+        }
+    }
+    if (bodies) {
+        std::vector<double> raw((size_t)C * (1 + dim));
+        HIP_TRY(hipMemcpy(raw.data(), s->d.raw, raw.size() * 8, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < C; k++) {
+            double m = raw[(size_t)k * (1 + dim)];
+            bodies[(size_t)k * (1 + dim)] = m;
+            for (uint32_t a = 0; a < dim; a++)
+                bodies[(size_t)k * (1 + dim) + 1 + a] = raw[(size_t)k * (1 + dim) + 1 + a] / m;
+        }
+    }
+    return ok();
+}
+
+gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,
+                              double *ms_out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (!stage || reps == 0) return fail(GR_ERR_PARAM, "stage", "a stage name and reps > 0");
+    std::string name(stage);
+    bool walk = name == "walk", tree = name == "tree", it = name == "iteration",
+         pm = name == "pm";
+    if (!walk && !tree && !it && !pm)
+        return fail(GR_ERR_PARAM, "stage", "walk, tree, iteration or pm");
+    if ((walk || tree) && s->d.repulsion != gr::REP_TREE)
+        return fail(GR_ERR_UNSUPPORTED, "stage", "walk/tree timing needs a tree session");
+    if (pm && s->d.repulsion != gr::REP_PM)
+        return fail(GR_ERR_UNSUPPORTED, "stage", "pm timing needs a PM session");
+    if (walk && !s->tree_valid) {
+        double *tmp = (double *)std::malloc((size_t)s->d.n * s->d.dim * 8 + 8);
+        gr_status r = gr_fa2_session_forces(s, 7, tmp);
+        std::free(tmp);
+        if (r) return r;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->d.stream));
+    for (uint32_t q = 0; q < reps; q++) {
+        if (walk) gr::launch_walk(s->d);
+        if (tree) gr::launch_tree(s->d);
+        if (it) gr::launch_iteration(s->d);
+        if (pm) gr::launch_forces(s->d, 2);
+    }
+    HIP_TRY(hipEventRecord(e1, s->d.stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = (double)ms / reps;
+    GrState st;
+    gr_status r = s->read_state(st);
+    if (r) return r;
+    if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "halted while timing");
+    return ok();
+}
+
+gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *source,
+                        const uint32_t *target, size_t m, const uint8_t *params,
+                        size_t params_len, const gr_fa2_ext *ext, float *out_x, float *out_y,
+                        float *out_z) {
+    auto zero = [&]() {
+        if (out_x) std::memset(out_x, 0, (size_t)n * 4);
+        if (out_y) std::memset(out_y, 0, (size_t)n * 4);
+        if (out_z) std::memset(out_z, 0, (size_t)n * 4);
+    };
+    if (!layout_id) {
+        zero();
+        return fail(GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id");
+    }
+    std::string id(layout_id);
+    uint32_t dim;
+    int rep;
+    if (id == "layout.forceatlas2") {
+        dim = 2;
+        rep = GR_REP_DENSE;
+    } else if (id == "layout.forceatlas2.barnes_hut") {
+        dim = 2;
+        rep = GR_REP_TREE;
+    } else if (id == "layout.forceatlas2.3d") {
+        dim = 3;
+        rep = GR_REP_DENSE;
+    } else {
+        zero();
+        return fail(GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id");
+    }
+    gr_fa2_params p;
+    gr_status r = gr_fa2_params_decode(params, params_len, &p);
+    if (r) {
+        zero();
+        return r;
+    }
+    p.dim = dim;
+    gr_fa2_ext x;
+    gr_fa2_ext_default(&x);
+    if (ext) {
+        if (ext->struct_size != sizeof(gr_fa2_ext)) {
+            zero();
+            return fail(GR_ERR_PARAM, "ext", "struct_size must equal sizeof(gr_fa2_ext)");
+        }
+        x = *ext;
+    }
+    if (x.repulsion == GR_REP_DEFAULT) x.repulsion = rep;
+    if (!out_x || !out_y || (dim == 3 && !out_z)) {
+        zero();
+        return fail(GR_ERR_PARAM, "output", "one f32 column per coordinate");
+    }
+    std::vector<double> pos((size_t)n * dim);
+    if (n > 0) {
+        gr_fa2_session *s = gr_fa2_session_create(n, source, target, m, &p, &x);
+        if (!s) {
+            zero();
+            return g_err.code;
+        }
+        uint32_t done = 0;
+        r = gr_fa2_session_step(s, p.max_iter, &done);
+        if (!r) r = gr_fa2_session_positions(s, pos.data());
+        gr_fa2_session_release(s);
+        if (r) {
+            zero();
+            return r;
+        }
+    }
+    // layout/forceatlas2.rs:86-88: the check reads the first `dim` values of the flattened
+    // rows, i.e. node 0's coordinates.
+    for (uint32_t a = 0; a < dim && n > 0; a++)
+        if (!std::isfinite(pos[a])) {
+            zero();
+            return fail(GR_ERR_NON_FINITE, "node.x", "finite positions");
+        }
+    for (uint32_t i = 0; i < n; i++) {
+        out_x[i] = (float)pos[(size_t)i * dim];
+        out_y[i] = (float)pos[(size_t)i * dim + 1];
+        if (dim == 3) out_z[i] = (float)pos[(size_t)i * dim + 2];
+    }
+    return ok();
+}
+
+gr_status gr_last_error(const char **name, const char **rule) {
+    if (name) *name = g_err.name.c_str();
+    if (rule) *rule = g_err.rule.c_str();
+    return g_err.code;
+}
+
+const char *gr_version(void) { return "graph_render_fa2 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// Shared device-side definitions for the FA2 / Barnes-Hut HIP path (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GR_WAVE 64
+#define GR_BLOCK 256
+
+// Per-session scalars kept resident on the device: the adaptive-speed state of
+// layout/forceatlas2/state.rs:151-171 (swing, traction, speed, speed_efficiency, iter_no) plus
+// the bookkeeping a launch-ahead iteration loop needs (early-exit flag, overflow flags, the
+// tree sizes of the current iteration).
+struct GrState {
+    double swing, traction, speed, speed_efficiency;
+    double moved;         // last applied step's sum |dp| (schedule.rs:62-77)
+    double mean[3];       // centroid of the current positions (state.rs:208-220)
+    double root_lo[3], root_hi[3];  // root square of the current tree (quadtree.rs:157-195)
+    uint32_t iter_no;     // jiggle tick, incremented once per iteration
+    uint32_t iters_done;  // iterations applied by this session
+    uint32_t done;        // set once an iteration moved < 1e-10 (state.rs:122-128)
+    uint32_t halt;        // set when a capacity check fails: the iteration is not applied
+    uint32_t n_finite;    // points inside the tree (finite coordinates)
+    uint32_t n_groups;    // distinct-key groups of the current tree
+    uint32_t n_cells;     // cells of the current tree
+    uint32_t need_cells;  // cells the current iteration asked for (capacity check)
+    uint32_t pool_used;   // deep-group scratch used
+    uint32_t pool_overflow;
+    uint32_t any_deep;
+    uint32_t has_root;    // at least one finite point
+    uint32_t lcell_used;  // deep-group local cells used
+    uint32_t pad;
+};
+
+__device__ __forceinline__ uint64_t gr_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t gr_lane() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t gr_lanemask_lt() {
+    uint32_t l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+template <typename T>
+__device__ __forceinline__ T gr_shfl_xor(T v, int m) {
+    return __shfl_xor(v, m, GR_WAVE);
+}
+
+// Wave-wide sum in a fixed butterfly order: every lane ends with the same value, and the
+// association depends only on lane positions, so repeated runs give the same bits.
+__device__ __forceinline__ double gr_wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, GR_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t gr_wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, GR_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ double gr_wave_min(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, GR_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ double gr_wave_max(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, GR_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t gr_uniform(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+static inline uint32_t gr_blocks(uint64_t n, uint32_t per) {
+    uint64_t b = (n + per - 1) / per;
+    return b == 0 ? 1u : (uint32_t)b;
+}

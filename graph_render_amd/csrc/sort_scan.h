@@ -1,0 +1,20 @@
+// Host entry points of the hand-written scan and radix sort (sort_scan.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gr {
+
+// Exclusive scan of in[0, count) into out; count = *d_count (clamped to max_n) when d_count
+// is non-null, else max_n. The grand total goes to *d_total when non-null.
+size_t scan_tmp_bytes(uint32_t max_n);
+void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uint32_t *d_count,
+                    uint32_t *d_total, void *tmp, hipStream_t stream);
+
+// Stable LSD sort of n (key, value) pairs by key bits [0, end_bit).
+size_t radix_tmp_bytes(uint32_t n);
+void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                      uint32_t n, int end_bit, void *tmp, hipStream_t stream);
+
+}  // namespace gr

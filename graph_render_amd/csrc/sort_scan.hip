@@ -1,0 +1,244 @@
+// Device-wide exclusive scan (u32) and a stable LSD radix sort of (u64 key, u32 value)
+// pairs, both hand-written for gfx950: 256-thread blocks of four wave64s, LDS histograms,
+// ballot-based stable ranking inside each wave, no library calls.
+#include "gr_device.h"
+#include "sort_scan.h"
+
+namespace gr {
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = GR_BLOCK * SCAN_ITEMS;
+
+// Block-wide inclusive scan of one value per thread (256 threads), LDS-based.
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t *lds) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // wave inclusive scan with shuffles
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; w++) add += lds[w];
+    __syncthreads();
+    return v + add;
+}
+
+__device__ __forceinline__ uint32_t load_count(const uint32_t *d_count, uint32_t max_n) {
+    if (!d_count) return max_n;
+    uint32_t c = *d_count;
+    return c < max_n ? c : max_n;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t max_n,
+                                                          const uint32_t *d_count,
+                                                          uint32_t *partials) {
+    __shared__ uint32_t lds[8];
+    const uint32_t n = load_count(d_count, max_n);
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; r++) {
+        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    s = gr_wave_sum_u32(s);
+    if ((threadIdx.x & 63u) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// One block scans the per-tile partials (exclusive) and writes the grand total.
+__global__ __launch_bounds__(GR_BLOCK) void k_scan_partials(uint32_t *partials, uint32_t nparts,
+                                                            uint32_t *d_total) {
+    __shared__ uint32_t lds[8];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < nparts; b += GR_BLOCK) {
+        uint32_t i = b + threadIdx.x;
+        uint32_t v = i < nparts ? partials[i] : 0;
+        uint32_t inc = block_inclusive_scan(v, lds);
+        uint32_t c = carry;
+        if (i < nparts) partials[i] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_scan_down(const uint32_t *in, uint32_t *out,
+                                                        uint32_t max_n, const uint32_t *d_count,
+                                                        const uint32_t *partials) {
+    __shared__ uint32_t lds[8];
+    __shared__ uint32_t carry;
+    const uint32_t n = load_count(d_count, max_n);
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    if (threadIdx.x == 0) carry = partials[blockIdx.x];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; r++) {
+        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
+        uint32_t v = i < n ? in[i] : 0;
+        uint32_t inc = block_inclusive_scan(v, lds);
+        uint32_t c = carry;
+        if (i < n) out[i] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
+        __syncthreads();
+    }
+}
+
+size_t scan_tmp_bytes(uint32_t max_n) {
+    return (size_t)gr_blocks(max_n, SCAN_TILE) * sizeof(uint32_t) + 256;
+}
+
+void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uint32_t *d_count,
+                    uint32_t *d_total, void *tmp, hipStream_t stream) {
+    uint32_t nb = gr_blocks(max_n, SCAN_TILE);
+    uint32_t *partials = (uint32_t *)tmp;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(GR_BLOCK), 0, stream, in, max_n, d_count,
+                       partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(GR_BLOCK), 0, stream, partials, nb,
+                       d_total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(GR_BLOCK), 0, stream, in, out, max_n, d_count,
+                       partials);
+}
+
+// ------------------------------------------------------------------ radix sort
+
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
+constexpr int RS_BITS = 8;
+constexpr int RS_BUCKETS = 1 << RS_BITS;
+
+__global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint32_t n, int shift,
+                                                      uint32_t *hist, uint32_t nb) {
+    __shared__ uint32_t h[RS_BUCKETS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_BUCKETS - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block d scans row d of the digit-major histogram in place (exclusive) and writes the
+// digit's total.
+__global__ __launch_bounds__(GR_BLOCK) void k_rs_scan(uint32_t *hist, uint32_t nb,
+                                                      uint32_t *totals) {
+    __shared__ uint32_t lds[8];
+    __shared__ uint32_t carry;
+    uint32_t *row = hist + (size_t)blockIdx.x * nb;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < nb; b += GR_BLOCK) {
+        uint32_t i = b + threadIdx.x;
+        uint32_t v = i < nb ? row[i] : 0;
+        uint32_t inc = block_inclusive_scan(v, lds);
+        uint32_t c = carry;
+        if (i < nb) row[i] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Stable scatter: keys of a tile are ranked round by round (256 consecutive keys per
+// round), inside a wave by ballot matching on the 8 digit bits, across the block's four
+// waves through LDS counts, so equal digits keep their input order.
+__global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin,
+                                                         uint64_t *kout, uint32_t *vout,
+                                                         uint32_t n, int shift,
+                                                         const uint32_t *hist, uint32_t nb,
+                                                         const uint32_t *totals) {
+    __shared__ uint32_t base[RS_BUCKETS];
+    __shared__ uint32_t wcnt[GR_BLOCK / GR_WAVE][RS_BUCKETS];
+    __shared__ uint32_t lds[8];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    {
+        uint32_t t = totals[tid];
+        uint32_t inc = block_inclusive_scan(t, lds);
+        base[tid] = inc - t + hist[(size_t)tid * nb + blockIdx.x];
+    }
+    __syncthreads();
+    const uint64_t tile = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t below = gr_lanemask_lt();
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint64_t i = tile + (uint64_t)r * GR_BLOCK + tid;
+        if (tile + (uint64_t)r * GR_BLOCK >= n) break;  // block-uniform
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0;
+        uint32_t val = valid ? vin[i] : 0;
+        uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
+#pragma unroll
+        for (int q = 0; q < GR_BLOCK / GR_WAVE; q++) wcnt[q][tid] = 0;
+        __syncthreads();
+        uint64_t peers = gr_ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RS_BITS; b++) {
+            uint64_t bb = gr_ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & below);
+        uint32_t cnt = (uint32_t)__popcll(peers);
+        if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + rank;
+            for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        base[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+    (void)lane;
+}
+
+size_t radix_tmp_bytes(uint32_t n) {
+    uint32_t nb = gr_blocks(n, RS_TILE);
+    return ((size_t)RS_BUCKETS * nb + RS_BUCKETS) * sizeof(uint32_t) + 256;
+}
+
+// Sorts bits [0, end_bit) of the keys, rounded up to whole 8-bit digits. The result lands
+// in (keys, vals); (keys_alt, vals_alt) are scratch of the same size.
+void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                      uint32_t n, int end_bit, void *tmp, hipStream_t stream) {
+    if (n == 0) return;
+    uint32_t nb = gr_blocks(n, RS_TILE);
+    uint32_t *hist = (uint32_t *)tmp;
+    uint32_t *totals = hist + (size_t)RS_BUCKETS * nb;
+    int passes = (end_bit + RS_BITS - 1) / RS_BITS;
+    uint64_t *ki = keys, *ko = keys_alt;
+    uint32_t *vi = vals, *vo = vals_alt;
+    for (int p = 0; p < passes; p++) {
+        int shift = p * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_scan, dim3(RS_BUCKETS), dim3(GR_BLOCK), 0, stream, hist, nb,
+                           totals);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, vi, ko, vo, n,
+                           shift, hist, nb, totals);
+        uint64_t *tk = ki;
+        ki = ko;
+        ko = tk;
+        uint32_t *tv = vi;
+        vi = vo;
+        vo = tv;
+    }
+    if (ki != keys) {
+        (void)hipMemcpyAsync(keys, ki, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
+        (void)hipMemcpyAsync(vals, vi, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+    }
+}
+
+}  // namespace gr

@@ -1,0 +1,157 @@
+/*
+ * graph_render_fa2 — the C-ABI boundary of the MI355X ForceAtlas2 / Barnes-Hut path.
+ *
+ * It replaces, behind the reference's layout registry, the three ForceAtlas2 entries of
+ * graph-core (paths relative to crates/graph-core/src/ of the reference):
+ *   "layout.forceatlas2"             layout/forceatlas2.rs:32-41   (dense, dim 2)
+ *   "layout.forceatlas2.barnes_hut"  layout/forceatlas2.rs:49-61   (quadtree, dim 2)
+ *   "layout.forceatlas2.3d"          layout/forceatlas2.rs:63-68   (dense, dim 3)
+ * registered at registry/layouts.rs:98-103, :263-268 and registry/arms_3d.rs:294-299.
+ *
+ * Conventions (mirroring crates/graph-wasm's ABI, docs/contract/wasm-abi.md):
+ *   - every call returns a gr_status; GR_OK is 0;
+ *   - gr_last_error() reports the calling thread's last failure as (code, name, rule),
+ *     the fields of StageError::Param / NonFinite (stage.rs:35-52);
+ *   - input pointers are borrowed for the call only; outputs go to caller buffers;
+ *   - device memory belongs to a session and is freed by gr_fa2_session_release();
+ *   - one HIP stream per session; distinct sessions may run concurrently on one device.
+ */
+#ifndef GRAPH_RENDER_FA2_H
+#define GRAPH_RENDER_FA2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: 1-4 are StageError's variants (stage.rs:35-52); 10-21 the wasm ABI's
+ * (docs/contract/wasm-abi.md:355-380); 30+ belong to this device path. */
+typedef enum gr_status {
+    GR_OK = 0,
+    GR_ERR_CAPACITY = 1,           /* StageError::Capacity: n or m past the u32 index space */
+    GR_ERR_NON_FINITE = 2,         /* StageError::NonFinite{column} */
+    GR_ERR_PARAM = 3,              /* StageError::Param{name, rule} */
+    GR_ERR_SNAPSHOT = 4,           /* StageError::Snapshot (reserved) */
+    GR_ERR_INVALID_HANDLE = 10,    /* wasm InvalidHandle */
+    GR_ERR_UNKNOWN_LAYOUT = 11,    /* wasm UnknownLayoutId */
+    GR_ERR_PARAM_OUT_OF_RANGE = 20,/* wasm ParamOutOfRange (ParamsError::OutOfRange) */
+    GR_ERR_PARAMS_MALFORMED = 21,  /* wasm ParamsMalformed (ParamsError::Malformed) */
+    GR_ERR_DEVICE = 30,            /* HIP runtime failure or no device */
+    GR_ERR_UNSUPPORTED = 31        /* a combination this build does not run */
+} gr_status;
+
+/* Fa2Params (layout/forceatlas2/state.rs:20-54) field for field. Defaults: max_iter 100,
+ * jitter_tolerance 1, scaling_ratio 2, gravity 1, seed 0, dim 2. */
+typedef struct gr_fa2_params {
+    uint32_t max_iter;
+    double jitter_tolerance;
+    double scaling_ratio;
+    double gravity;
+    uint32_t seed;
+    uint32_t dim;
+} gr_fa2_params;
+
+typedef enum gr_precision {
+    GR_PREC_FP64 = 0,       /* the reference's arithmetic, bit for bit */
+    GR_PREC_FP32 = 1,       /* FP32 forces; FP64 centres of mass and global reductions */
+    GR_PREC_FP32_KAHAN = 2  /* as FP32, with compensated per-body accumulation */
+} gr_precision;
+
+typedef enum gr_repulsion {
+    GR_REP_DEFAULT = 0,  /* what the layout id runs: tree for barnes_hut, dense otherwise */
+    GR_REP_TREE = 1,     /* Barnes-Hut over the reference's quadtree (octree at dim 3) */
+    GR_REP_DENSE = 2,    /* exact pair sum, the reference's dense arm */
+    GR_REP_PM = 3        /* particle mesh: CIC deposit, FFT convolution, CIC gradient read */
+} gr_repulsion;
+
+#define GR_EXT_EXPORT_TREE 1u /* keep cell bounds for gr_fa2_session_tree() */
+
+/* Extensions the reference does not have. A NULL gr_fa2_ext pointer means exactly the
+ * reference's semantics: theta 0.8 (barnes_hut.rs:24), FP64, unit edge weights, Mulberry32
+ * start positions (state.rs:272-283). */
+typedef struct gr_fa2_ext {
+    uint32_t struct_size;       /* sizeof(gr_fa2_ext) */
+    uint32_t flags;             /* GR_EXT_* */
+    double theta;               /* Barnes-Hut opening angle; < 0 means the reference's 0.8 */
+    int32_t precision;          /* gr_precision */
+    int32_t repulsion;          /* gr_repulsion */
+    const double *edge_weights; /* one per raw edge, NULL = unit; a simple edge keeps the
+                                   weight of its first raw edge */
+    const double *start;        /* n*dim row-major start positions, NULL = Mulberry32 */
+    uint32_t pm_grid;           /* PM mesh side (power of two, 64..4096), 0 = 2048 */
+    uint32_t reserved;
+} gr_fa2_ext;
+
+/* Fills the reference defaults (Fa2Params::default, state.rs:43-54). */
+void gr_fa2_params_default(gr_fa2_params *out);
+void gr_fa2_ext_default(gr_fa2_ext *out);
+
+/* The registry's wire form (registry/params.rs:144-202): one little-endian f64 per
+ * published parameter in schema order max_iter, jitter_tolerance, scaling_ratio, gravity,
+ * seed (registry/tunable.rs:229-240); an empty buffer is the defaults; a value outside the
+ * schema's range, non-finite, or fractional for an Int is refused, never clamped. */
+gr_status gr_fa2_params_decode(const uint8_t *bytes, size_t len, gr_fa2_params *out);
+
+/* Stage::run for one of the three ids (stage.rs:23-31): dedups the raw edge list as
+ * simple_graph does (layout/force/mod.rs:101-120), runs to max_iter or the 1e-10 early
+ * exit, and writes f32 columns (x, y, and z for the 3D id) narrowed as settle() narrows
+ * them (layout/forceatlas2.rs:82-91). On failure the output columns are zeroed. */
+gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *source,
+                        const uint32_t *target, size_t m, const uint8_t *params,
+                        size_t params_len, const gr_fa2_ext *ext, float *out_x, float *out_y,
+                        float *out_z);
+
+/* ---- resident session: the state of Fa2State (state.rs:60-80) kept in HBM ---- */
+typedef struct gr_fa2_session gr_fa2_session;
+
+/* Builds the simple graph, masses (degree + 1) and start positions, and uploads them.
+ * repulsion GR_REP_DEFAULT is the tree. Returns NULL on failure (see gr_last_error). */
+gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
+                                      const uint32_t *target, size_t m,
+                                      const gr_fa2_params *params, const gr_fa2_ext *ext);
+void gr_fa2_session_release(gr_fa2_session *s);
+
+/* Runs up to `iters` iterations of iterate() (state.rs:130-148), stopping early once an
+ * iteration moves less than 1e-10 in total (state.rs:122-128). *done_out (may be NULL)
+ * receives the iterations actually applied by this call. */
+gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_out);
+
+/* Current positions, n*dim row-major f64 (widened from FP32 in the FP32 precisions). */
+gr_status gr_fa2_session_positions(gr_fa2_session *s, double *out);
+gr_status gr_fa2_session_set_positions(gr_fa2_session *s, const double *in);
+
+/* One force evaluation on the current positions with no state change: parts is a mask of
+ * 1 attraction, 2 repulsion, 4 gravity, summed in iterate()'s order. out: n*dim f64. */
+gr_status gr_fa2_session_forces(gr_fa2_session *s, uint32_t parts, double *out);
+
+/* swing, traction, speed, speed_efficiency, iter_no, iterations applied, early-exit flag */
+gr_status gr_fa2_session_scalars(gr_fa2_session *s, double *out7);
+
+/* Simple edges (lo < hi, first raw edge kept) and per-node mass, as the session holds them. */
+gr_status gr_fa2_session_graph(gr_fa2_session *s, uint32_t *m_out, uint32_t *lo, uint32_t *hi,
+                               double *mass);
+
+/* The tree of the last force evaluation or iteration (GR_REP_TREE only). Any output pointer
+ * may be NULL. bounds: 2*dim f64 per cell (lo axes, then hi axes; needs GR_EXT_EXPORT_TREE);
+ * bodies: (mass, centre...) f64 per cell; order: n tree positions -> point index. */
+gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t cap,
+                              double *bounds, uint32_t *skip, uint32_t *start, uint32_t *end,
+                              double *bodies, uint32_t *order);
+
+/* Times `reps` launches of one stage on the session's stream with HIP events: "walk",
+ * "tree", "iteration", "pm". Writes the mean milliseconds per launch. */
+gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,
+                              double *ms_out);
+
+/* Last failure of the calling thread: code, parameter/column name, rule text. */
+gr_status gr_last_error(const char **name, const char **rule);
+
+/* Library version string. */
+const char *gr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,854 @@
+/*
+ * CPU oracle: TEST INFRASTRUCTURE ONLY (see fa2_oracle.h for what pins each part).
+ *
+ * A restatement, in C, of graph-core's f64 ForceAtlas2 and its Barnes-Hut arm. Every
+ * floating-point expression keeps the operand order of the Rust source it cites, and the
+ * file is compiled with -ffp-contract=off, so results are the reference's bit for bit.
+ * Reference paths are relative to crates/graph-core/src/.
+ */
+#include "fa2_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXD 3
+
+/* ------------------------------------------------------------------ rng */
+
+/* synthetic.rs:44-50: integer steps, then / 2^32. */
+double gro_mulberry32_next(uint32_t *state) {
+    *state += 0x6D2B79F5u;
+    uint32_t a = *state;
+    uint32_t t = (a ^ (a >> 15)) * (1u | a);
+    t = (t + ((t ^ (t >> 7)) * (61u | t))) ^ t;
+    return (double)(t ^ (t >> 14)) / 4294967296.0;
+}
+
+static uint32_t mulberry_pick(uint32_t *state, size_t len) {
+    /* synthetic.rs:60-63: floor(next * len) */
+    return (uint32_t)floor(gro_mulberry32_next(state) * (double)len);
+}
+
+/* layout/forceatlas2/state.rs:272-283: row-major, dim draws per node. */
+void gro_initial_positions(uint32_t n, uint32_t seed, uint32_t dim, double *out) {
+    uint32_t st = seed;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t a = 0; a < dim; a++) out[(size_t)i * dim + a] = gro_mulberry32_next(&st);
+}
+
+/* rng.rs:27-34: MurmurHash3 fmix64. */
+uint64_t gro_fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+/* rng.rs:39-44 */
+static uint64_t jig_fold(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t lo, uint32_t hi) {
+    uint64_t a = gro_fmix64((uint64_t)seed ^ ((uint64_t)tick << 32));
+    uint64_t b = gro_fmix64(a ^ ((uint64_t)pass << 32) ^ (uint64_t)lo);
+    return gro_fmix64(b ^ ((uint64_t)hi << 32) ^ a);
+}
+
+/* rng.rs:61-84 */
+double gro_jiggle(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t i, uint32_t j) {
+    uint32_t lo = i <= j ? i : j, hi = i <= j ? j : i;
+    uint64_t w = jig_fold(seed, tick, pass, lo, hi) >> 11;
+    if (w == (1ull << 52)) w = (1ull << 52) + 1;
+    double u = (double)w * (1.0 / (double)(1ull << 53));
+    return (u - 0.5) * 1e-6;
+}
+
+/* ------------------------------------------------------------ gate model */
+
+/* synthetic.rs:92-170: the node draws (roll, then one pick for 80% of nodes) precede the
+ * edges; an edge's strength is drawn after its endpoints and only when it is kept. The
+ * seed remix (stage/topology.rs:37-46) redraws kinds only, so endpoints are seed-free. */
+size_t gro_gate_edges(uint32_t count, uint32_t *source, uint32_t *target, size_t cap) {
+    uint32_t st = 0x051042u;
+    for (uint32_t i = 0; i < count; i++) {
+        double roll = gro_mulberry32_next(&st);
+        if (roll < 0.6)
+            (void)mulberry_pick(&st, 12);
+        else if (roll < 0.8)
+            (void)mulberry_pick(&st, 4);
+    }
+    size_t m = 0;
+#define PUSH(A, B)                                                                          \
+    do {                                                                                    \
+        uint32_t a_ = (A), b_ = (B);                                                        \
+        if (a_ != b_ && m < cap) {                                                          \
+            source[m] = a_;                                                                 \
+            target[m] = b_;                                                                 \
+            m++;                                                                            \
+            (void)gro_mulberry32_next(&st);                                                 \
+        }                                                                                   \
+    } while (0)
+    for (uint32_t i = 1; i < count; i++) {
+        double x = gro_mulberry32_next(&st), y = gro_mulberry32_next(&st);
+        uint32_t t = (uint32_t)floor(x * y * (double)i);
+        PUSH(i, t);
+        if (gro_mulberry32_next(&st) < 0.5) {
+            x = gro_mulberry32_next(&st);
+            y = gro_mulberry32_next(&st);
+            t = (uint32_t)floor(x * y * (double)i);
+            PUSH(i, t);
+        }
+    }
+    uint32_t extras = (uint32_t)floor((double)count * 0.05);
+    for (uint32_t e = 0; e < extras; e++) {
+        uint32_t a = mulberry_pick(&st, count);
+        uint32_t b = mulberry_pick(&st, count);
+        PUSH(a, b);
+    }
+#undef PUSH
+    return m;
+}
+
+/* ----------------------------------------------------------- tree build */
+
+typedef struct {
+    double lo[MAXD], hi[MAXD];
+} box_t;
+
+typedef struct {
+    int leaf;
+    int32_t head;        /* leaf: chain head */
+    int32_t child[8];    /* internal: children by slot, -1 empty */
+} shape_t;
+
+struct gro_tree {
+    uint32_t n, dim;
+    const double *pts[MAXD];
+    box_t root_bounds;
+    int root_set;
+    int32_t root;
+    shape_t *shape;
+    size_t nshape, capshape;
+    int32_t *chain_next;
+    /* preorder arena */
+    box_t *cbox;
+    uint32_t *skip, *start, *end;
+    size_t ncells, capcells;
+    uint32_t *order;
+    size_t norder;
+};
+
+static double pt(const gro_tree *t, uint32_t i, uint32_t a) { return t->pts[a][i]; }
+
+static int finite_pt(const gro_tree *t, uint32_t i) {
+    for (uint32_t a = 0; a < t->dim; a++)
+        if (!isfinite(pt(t, i, a))) return 0;
+    return 1;
+}
+
+static int32_t push_shape(gro_tree *t, int leaf, int32_t head) {
+    if (t->nshape == t->capshape) {
+        t->capshape = t->capshape ? t->capshape * 2 : 64;
+        t->shape = (shape_t *)realloc(t->shape, t->capshape * sizeof(shape_t));
+    }
+    shape_t *s = &t->shape[t->nshape];
+    s->leaf = leaf;
+    s->head = head;
+    for (int c = 0; c < 8; c++) s->child[c] = -1;
+    return (int32_t)t->nshape++;
+}
+
+/* quadtree.rs:157-195 / octree.rs:96-112,220-254: root square from floor, then doubling. */
+static void cover(gro_tree *t, const double *p) {
+    box_t *b = &t->root_bounds;
+    if (!t->root_set) {
+        for (uint32_t a = 0; a < t->dim; a++) {
+            double h = floor(p[a]) + 1.0;
+            b->lo[a] = h - 1.0;
+            b->hi[a] = h;
+        }
+        t->root_set = 1;
+        return;
+    }
+    double z = (b->hi[0] - b->lo[0] != 0.0) ? b->hi[0] - b->lo[0] : 1.0;
+    for (;;) {
+        int out = 0;
+        for (uint32_t a = 0; a < t->dim; a++)
+            if (b->lo[a] > p[a] || p[a] >= b->hi[a]) out = 1;
+        if (!out) break;
+        z *= 2.0;
+        for (uint32_t a = 0; a < t->dim; a++) {
+            if (p[a] < b->lo[a])
+                b->lo[a] = b->hi[a] - z;
+            else
+                b->hi[a] = b->lo[a] + z;
+        }
+        if (!(isfinite(z) && isfinite(b->hi[0] - b->lo[0]))) return;
+    }
+}
+
+/* bounds.rs:34-50: narrow toward p, slot bit a set when p[a] >= mid[a]. */
+static int narrow(box_t *b, uint32_t dim, const double *p) {
+    int slot = 0;
+    for (uint32_t a = 0; a < dim; a++) {
+        double m = (b->lo[a] + b->hi[a]) / 2.0;
+        if (p[a] >= m) {
+            b->lo[a] = m;
+            slot |= 1 << a;
+        } else {
+            b->hi[a] = m;
+        }
+    }
+    return slot;
+}
+
+static int slot_of(const box_t *b, uint32_t dim, const double *p) {
+    int slot = 0;
+    for (uint32_t a = 0; a < dim; a++) {
+        double m = (b->lo[a] + b->hi[a]) / 2.0;
+        if (p[a] >= m) slot |= 1 << a;
+    }
+    return slot;
+}
+
+static void set_child(gro_tree *t, int32_t parent, int slot, int32_t child) {
+    if (!t->shape[parent].leaf) t->shape[parent].child[slot] = child;
+}
+
+static void attach(gro_tree *t, int32_t parent, int slot, int32_t child) {
+    if (parent >= 0)
+        set_child(t, parent, slot, child);
+    else
+        t->root = child;
+}
+
+static void chain(gro_tree *t, uint32_t point, int32_t node) {
+    t->chain_next[point] = t->shape[node].head;
+    t->shape[node].head = (int32_t)point;
+}
+
+/* quadtree/build.rs:60-96 */
+static void insert_leaf(gro_tree *t, uint32_t point, int32_t node, int32_t parent, int pslot,
+                        box_t bounds) {
+    double p[MAXD], q[MAXD];
+    int32_t head = t->shape[node].head;
+    int same = 1;
+    for (uint32_t a = 0; a < t->dim; a++) {
+        p[a] = pt(t, point, a);
+        q[a] = pt(t, (uint32_t)head, a);
+        if (p[a] != q[a]) same = 0;
+    }
+    if (same) {
+        chain(t, point, node);
+        return;
+    }
+    int32_t at = parent;
+    int slot = parent >= 0 ? pslot : 0;
+    for (;;) {
+        double before = bounds.hi[0] - bounds.lo[0];
+        box_t pre = bounds;
+        int i = narrow(&bounds, t->dim, p);
+        int j = slot_of(&pre, t->dim, q);
+        double after = bounds.hi[0] - bounds.lo[0];
+        if (!(after < before)) {
+            attach(t, at, slot, node);
+            chain(t, point, node);
+            return;
+        }
+        int32_t internal = push_shape(t, 0, -1);
+        attach(t, at, slot, internal);
+        if (i != j) {
+            set_child(t, internal, j, node);
+            int32_t leaf = push_shape(t, 1, (int32_t)point);
+            set_child(t, internal, i, leaf);
+            return;
+        }
+        at = internal;
+        slot = i;
+    }
+}
+
+/* quadtree/build.rs:26-55 */
+static void add(gro_tree *t, uint32_t point) {
+    if (!finite_pt(t, point)) return;
+    if (t->root < 0) {
+        t->root = push_shape(t, 1, (int32_t)point);
+        return;
+    }
+    double p[MAXD];
+    for (uint32_t a = 0; a < t->dim; a++) p[a] = pt(t, point, a);
+    int32_t node = t->root, parent = -1;
+    int pslot = 0;
+    box_t bounds = t->root_bounds;
+    for (;;) {
+        if (t->shape[node].leaf) {
+            insert_leaf(t, point, node, parent, pslot, bounds);
+            return;
+        }
+        int slot = narrow(&bounds, t->dim, p);
+        int32_t child = t->shape[node].child[slot];
+        if (child >= 0) {
+            parent = node;
+            pslot = slot;
+            node = child;
+        } else {
+            int32_t leaf = push_shape(t, 1, (int32_t)point);
+            set_child(t, node, slot, leaf);
+            return;
+        }
+    }
+}
+
+static box_t quadrant(box_t b, uint32_t dim, int slot) {
+    box_t r = b;
+    for (uint32_t a = 0; a < dim; a++) {
+        double m = (b.lo[a] + b.hi[a]) / 2.0;
+        if (slot & (1 << a))
+            r.lo[a] = m;
+        else
+            r.hi[a] = m;
+    }
+    return r;
+}
+
+static void push_cell(gro_tree *t, box_t b, uint32_t skip, uint32_t start, uint32_t end) {
+    if (t->ncells == t->capcells) {
+        t->capcells = t->capcells ? t->capcells * 2 : 64;
+        t->cbox = (box_t *)realloc(t->cbox, t->capcells * sizeof(box_t));
+        t->skip = (uint32_t *)realloc(t->skip, t->capcells * sizeof(uint32_t));
+        t->start = (uint32_t *)realloc(t->start, t->capcells * sizeof(uint32_t));
+        t->end = (uint32_t *)realloc(t->end, t->capcells * sizeof(uint32_t));
+    }
+    t->cbox[t->ncells] = b;
+    t->skip[t->ncells] = skip;
+    t->start[t->ncells] = start;
+    t->end[t->ncells] = end;
+    t->ncells++;
+}
+
+typedef struct {
+    int close;
+    int32_t node;
+    uint32_t k;
+    box_t b;
+} visit_t;
+
+/* quadtree/preorder.rs:41-99: preorder arena, children in slot order, non-finite points
+ * appended ascending. */
+static void flatten(gro_tree *t) {
+    size_t cap = 64, top = 0;
+    visit_t *stack = (visit_t *)malloc(cap * sizeof(visit_t));
+    if (t->root >= 0) {
+        stack[top].close = 0;
+        stack[top].node = t->root;
+        stack[top].b = t->root_bounds;
+        top++;
+    }
+    uint32_t nslots = 1u << t->dim;
+    while (top > 0) {
+        visit_t v = stack[--top];
+        if (v.close) {
+            t->skip[v.k] = (uint32_t)t->ncells;
+            t->end[v.k] = (uint32_t)t->norder;
+            continue;
+        }
+        uint32_t k = (uint32_t)t->ncells;
+        uint32_t start = (uint32_t)t->norder;
+        shape_t *s = &t->shape[v.node];
+        if (s->leaf) {
+            int32_t p = s->head;
+            while (p >= 0) {
+                t->order[t->norder++] = (uint32_t)p;
+                p = t->chain_next[p];
+            }
+            push_cell(t, v.b, k + 1, start, (uint32_t)t->norder);
+        } else {
+            push_cell(t, v.b, 0, start, 0);
+            if (top + nslots + 1 >= cap) {
+                cap = cap * 2 + nslots + 1;
+                stack = (visit_t *)realloc(stack, cap * sizeof(visit_t));
+            }
+            stack[top].close = 1;
+            stack[top].k = k;
+            top++;
+            for (int slot = (int)nslots - 1; slot >= 0; slot--) {
+                int32_t c = t->shape[v.node].child[slot];
+                if (c >= 0) {
+                    stack[top].close = 0;
+                    stack[top].node = c;
+                    stack[top].b = quadrant(v.b, t->dim, slot);
+                    top++;
+                }
+            }
+        }
+    }
+    free(stack);
+    for (uint32_t i = 0; i < t->n; i++)
+        if (!finite_pt(t, i)) t->order[t->norder++] = i;
+}
+
+/* quadtree.rs:116-140 */
+gro_tree *gro_tree_build(uint32_t n, uint32_t dim, const double *xs, const double *ys,
+                         const double *zs) {
+    gro_tree *t = (gro_tree *)calloc(1, sizeof(gro_tree));
+    t->n = n;
+    t->dim = dim;
+    t->pts[0] = xs;
+    t->pts[1] = ys;
+    t->pts[2] = zs;
+    t->root = -1;
+    t->chain_next = (int32_t *)malloc((n ? n : 1) * sizeof(int32_t));
+    for (uint32_t i = 0; i < n; i++) t->chain_next[i] = -1;
+    t->order = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    /* bounds_of: the finite extent (quadtree.rs:46-53) */
+    int any = 0;
+    double lo[MAXD], hi[MAXD];
+    for (uint32_t i = 0; i < n; i++) {
+        if (!finite_pt(t, i)) continue;
+        for (uint32_t a = 0; a < dim; a++) {
+            double v = pt(t, i, a);
+            if (!any) {
+                lo[a] = v;
+                hi[a] = v;
+            } else {
+                if (v < lo[a]) lo[a] = v;
+                if (v > hi[a]) hi[a] = v;
+            }
+        }
+        any = 1;
+    }
+    if (any) {
+        cover(t, lo);
+        cover(t, hi);
+        for (uint32_t i = 0; i < n; i++) add(t, i);
+    }
+    flatten(t);
+    return t;
+}
+
+void gro_tree_free(gro_tree *t) {
+    if (!t) return;
+    free(t->shape);
+    free(t->chain_next);
+    free(t->cbox);
+    free(t->skip);
+    free(t->start);
+    free(t->end);
+    free(t->order);
+    free(t);
+}
+
+uint32_t gro_tree_ncells(const gro_tree *t) { return (uint32_t)t->ncells; }
+
+void gro_tree_cells(const gro_tree *t, double *bounds, uint32_t *skip, uint32_t *start,
+                    uint32_t *end) {
+    uint32_t d = t->dim;
+    for (size_t k = 0; k < t->ncells; k++) {
+        for (uint32_t a = 0; a < d; a++) {
+            bounds[k * 2 * d + a] = t->cbox[k].lo[a];
+            bounds[k * 2 * d + d + a] = t->cbox[k].hi[a];
+        }
+        skip[k] = t->skip[k];
+        start[k] = t->start[k];
+        end[k] = t->end[k];
+    }
+}
+
+void gro_tree_order(const gro_tree *t, uint32_t *order) {
+    memcpy(order, t->order, t->n * sizeof(uint32_t));
+}
+
+/* barnes_hut.rs:62-110: reverse preorder; a leaf sums its chain, an internal cell its
+ * children's unnormalised sums in slot order; then every sum becomes a centre. */
+static void aggregate(const gro_tree *t, const double *mass, const double *const *pos,
+                      double *bodies /* (1+dim) per cell */) {
+    uint32_t d = t->dim, w = 1 + d;
+    for (size_t kk = t->ncells; kk-- > 0;) {
+        double s[1 + MAXD] = {0.0, 0.0, 0.0, 0.0};
+        if (t->skip[kk] == kk + 1) {
+            for (uint32_t q = t->start[kk]; q < t->end[kk]; q++) {
+                uint32_t p = t->order[q];
+                double m = mass[p];
+                s[0] += m;
+                for (uint32_t a = 0; a < d; a++) s[1 + a] += m * pos[a][p];
+            }
+        } else {
+            size_t c = kk + 1;
+            while (c < t->skip[kk]) {
+                for (uint32_t a = 0; a < w; a++) s[a] += bodies[c * w + a];
+                c = t->skip[c];
+            }
+        }
+        for (uint32_t a = 0; a < w; a++) bodies[kk * w + a] = s[a];
+    }
+    for (size_t k = 0; k < t->ncells; k++)
+        for (uint32_t a = 0; a < d; a++) bodies[k * w + 1 + a] = bodies[k * w + 1 + a] / bodies[k * w];
+}
+
+void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, const double *ys,
+                     const double *zs, double *out) {
+    const double *pos[MAXD] = {xs, ys, zs};
+    aggregate(t, mass, pos, out);
+}
+
+/* ------------------------------------------------------------- FA2 state */
+
+struct gro_fa2 {
+    uint32_t n, dim, m;
+    uint32_t *lo, *hi;
+    double *w; /* per simple edge weight, NULL = unit (the reference) */
+    double jitter_tolerance, scaling_ratio, gravity;
+    uint32_t seed;
+    uint32_t iter_no;
+    double swing, traction, speed, speed_efficiency;
+    double *p, *mass, *u; /* p,u: n*MAXD rows */
+    int repulsion;
+    double theta2;
+};
+
+/* layout/force/mod.rs:101-120: drop self-loops and repeated unordered pairs, first kept. */
+static void simple_graph(gro_fa2 *s, const uint32_t *src, const uint32_t *dst, size_t m_raw,
+                         const double *weights) {
+    size_t cap = 16;
+    while (cap < 2 * m_raw + 2) cap <<= 1;
+    uint64_t *table = (uint64_t *)malloc(cap * sizeof(uint64_t));
+    for (size_t i = 0; i < cap; i++) table[i] = ~0ull;
+    s->lo = (uint32_t *)malloc((m_raw ? m_raw : 1) * sizeof(uint32_t));
+    s->hi = (uint32_t *)malloc((m_raw ? m_raw : 1) * sizeof(uint32_t));
+    s->w = weights ? (double *)malloc((m_raw ? m_raw : 1) * sizeof(double)) : NULL;
+    uint32_t m = 0;
+    for (size_t e = 0; e < m_raw; e++) {
+        uint32_t a = src[e], b = dst[e];
+        if (a == b) continue;
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        uint64_t key = ((uint64_t)lo << 32) | hi;
+        size_t h = (size_t)(gro_fmix64(key) & (cap - 1));
+        int dup = 0;
+        while (table[h] != ~0ull) {
+            if (table[h] == key) {
+                dup = 1;
+                break;
+            }
+            h = (h + 1) & (cap - 1);
+        }
+        if (dup) continue;
+        table[h] = key;
+        s->lo[m] = lo;
+        s->hi[m] = hi;
+        if (s->w) s->w[m] = weights[e];
+        m++;
+    }
+    free(table);
+    s->m = m;
+}
+
+gro_fa2 *gro_fa2_new(uint32_t n, const uint32_t *source, const uint32_t *target, size_t m_raw,
+                     const double *weights_raw, uint32_t dim, double jitter_tolerance,
+                     double scaling_ratio, double gravity, uint32_t seed, const double *start,
+                     int repulsion, double theta2) {
+    gro_fa2 *s = (gro_fa2 *)calloc(1, sizeof(gro_fa2));
+    s->n = n;
+    s->dim = dim;
+    s->jitter_tolerance = jitter_tolerance;
+    s->scaling_ratio = scaling_ratio;
+    s->gravity = gravity;
+    s->seed = seed;
+    s->swing = s->traction = s->speed = s->speed_efficiency = 1.0;
+    s->repulsion = repulsion;
+    s->theta2 = theta2;
+    simple_graph(s, source, target, m_raw, weights_raw);
+    size_t nn = n ? n : 1;
+    s->p = (double *)calloc(nn * MAXD, sizeof(double));
+    s->u = (double *)calloc(nn * MAXD, sizeof(double));
+    s->mass = (double *)calloc(nn, sizeof(double));
+    uint32_t *deg = (uint32_t *)calloc(nn, sizeof(uint32_t));
+    for (uint32_t e = 0; e < s->m; e++) {
+        deg[s->lo[e]]++;
+        deg[s->hi[e]]++;
+    }
+    for (uint32_t v = 0; v < n; v++) s->mass[v] = (double)deg[v] + 1.0; /* state.rs:86 */
+    free(deg);
+    if (start) {
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t a = 0; a < dim; a++) s->p[(size_t)i * MAXD + a] = start[(size_t)i * dim + a];
+    } else {
+        uint32_t st = seed;
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t a = 0; a < dim; a++) s->p[(size_t)i * MAXD + a] = gro_mulberry32_next(&st);
+    }
+    return s;
+}
+
+void gro_fa2_free(gro_fa2 *s) {
+    if (!s) return;
+    free(s->lo);
+    free(s->hi);
+    free(s->w);
+    free(s->p);
+    free(s->u);
+    free(s->mass);
+    free(s);
+}
+
+void gro_fa2_set_theta2(gro_fa2 *s, double theta2, int repulsion) {
+    s->theta2 = theta2;
+    s->repulsion = repulsion;
+}
+
+#define P(s, i, a) ((s)->p[(size_t)(i) * MAXD + (a)])
+#define U(s, i, a) ((s)->u[(size_t)(i) * MAXD + (a)])
+
+static double norm2(const double *v, uint32_t dim) {
+    double sum = 0.0;
+    for (uint32_t a = 0; a < dim; a++) sum += v[a] * v[a];
+    return sum;
+}
+
+/* state.rs:152-162 (weights: the extension multiplies the pull per simple edge) */
+static void attraction(gro_fa2 *s) {
+    for (uint32_t e = 0; e < s->m; e++) {
+        uint32_t lo = s->lo[e], hi = s->hi[e];
+        for (uint32_t a = 0; a < s->dim; a++) {
+            double d = P(s, hi, a) - P(s, lo, a);
+            if (s->w) d = d * s->w[e];
+            U(s, lo, a) += d;
+            U(s, hi, a) -= d;
+        }
+    }
+}
+
+/* state.rs:185-203 */
+static void repel_delta(const gro_fa2 *s, uint32_t i, uint32_t j, double *d) {
+    int all_zero = 1;
+    for (uint32_t a = 0; a < s->dim; a++) {
+        d[a] = P(s, i, a) - P(s, j, a);
+        if (d[a] != 0.0) all_zero = 0;
+    }
+    if (all_zero)
+        for (uint32_t a = 0; a < s->dim; a++) d[a] = gro_jiggle(s->seed, s->iter_no, a, i, j);
+}
+
+/* state.rs:169-181 */
+static void repulsion_dense(gro_fa2 *s) {
+    double k = s->scaling_ratio;
+    for (uint32_t i = 0; i < s->n; i++) {
+        for (uint32_t j = i + 1; j < s->n; j++) {
+            double d[MAXD];
+            repel_delta(s, i, j, d);
+            double d2 = norm2(d, s->dim);
+            double f = s->mass[i] * s->mass[j] / d2 * k;
+            for (uint32_t a = 0; a < s->dim; a++) {
+                U(s, i, a) += d[a] * f;
+                U(s, j, a) -= d[a] * f;
+            }
+        }
+    }
+}
+
+/* barnes_hut.rs:163-183 */
+static void pairs(const gro_fa2 *s, uint32_t i, const uint32_t *leaf, uint32_t len, double *f) {
+    double kr = s->scaling_ratio;
+    for (uint32_t q = 0; q < len; q++) {
+        uint32_t j = leaf[q];
+        if (j == i) continue;
+        double d[MAXD];
+        if (i < j) {
+            repel_delta(s, i, j, d);
+        } else {
+            repel_delta(s, j, i, d);
+            for (uint32_t a = 0; a < s->dim; a++) d[a] = -d[a];
+        }
+        double d2 = d[0] * d[0] + d[1] * d[1];
+        if (s->dim == 3) d2 = d2 + d[2] * d[2];
+        double ff = s->mass[i] * s->mass[j] / d2 * kr;
+        for (uint32_t a = 0; a < s->dim; a++) f[a] = f[a] + d[a] * ff;
+    }
+}
+
+/* barnes_hut.rs:112-159: one stackless preorder walk per tree position. */
+static void repulsion_tree(gro_fa2 *s) {
+    uint32_t n = s->n, d = s->dim, w = 1 + d;
+    double *cols[MAXD];
+    for (uint32_t a = 0; a < d; a++) {
+        cols[a] = (double *)malloc((n ? n : 1) * sizeof(double));
+        for (uint32_t i = 0; i < n; i++) cols[a][i] = P(s, i, a);
+    }
+    gro_tree *t = gro_tree_build(n, d, cols[0], cols[1], d == 3 ? cols[2] : NULL);
+    double *bodies = (double *)malloc((t->ncells ? t->ncells : 1) * w * sizeof(double));
+    aggregate(t, s->mass, (const double *const *)cols, bodies);
+    double kr = s->scaling_ratio;
+    for (uint32_t p = 0; p < n; p++) {
+        uint32_t i = t->order[p];
+        double xi[MAXD], f[MAXD] = {0.0, 0.0, 0.0};
+        for (uint32_t a = 0; a < d; a++) xi[a] = P(s, i, a);
+        double mi = s->mass[i];
+        size_t k = 0;
+        while (k < t->ncells) {
+            int holds = t->start[k] <= p && p < t->end[k];
+            const double *b = &bodies[k * w];
+            double dx[MAXD] = {0.0, 0.0, 0.0};
+            for (uint32_t a = 0; a < d; a++) dx[a] = xi[a] - b[1 + a];
+            double d2 = dx[0] * dx[0] + dx[1] * dx[1];
+            if (d == 3) d2 = d2 + dx[2] * dx[2];
+            double wd = t->cbox[k].hi[0] - t->cbox[k].lo[0];
+            if (!holds && d2 > 0.0 && wd * wd < s->theta2 * d2) {
+                double ff = mi * b[0] / d2 * kr;
+                for (uint32_t a = 0; a < d; a++) f[a] = f[a] + dx[a] * ff;
+                k = t->skip[k];
+            } else if (t->skip[k] == k + 1) {
+                double lf[MAXD] = {0.0, 0.0, 0.0};
+                pairs(s, i, &t->order[t->start[k]], t->end[k] - t->start[k], lf);
+                for (uint32_t a = 0; a < d; a++) f[a] = f[a] + lf[a];
+                k += 1;
+            } else {
+                k += 1;
+            }
+        }
+        for (uint32_t a = 0; a < d; a++) U(s, i, a) += f[a];
+    }
+    free(bodies);
+    gro_tree_free(t);
+    for (uint32_t a = 0; a < d; a++) free(cols[a]);
+}
+
+/* state.rs:208-251: sequential centroid, then the unit vector scaled by gravity*mass. */
+static void gravity_pull(gro_fa2 *s) {
+    uint32_t n = s->n, d = s->dim;
+    if (n == 0) return;
+    double mean[MAXD] = {0.0, 0.0, 0.0};
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t a = 0; a < d; a++) mean[a] += P(s, i, a);
+    for (uint32_t a = 0; a < d; a++) mean[a] /= (double)n;
+    for (uint32_t i = 0; i < n; i++) {
+        double c[MAXD];
+        for (uint32_t a = 0; a < d; a++) c[a] = P(s, i, a) - mean[a];
+        double norm = sqrt(norm2(c, d));
+        double g = s->gravity * s->mass[i];
+        for (uint32_t a = 0; a < d; a++) {
+            double unit = norm > 0.0 ? c[a] / norm : 0.0;
+            U(s, i, a) -= g * unit;
+        }
+    }
+}
+
+/* schedule.rs:16-30 */
+static void swing_and_traction(const gro_fa2 *s, double *sw, double *tr) {
+    double swing = 0.0, traction = 0.0;
+    for (uint32_t i = 0; i < s->n; i++) {
+        double back[MAXD], fwd[MAXD];
+        for (uint32_t a = 0; a < s->dim; a++) {
+            back[a] = P(s, i, a) - U(s, i, a);
+            fwd[a] = P(s, i, a) + U(s, i, a);
+        }
+        swing += s->mass[i] * sqrt(norm2(back, s->dim));
+        traction += 0.5 * s->mass[i] * sqrt(norm2(fwd, s->dim));
+    }
+    *sw = swing;
+    *tr = traction;
+}
+
+/* schedule.rs:33-60: networkx's estimate_factor. */
+static void estimate_factor(gro_fa2 *s, double swing, double traction) {
+    double n = (double)s->n;
+    double jt = s->jitter_tolerance;
+    double opt_jitter = 0.05 * sqrt(n);
+    double min_jitter = sqrt(opt_jitter);
+    double min_speed_efficiency = 0.05;
+    double other = fmin(10.0, opt_jitter * traction / (n * n));
+    double jitter = jt * fmax(min_jitter, other);
+    if (swing / traction > 2.0) {
+        if (s->speed_efficiency > min_speed_efficiency) s->speed_efficiency *= 0.5;
+        jitter = fmax(jitter, jt);
+    }
+    double target_speed = swing == 0.0 ? INFINITY : jitter * s->speed_efficiency * traction / swing;
+    if (swing > jitter * traction) {
+        if (s->speed_efficiency > min_speed_efficiency) s->speed_efficiency *= 0.7;
+    } else if (s->speed < 1000.0) {
+        s->speed_efficiency *= 1.3;
+    }
+    s->speed += fmin(target_speed - s->speed, 0.5 * s->speed);
+}
+
+/* schedule.rs:62-77 */
+static double apply_update(gro_fa2 *s) {
+    double moved = 0.0;
+    for (uint32_t i = 0; i < s->n; i++) {
+        double norm = sqrt(norm2(&s->u[(size_t)i * MAXD], s->dim));
+        double factor = s->speed / (1.0 + sqrt(s->speed * s->mass[i] * norm));
+        double step = 0.0;
+        for (uint32_t a = 0; a < s->dim; a++) {
+            double dd = U(s, i, a) * factor;
+            P(s, i, a) += dd;
+            step += fabs(dd);
+        }
+        moved += step;
+    }
+    return moved;
+}
+
+static void forces(gro_fa2 *s, uint32_t parts) {
+    memset(s->u, 0, (size_t)(s->n ? s->n : 1) * MAXD * sizeof(double));
+    if (parts & 1) attraction(s);
+    if (parts & 2) {
+        if (s->repulsion == 1)
+            repulsion_tree(s);
+        else
+            repulsion_dense(s);
+    }
+    if (parts & 4) gravity_pull(s);
+}
+
+/* state.rs:130-148 */
+double gro_fa2_iterate(gro_fa2 *s) {
+    forces(s, 7);
+    double sw, tr;
+    swing_and_traction(s, &sw, &tr);
+    s->swing += sw;
+    s->traction += tr;
+    estimate_factor(s, s->swing, s->traction);
+    s->iter_no += 1;
+    return apply_update(s);
+}
+
+/* state.rs:122-128 */
+uint32_t gro_fa2_run(gro_fa2 *s, uint32_t max_iter) {
+    uint32_t it = 0;
+    for (; it < max_iter;) {
+        double moved = gro_fa2_iterate(s);
+        it++;
+        if (moved < 1e-10) break;
+    }
+    return it;
+}
+
+void gro_fa2_positions(const gro_fa2 *s, double *out) {
+    for (uint32_t i = 0; i < s->n; i++)
+        for (uint32_t a = 0; a < s->dim; a++) out[(size_t)i * s->dim + a] = P(s, i, a);
+}
+
+void gro_fa2_set_positions(gro_fa2 *s, const double *in) {
+    for (uint32_t i = 0; i < s->n; i++)
+        for (uint32_t a = 0; a < s->dim; a++) P(s, i, a) = in[(size_t)i * s->dim + a];
+}
+
+void gro_fa2_mass(const gro_fa2 *s, double *out) { memcpy(out, s->mass, s->n * sizeof(double)); }
+
+void gro_fa2_forces(gro_fa2 *s, uint32_t parts, double *out) {
+    forces(s, parts);
+    for (uint32_t i = 0; i < s->n; i++)
+        for (uint32_t a = 0; a < s->dim; a++) out[(size_t)i * s->dim + a] = U(s, i, a);
+}
+
+void gro_fa2_scalars(const gro_fa2 *s, double *out5) {
+    out5[0] = s->swing;
+    out5[1] = s->traction;
+    out5[2] = s->speed;
+    out5[3] = s->speed_efficiency;
+    out5[4] = (double)s->iter_no;
+}
+
+uint32_t gro_fa2_simple_edges(const gro_fa2 *s, uint32_t *lo, uint32_t *hi) {
+    if (lo) memcpy(lo, s->lo, s->m * sizeof(uint32_t));
+    if (hi) memcpy(hi, s->hi, s->m * sizeof(uint32_t));
+    return s->m;
+}

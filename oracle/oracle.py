@@ -1,0 +1,220 @@
+"""ctypes face of the CPU oracle: TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only to check (or time, as the CPU baseline) the product path. The product library
+never calls it. Build with ``make -C oracle`` (or ``build()`` below).
+
+Parity status of each function is stated in fa2_oracle.h; in short: the dense 2D FA2 is
+pinned by the reference's networkx fixture, the tree by the brute-force PR-quadtree
+definition and the reference's quadtree invariants, the BH walk by theta=0 == dense.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfa2_oracle.so")
+
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.gro_mulberry32_next.restype = C.c_double
+        L.gro_mulberry32_next.argtypes = [C.POINTER(C.c_uint32)]
+        L.gro_initial_positions.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, f64p]
+        L.gro_fmix64.restype = C.c_uint64
+        L.gro_fmix64.argtypes = [C.c_uint64]
+        L.gro_jiggle.restype = C.c_double
+        L.gro_jiggle.argtypes = [C.c_uint32] * 5
+        L.gro_gate_edges.restype = C.c_size_t
+        L.gro_gate_edges.argtypes = [C.c_uint32, u32p, u32p, C.c_size_t]
+        L.gro_fa2_new.restype = C.c_void_p
+        L.gro_fa2_new.argtypes = [C.c_uint32, u32p, u32p, C.c_size_t, C.c_void_p, C.c_uint32,
+                                  C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_void_p,
+                                  C.c_int, C.c_double]
+        L.gro_fa2_free.argtypes = [C.c_void_p]
+        L.gro_fa2_iterate.restype = C.c_double
+        L.gro_fa2_iterate.argtypes = [C.c_void_p]
+        L.gro_fa2_run.restype = C.c_uint32
+        L.gro_fa2_run.argtypes = [C.c_void_p, C.c_uint32]
+        L.gro_fa2_positions.argtypes = [C.c_void_p, f64p]
+        L.gro_fa2_set_positions.argtypes = [C.c_void_p, f64p]
+        L.gro_fa2_mass.argtypes = [C.c_void_p, f64p]
+        L.gro_fa2_forces.argtypes = [C.c_void_p, C.c_uint32, f64p]
+        L.gro_fa2_scalars.argtypes = [C.c_void_p, f64p]
+        L.gro_fa2_simple_edges.restype = C.c_uint32
+        L.gro_fa2_simple_edges.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.gro_fa2_set_theta2.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.gro_tree_build.restype = C.c_void_p
+        L.gro_tree_build.argtypes = [C.c_uint32, C.c_uint32, f64p, f64p, C.c_void_p]
+        L.gro_tree_free.argtypes = [C.c_void_p]
+        L.gro_tree_ncells.restype = C.c_uint32
+        L.gro_tree_ncells.argtypes = [C.c_void_p]
+        L.gro_tree_cells.argtypes = [C.c_void_p, f64p, u32p, u32p, u32p]
+        L.gro_tree_order.argtypes = [C.c_void_p, u32p]
+        L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
+        _lib = L
+    return _lib
+
+
+THETA2_REFERENCE = 0.8 * 0.8  # barnes_hut.rs:24
+
+
+def initial_positions(n: int, seed: int, dim: int) -> np.ndarray:
+    out = np.zeros(max(n, 1) * dim, np.float64)
+    lib().gro_initial_positions(n, seed, dim, out)
+    return out[: n * dim].reshape(n, dim)
+
+
+def jiggle(seed, tick, pss, i, j) -> float:
+    return lib().gro_jiggle(seed, tick, pss, i, j)
+
+
+def fmix64(k: int) -> int:
+    return lib().gro_fmix64(k)
+
+
+def mulberry32(seed: int, count: int) -> np.ndarray:
+    st = C.c_uint32(seed)
+    return np.array([lib().gro_mulberry32_next(C.byref(st)) for _ in range(count)])
+
+
+def gate_edges(count: int):
+    """Raw (source, target) of the reference gate model seeded_model(*, count, 8)."""
+    cap = 2 * count + count // 20 + 1
+    s = np.zeros(cap, np.uint32)
+    t = np.zeros(cap, np.uint32)
+    m = lib().gro_gate_edges(count, s, t, cap)
+    return s[:m].copy(), t[:m].copy()
+
+
+class Fa2:
+    """The reference Fa2State, dense (repulsion=0) or Barnes-Hut (repulsion=1)."""
+
+    def __init__(self, n, source, target, *, dim=2, jitter_tolerance=1.0, scaling_ratio=2.0,
+                 gravity=1.0, seed=0, start=None, weights=None, tree=False,
+                 theta2=THETA2_REFERENCE):
+        self.n, self.dim = int(n), int(dim)
+        src = np.ascontiguousarray(source, np.uint32)
+        dst = np.ascontiguousarray(target, np.uint32)
+        self._keep = []
+        wp = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, np.float64)
+            self._keep.append(w)
+            wp = w.ctypes.data
+        sp = None
+        if start is not None:
+            st = np.ascontiguousarray(start, np.float64).reshape(-1)
+            assert st.size == self.n * self.dim
+            self._keep.append(st)
+            sp = st.ctypes.data
+        if src.size == 0:
+            src = np.zeros(1, np.uint32)
+            dst = np.zeros(1, np.uint32)
+            m = 0
+        else:
+            m = src.size
+        self._h = lib().gro_fa2_new(self.n, src, dst, m, wp, self.dim, jitter_tolerance,
+                                   scaling_ratio, gravity, seed, sp, 1 if tree else 0, theta2)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().gro_fa2_free(h)
+            self._h = None
+
+    def iterate(self) -> float:
+        return lib().gro_fa2_iterate(self._h)
+
+    def run(self, max_iter: int) -> int:
+        return lib().gro_fa2_run(self._h, max_iter)
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1) * self.dim)
+        lib().gro_fa2_positions(self._h, out)
+        return out[: self.n * self.dim].reshape(self.n, self.dim)
+
+    def set_positions(self, p):
+        p = np.ascontiguousarray(p, np.float64).reshape(-1)
+        lib().gro_fa2_set_positions(self._h, p)
+
+    def mass(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1))
+        lib().gro_fa2_mass(self._h, out)
+        return out[: self.n]
+
+    def forces(self, parts: int = 7) -> np.ndarray:
+        out = np.zeros(max(self.n, 1) * self.dim)
+        lib().gro_fa2_forces(self._h, parts, out)
+        return out[: self.n * self.dim].reshape(self.n, self.dim)
+
+    def scalars(self) -> np.ndarray:
+        out = np.zeros(5)
+        lib().gro_fa2_scalars(self._h, out)
+        return out
+
+    def set_repulsion(self, tree: bool, theta2: float = THETA2_REFERENCE):
+        lib().gro_fa2_set_theta2(self._h, theta2, 1 if tree else 0)
+
+    def simple_edges(self):
+        m = lib().gro_fa2_simple_edges(self._h, None, None)
+        lo = np.zeros(max(m, 1), np.uint32)
+        hi = np.zeros(max(m, 1), np.uint32)
+        lib().gro_fa2_simple_edges(self._h, lo.ctypes.data, hi.ctypes.data)
+        return lo[:m], hi[:m]
+
+
+class Tree:
+    """The reference's preorder quadtree (dim 2) or octree (dim 3) over given points."""
+
+    def __init__(self, pts: np.ndarray):
+        pts = np.asarray(pts, np.float64)
+        self.n, self.dim = pts.shape
+        self._cols = [np.ascontiguousarray(pts[:, a]) for a in range(self.dim)]
+        if self.n == 0:
+            self._cols = [np.zeros(1) for _ in range(self.dim)]
+        zs = self._cols[2].ctypes.data if self.dim == 3 else None
+        self._h = lib().gro_tree_build(self.n, self.dim, self._cols[0], self._cols[1], zs)
+        nc = lib().gro_tree_ncells(self._h)
+        self.ncells = nc
+        b = np.zeros(max(nc, 1) * 2 * self.dim)
+        sk = np.zeros(max(nc, 1), np.uint32)
+        s0 = np.zeros(max(nc, 1), np.uint32)
+        s1 = np.zeros(max(nc, 1), np.uint32)
+        lib().gro_tree_cells(self._h, b, sk, s0, s1)
+        self.bounds = b[: nc * 2 * self.dim].reshape(nc, 2 * self.dim)
+        self.skip, self.start, self.end = sk[:nc], s0[:nc], s1[:nc]
+        o = np.zeros(max(self.n, 1), np.uint32)
+        lib().gro_tree_order(self._h, o)
+        self.order = o[: self.n]
+
+    def bodies(self, mass: np.ndarray) -> np.ndarray:
+        m = np.ascontiguousarray(mass, np.float64)
+        out = np.zeros(max(self.ncells, 1) * (1 + self.dim))
+        zs = self._cols[2].ctypes.data if self.dim == 3 else None
+        lib().gro_tree_bodies(self._h, m, self._cols[0], self._cols[1], zs, out)
+        return out[: self.ncells * (1 + self.dim)].reshape(self.ncells, 1 + self.dim)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().gro_tree_free(h)
+            self._h = None

@@ -1,0 +1,278 @@
+"""GPU parity of the HIP FA2 path against the CPU oracle, through the C ABI.
+
+Bars: bit-exact for the tree (keys, order, topology, bounds) and for every FP64 force part
+whose reduction order the reference fixes per body (attraction, tree repulsion, dense
+repulsion, centres of mass); within 1e-12 where a global sum (centroid, swing, traction)
+enters; FP32 forces within 1e-5 of FP64 (both of barnes_hut/tests.rs:34-47's metrics).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graph_render_amd as G
+from tests.refgraphs import bh_graph, errors, line
+from tests.test_oracle import nx_cases, gap, tricky_points
+
+pytestmark = pytest.mark.gpu
+
+THETA_REF = 0.8
+
+
+def session(n, s, t, dim=2, start=None, theta=None, precision="fp64", repulsion="tree",
+            weights=None, export=True, **params):
+    p = G.Fa2Params(dim=dim, **params)
+    x = G.Fa2Ext(theta=theta, precision=precision, repulsion=repulsion, start=start,
+                 edge_weights=weights, export_tree=export)
+    return G.Session(G.Topology(n, s, t), p, x)
+
+
+def oracle_state(oracle, n, s, t, dim=2, start=None, tree=True, theta=THETA_REF,
+                 weights=None, **params):
+    return oracle.Fa2(n, s, t, dim=dim, start=start, tree=tree, theta2=theta * theta,
+                      weights=weights, **params)
+
+
+# ------------------------------------------------------------------ tree
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", range(12))
+def test_tree_is_bit_exact(oracle, seed, dim):
+    pts = tricky_points(seed, 200 + 37 * seed, dim)
+    n = pts.shape[0]
+    print(f"tree case seed={seed} dim={dim} n={n}", flush=True)
+    s, t = np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    with session(n, s, t, dim=dim, start=pts) as sess:
+        sess.forces(2)
+        tr = sess.tree()
+    ref = oracle.Tree(pts)
+    assert tr["ncells"] == ref.ncells
+    assert np.array_equal(tr["order"], ref.order)
+    assert np.array_equal(tr["skip"], ref.skip)
+    assert np.array_equal(tr["start"], ref.start)
+    assert np.array_equal(tr["end"], ref.end)
+    assert np.array_equal(tr["bounds"], ref.bounds)
+    mass = np.ones(n)
+    rb = ref.bodies(mass)
+    assert np.array_equal(tr["bodies"], rb, equal_nan=True)
+
+
+def test_tree_on_a_layout_is_bit_exact(oracle):
+    n = 20000
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t)
+    ref.run(3)
+    p = ref.positions()
+    with session(n, s, t, start=p) as sess:
+        sess.forces(2)
+        tr = sess.tree()
+    rt = oracle.Tree(p)
+    assert tr["ncells"] == rt.ncells
+    assert np.array_equal(tr["order"], rt.order)
+    assert np.array_equal(tr["skip"], rt.skip)
+    assert np.array_equal(tr["bounds"], rt.bounds)
+    assert np.array_equal(tr["bodies"], rt.bodies(ref.mass()))
+
+
+# ------------------------------------------------------------------ forces
+
+
+@pytest.mark.parametrize("n", [2, 3, 61, 600, 5000])
+def test_force_parts_are_bit_exact(oracle, n):
+    s, t = oracle.gate_edges(n)
+    ref = oracle_state(oracle, n, s, t)
+    ref.run(4)
+    p = ref.positions()
+    with session(n, s, t, start=p) as sess:
+        for parts in (1, 2, 3):
+            g = sess.forces(parts)
+            r = ref.forces(parts)
+            assert np.array_equal(g, r), (n, parts, np.abs(g - r).max())
+        g = sess.forces(7)
+    r = ref.forces(7)
+    scale = np.abs(r).max()
+    assert np.abs(g - r).max() <= 1e-12 * scale
+
+
+def test_dense_repulsion_is_bit_exact(oracle):
+    n = 700
+    s, t = oracle.gate_edges(n)
+    ref = oracle_state(oracle, n, s, t, tree=False)
+    ref.run(3)
+    p = ref.positions()
+    with session(n, s, t, start=p, repulsion="dense") as sess:
+        for parts in (1, 2, 3):
+            assert np.array_equal(sess.forces(parts), ref.forces(parts)), parts
+
+
+def test_weighted_attraction_is_bit_exact(oracle):
+    n = 3000
+    s, t = bh_graph(n)
+    w = np.random.default_rng(3).lognormal(0.0, 0.5, s.size)
+    ref = oracle_state(oracle, n, s, t, weights=w)
+    with session(n, s, t, start=ref.positions(), weights=w) as sess:
+        assert np.array_equal(sess.forces(1), ref.forces(1))
+        assert np.array_equal(sess.forces(3), ref.forces(3))
+
+
+def test_theta_zero_equals_dense(oracle):
+    """barnes_hut/tests.rs:61-70 on the device."""
+    n = 400
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t, tree=False)
+    ref.run(5)
+    exact = ref.forces(2)
+    with session(n, s, t, start=ref.positions(), theta=0.0) as sess:
+        tree = sess.forces(2)
+    node, total = errors(tree, exact)
+    assert node < 1e-12 and total < 1e-12
+
+
+@pytest.mark.parametrize("n,its", [(1500, 0), (1500, 30), (1500, 100), (5000, 100)])
+def test_theta_bounds_on_device(oracle, n, its):
+    """barnes_hut/tests.rs:72-88: dense state, tree repulsion within 2% / 5%."""
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t, tree=False)
+    ref.run(its)
+    exact = ref.forces(2)
+    with session(n, s, t, start=ref.positions(), repulsion="dense") as dense:
+        assert np.array_equal(dense.forces(2), exact)
+    with session(n, s, t, start=ref.positions()) as sess:
+        node, total = errors(sess.forces(2), exact)
+    assert node < 0.02 and total < 0.05
+
+
+# ------------------------------------------------------------------ trajectories
+
+
+@pytest.mark.parametrize("tree", [True, False])
+def test_two_iterations_track_the_oracle(oracle, tree):
+    n = 2000
+    s, t = oracle.gate_edges(n)
+    ref = oracle_state(oracle, n, s, t, tree=tree)
+    with session(n, s, t, repulsion="tree" if tree else "dense") as sess:
+        assert np.array_equal(sess.positions(), ref.positions())  # Mulberry32 starts
+        for it in range(2):
+            sess.step(1)
+            ref.iterate()
+            g, r = sess.positions(), ref.positions()
+            ext = (r.max(0) - r.min(0)).max()
+            assert np.abs(g - r).max() <= 1e-12 * ext, it
+        sc = sess.scalars()
+    rs = ref.scalars()
+    for k, v in zip(("swing", "traction", "speed", "speed_efficiency"), rs[:4]):
+        assert abs(sc[k] - v) <= 1e-12 * abs(v), k
+    assert sc["iter_no"] == rs[4] == 2
+
+
+def test_dense_stage_matches_networkx_vectors(oracle):
+    """cli_fa2.rs:21-28 through gr_layout_run("layout.forceatlas2")."""
+    worst = 0.0
+    for c in nx_cases():
+        n = c["n"]
+        s, t = oracle.gate_edges(n)
+        geo = G.ForceAtlas2.run(G.Topology(n, s, t), G.Fa2Params(max_iter=c["max_iter"]))
+        ours = np.column_stack([geo.x, geo.y]).astype(np.float64)
+        worst = max(worst, gap(ours, c))
+    assert worst <= 1e-7, worst
+
+
+def test_barnes_hut_stage_is_repeatable_and_finite():
+    s, t = bh_graph(300)
+    topo = G.Topology(300, s, t)
+    a = G.ForceAtlas2BarnesHut.run(topo)
+    b = G.ForceAtlas2BarnesHut.run(topo)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    assert np.isfinite(a.x).all() and np.isfinite(a.y).all()
+
+
+def test_barnes_hut_stage_tracks_the_oracle_over_100_iterations(oracle):
+    """Short graphs stay on the oracle's trajectory to within float32 narrowing."""
+    s, t = line(20)
+    geo = G.ForceAtlas2BarnesHut.run(G.Topology(20, s, t))
+    ref = oracle_state(oracle, 20, s, t)
+    ref.run(100)
+    r = ref.positions().astype(np.float32)
+    ext = (r.max(0) - r.min(0)).max()
+    assert np.abs(np.column_stack([geo.x, geo.y]) - r).max() <= 1e-5 * ext
+
+
+def test_early_exit_matches(oracle):
+    s, t = line(2)
+    with session(2, s, t) as sess:
+        done = sess.step(100)
+    ref = oracle_state(oracle, 2, s, t)
+    assert done == ref.run(100)
+
+
+def test_3d_dense_is_bit_exact(oracle):
+    n = 500
+    s, t = oracle.gate_edges(n)
+    ref = oracle_state(oracle, n, s, t, dim=3, tree=False)
+    ref.run(2)
+    with session(n, s, t, dim=3, start=ref.positions(), repulsion="dense") as sess:
+        assert np.array_equal(sess.forces(3), ref.forces(3))
+
+
+def test_3d_octree_forces_are_bit_exact(oracle):
+    n = 4000
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t, dim=3)
+    ref.run(2)
+    with session(n, s, t, dim=3, start=ref.positions()) as sess:
+        assert np.array_equal(sess.forces(2), ref.forces(2))
+
+
+# ------------------------------------------------------------------ precision
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32_kahan"])
+def test_fp32_forces_within_1e5_of_fp64(oracle, precision):
+    n = 30000
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t)
+    ref.run(10)
+    p = ref.positions().astype(np.float32).astype(np.float64)
+    with session(n, s, t, start=p) as s64:
+        f64 = s64.forces(7)
+    with session(n, s, t, start=p, precision=precision) as s32:
+        f32 = s32.forces(7)
+    node, total = errors(f32, f64)
+    print(precision, "per-node rms", node, "whole-field", total)
+    assert total <= 1e-5 and node <= 1e-5, (node, total)
+
+
+# ------------------------------------------------------------------ edges
+
+
+def test_empty_and_single_node():
+    e = np.zeros(0, np.uint32)
+    geo = G.ForceAtlas2BarnesHut.run(G.Topology(0, e, e))
+    assert geo.x.size == 0
+    geo = G.ForceAtlas2BarnesHut.run(G.Topology(1, e, e))
+    assert np.isfinite(geo.x).all() and np.isfinite(geo.y).all()
+
+
+def test_all_points_coincident(oracle):
+    n = 50
+    s, t = line(n)
+    start = np.tile([[0.5, 0.25]], (n, 1))
+    ref = oracle_state(oracle, n, s, t, start=start)
+    with session(n, s, t, start=start) as sess:
+        assert np.array_equal(sess.forces(2), ref.forces(2))
+        tr = sess.tree()
+    assert tr["ncells"] == 1
+
+
+def test_params_are_refused_not_clamped():
+    import struct
+    s, t = line(5)
+    topo = G.Topology(5, s, t)
+    with pytest.raises(G.ParamError):
+        G.run_params("layout.forceatlas2.barnes_hut", topo, struct.pack("<5d", 0, 1, 2, 1, 0))
+    with pytest.raises(G.StageError):
+        G.run_params("layout.nope", topo, b"")
+    geo = G.run_params("layout.forceatlas2.barnes_hut", topo, b"")
+    assert geo.x.size == 5

@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""FA2-BH iterations/s on one MI355X: 1M-node / ~5M-edge stochastic block model, 2D, FP32
+forces with FP64 swing/traction reductions, theta 1.2 (the large Barnes-Hut workload).
+
+Prints one JSON line. `value` is FP32 iterations/s with the graph and state already
+resident in HBM; the same line carries FP64 iterations/s, the FP32-vs-FP64 force error
+(per-node rms and whole-field, as the reference's BH tests measure it), the walk kernel's
+roofline position (HIP events on the session stream) and the CPU oracle's rate on a
+bounded sample of the same workload (one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FA2-BH iters/sec @1M nodes/5M edges, FP32 vs FP64, force err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def errors(approx, exact):
+    d2 = ((approx - exact) ** 2).sum(1)
+    e2 = (exact ** 2).sum(1)
+    ok = e2 > 0
+    return float(np.sqrt((d2[ok] / e2[ok]).mean())), float(np.sqrt(d2.sum() / e2.sum()))
+
+
+def walk_bytes(n: int, ncells: int, precision: str) -> int:
+    """Compulsory bytes of one walk launch: per body its sorted position and mass, its
+    point index and its force write; per cell one read of its walk record."""
+    es = 8 if precision == "fp64" else 4
+    cell = 48 if precision == "fp64" else 32
+    return n * (3 * es + 4 + 2 * es) + ncells * cell
+
+
+def cpu_baseline(topo, params, start, weights, budget_s: float):
+    """The oracle (the reference's FA2-BH restated in C, FP64, one thread) on the same graph
+    and start, iterated until about budget_s seconds of CPU time are spent."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    st = O.Fa2(topo.n, topo.source, topo.target, start=start, weights=weights, tree=True,
+               theta2=1.2 * 1.2, scaling_ratio=params.scaling_ratio, gravity=params.gravity,
+               jitter_tolerance=params.jitter_tolerance)
+    setup = time.perf_counter() - t0
+    its, t0 = 0, time.perf_counter()
+    while True:
+        st.iterate()
+        its += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or its >= 50:
+            break
+    return {"value": its / el, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"{its} FA2-BH FP64 iterations of the same 1M-node graph and start "
+                      f"(oracle/fa2_oracle.c, gcc -O2, one thread; {setup:.1f}s setup untimed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nodes", type=int, default=1_000_000)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import graph_render_amd as G
+    from graph_render_amd import synth
+
+    n = args.nodes
+    s, t, w = synth.sbm_zipf(n, 64, 1.1, 10.0, seed=1)
+    start = synth.uniform_start(n, 2, seed=1)
+    topo = G.Topology(n, s, t)
+    params = G.Fa2Params(max_iter=500, scaling_ratio=2.0, gravity=1.0, jitter_tolerance=1.0)
+
+    def make(precision):
+        return G.Session(topo, params, G.Fa2Ext(theta=1.2, precision=precision, start=start,
+                                                 edge_weights=w))
+
+    # FP32 (the headline)
+    s32 = make("fp32")
+    m_simple = len(s32.graph()[0])
+    s32.step(args.warmup)
+    t0 = time.perf_counter()
+    done = s32.step(args.steps)
+    el = time.perf_counter() - t0
+    assert done == args.steps, (done, args.steps)
+    fp32_its = done / el
+    # walk roofline on the current tree
+    walk_ms = s32.time("walk", 10)
+    tree_ms = s32.time("tree", 5)
+    ncells = s32.tree(bounds=False)["ncells"]
+    wb = walk_bytes(n, ncells, "fp32")
+    achieved = wb / (walk_ms * 1e-3) / 1e9
+    pos32 = s32.positions()
+    f32 = s32.forces(7)
+    s32.close()
+
+    # FP64 on the same start
+    s64 = make("fp64")
+    s64.step(2)
+    t0 = time.perf_counter()
+    d64 = s64.step(args.fp64_steps)
+    fp64_its = d64 / (time.perf_counter() - t0)
+    s64.set_positions(pos32)
+    f64 = s64.forces(7)
+    s64.close()
+    node_err, field_err = errors(f32, f64)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(topo, params, start, w, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(fp32_its, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / fp32_its, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "FA2-BH, 2D, SBM 1M nodes / 64 Zipf(1.1) blocks / mean degree 10, "
+                        "lognormal(0,0.5) weights, seed 1, theta 1.2, FP32 forces + FP64 "
+                        "reductions",
+            "nodes": n, "raw_edges": int(s.size), "simple_edges": int(m_simple),
+            "edges_sha256": synth.digest(s, t),
+        },
+        "fp64_iters_per_s": round(fp64_its, 3),
+        "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err},
+        "walk_ms": round(walk_ms, 4),
+        "tree_build_ms": round(tree_ms, 4),
+        "tree_cells": int(ncells),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_walk<float,2>",
+                     "algorithmic_bytes": int(wb)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

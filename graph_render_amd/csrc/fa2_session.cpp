@@ -185,7 +185,9 @@ struct gr_fa2_session {
     // After a halted iteration: grow what ran out, clear the flag. The halted iteration
     // applied nothing, so it is simply run again.
     gr_status recover(const GrState &st) {
+        bool grew = false;
         if (st.pool_overflow) {
+            grew = true;
             uint32_t grow = d.pool_cap * 2;
             void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
             HIP_TRY(hipMalloc(&p1, (size_t)grow * 4));
@@ -201,12 +203,12 @@ struct gr_fa2_session {
             d.lcell_cap *= 2;
         }
         if (st.need_cells > d.cap) {
+            grew = true;
             uint32_t want = st.need_cells + st.need_cells / 4 + 64;
             gr_status s = alloc_cells(want);
             if (s) return s;
         }
-        if (!st.pool_overflow && st.need_cells <= d.cap)
-            return fail(GR_ERR_DEVICE, "halt", "iteration halted without a capacity cause");
+        if (!grew) return fail(GR_ERR_DEVICE, "halt", "iteration halted without a capacity cause");
         gr::launch_clear_halt(d);
         return GR_OK;
     }

@@ -36,7 +36,7 @@ def walk_bytes(n: int, ncells: int, precision: str) -> int:
     """Compulsory bytes of one walk launch: per body its sorted position and mass, its
     point index and its force write; per cell one read of its walk record."""
     es = 8 if precision == "fp64" else 4
-    cell = 48 if precision == "fp64" else 32
+    cell = 48 if precision == "fp64" else 36  # FP32 cells carry a float-pair centre
     return n * (3 * es + 4 + 2 * es) + ncells * cell
 
 
@@ -108,6 +108,16 @@ def main():
     f32 = s32.forces(7)
     s32.close()
 
+    # FP32 with compensated accumulation, same start and steps
+    sk = make("fp32_kahan")
+    sk.step(args.warmup)
+    t0 = time.perf_counter()
+    dk = sk.step(args.steps)
+    kahan_its = dk / (time.perf_counter() - t0)
+    sk.set_positions(pos32)
+    fk = sk.forces(7)
+    sk.close()
+
     # FP64 on the same start
     s64 = make("fp64")
     s64.step(2)
@@ -118,6 +128,7 @@ def main():
     f64 = s64.forces(7)
     s64.close()
     node_err, field_err = errors(f32, f64)
+    knode_err, kfield_err = errors(fk, f64)
 
     cpu = None
     if not args.no_cpu:
@@ -145,6 +156,9 @@ def main():
         },
         "fp64_iters_per_s": round(fp64_its, 3),
         "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err},
+        "fp32_kahan_iters_per_s": round(kahan_its, 3),
+        "force_err_fp32_kahan_vs_fp64": {"per_node_rms": knode_err,
+                                         "whole_field": kfield_err},
         "walk_ms": round(walk_ms, 4),
         "tree_build_ms": round(tree_ms, 4),
         "tree_cells": int(ncells),

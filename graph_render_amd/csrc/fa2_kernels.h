@@ -28,6 +28,7 @@ struct FaDev {
 
     // state, element type T (double or float) chosen by precision
     void *pos[3] = {};
+    void *pi = nullptr;  // positions again, interleaved (x, y[, z]) per node, for gathers
     void *mass = nullptr;
     void *u[3] = {};
     void *rep[3] = {};
@@ -48,8 +49,9 @@ struct FaDev {
     uint64_t *gkey = nullptr;
     uint32_t cap = 0;  // cell capacity
     void *wc = nullptr;
-    int32_t *parent = nullptr;
-    uint32_t *cnt = nullptr;
+    uint8_t *clevel = nullptr;  // cell depth; 255 for cells summed by the deep builder
+    uint32_t *span = nullptr;   // cells whose subtree crosses a centre-of-mass tile
+    uint32_t *span_n = nullptr; // per tile: how many of its span slots are used
     double *raw = nullptr;      // unnormalised (mass, m*x...) per cell
     double *cbounds = nullptr;  // 2*dim per cell when the session exports the tree
     int export_bounds = 0;
@@ -59,6 +61,7 @@ struct FaDev {
     int32_t *pool = nullptr;
     int32_t *lc_int = nullptr;
     double *lc_dbl = nullptr;
+    double *lc_sum = nullptr;  // unnormalised sums of local cells, (1 + dim) each
     int32_t *chain_next = nullptr;
 
     // reductions

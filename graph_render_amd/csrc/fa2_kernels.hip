@@ -14,6 +14,8 @@
 // so the tree topology, the centres and the per-body force sums match it bit for bit.
 #include <math.h>
 
+#include <type_traits>
+
 #include "fa2_kernels.h"
 #include "sort_scan.h"
 
@@ -28,8 +30,18 @@ struct KeyFmt {
     static constexpr int USED = D * LEVELS;
 };
 
+// The FP32 precisions keep each centre as an unevaluated pair hi + lo of floats (about 48
+// bits), so the offset of a nearby body from it is exact to float rounding even where
+// coordinates are large and the cell is small. FP64 keeps the reference's one double.
 template <typename T, int D>
-struct WalkCell {
+struct CentreLo {
+    T clo[D];
+};
+template <int D>
+struct CentreLo<double, D> {};
+
+template <typename T, int D>
+struct WalkCell : CentreLo<T, D> {
     T c[D];
     T mass;
     T w2;
@@ -249,7 +261,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_bounds(const FaDev d) {
     for (uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * GR_BLOCK) {
         double p[D];
 #pragma unroll
-        for (int a = 0; a < D; a++) p[a] = (double)P[a][i];
+        for (int a = 0; a < D; a++) {
+            p[a] = (double)P[a][i];
+            ((T *)d.pi)[(size_t)i * D + a] = P[a][i];
+        }
         bounds_take<D>(p, lo, hi, sum, nfin);
     }
     bounds_write<D>(lo, hi, sum, nfin, d.bpart);
@@ -311,6 +326,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
     block_reduce<3, 2>(vmax, lds);
     block_reduce<4, 0>(vsum, lds);
     if (threadIdx.x == 0) {
+        st->pool_used = 0;
+        st->lcell_used = 0;
+        st->span_count = 0;
+        st->any_deep = 0;
         uint32_t nfin = (uint32_t)vsum[3];
         st->n_finite = nfin;
         st->has_root = nfin > 0;
@@ -593,6 +612,30 @@ __device__ bool local_build(const FaDev &d, uint32_t s, uint32_t e, Box<D> rootb
             }
         }
     }
+    // the local subtree's sums, in the reference's aggregate order (reverse preorder, a
+    // leaf over its chain, an internal cell over its children in slot order)
+    for (uint32_t c = nshapes; c-- > 0;) {
+        const int32_t *ik = d.lc_int + (size_t)(lc + c) * 4;
+        double sum[1 + D];
+        for (int a = 0; a <= D; a++) sum[a] = 0.0;
+        if ((uint32_t)ik[0] == c + 1) {
+            for (int32_t q = ik[1]; q < ik[2]; q++) {
+                double m = (double)((const T *)d.sm)[q];
+                sum[0] += m;
+                for (int a = 0; a < D; a++) sum[1 + a] += m * (double)((const T *)d.sp[a])[q];
+            }
+        } else {
+            uint32_t ch = c + 1;
+            for (int q = 0; q < (1 << D) && ch < (uint32_t)ik[0]; q++) {
+                const double *cs = d.lc_sum + (size_t)(lc + ch) * (1 + D);
+                for (int a = 0; a <= D; a++) sum[a] += cs[a];
+                uint32_t nx = (uint32_t)d.lc_int[(size_t)(lc + ch) * 4];
+                ch = nx > ch ? nx : (uint32_t)ik[0];
+            }
+        }
+        double *o = d.lc_sum + (size_t)(lc + c) * (1 + D);
+        for (int a = 0; a <= D; a++) o[a] = sum[a];
+    }
     *out_lc = lc;
     *out_count = nshapes;
     return true;
@@ -660,12 +703,6 @@ __global__ void k_cells_ready(const FaDev d) {
     st->n_cells = st->need_cells;
 }
 
-__global__ __launch_bounds__(GR_BLOCK) void k_zero_cnt(const FaDev d) {
-    const GrState *st = d.st;
-    if (halted(d)) return;
-    uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (k < st->n_cells) d.cnt[k] = 0;
-}
 
 // The last group h >= g whose key shares `level` levels with group g's.
 template <int D>
@@ -709,17 +746,31 @@ __device__ uint32_t gallop_bwd(const uint64_t *gkey, uint32_t g, int level) {
 }
 
 template <typename T, int D>
+__device__ __forceinline__ void body_final(const FaDev &d, uint32_t k, const double *s) {
+    WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
+    double *r = d.raw + (size_t)k * (1 + D);
+#pragma unroll
+    for (int a = 0; a <= D; a++) r[a] = s[a];
+    wc[k].mass = (T)s[0];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const double c = s[1 + a] / s[0];
+        wc[k].c[a] = (T)c;
+        if constexpr (!std::is_same<T, double>::value) wc[k].clo[a] = (T)(c - (double)(T)c);
+    }
+}
+
+template <typename T, int D>
 __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<D> &b,
                                           uint32_t skip, uint32_t start, uint32_t end,
-                                          int32_t parent) {
+                                          uint32_t level) {
     WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
     double w = b.hi[0] - b.lo[0];
     wc[k].w2 = (T)(w * w);
     wc[k].skip = skip;
     wc[k].start = start;
     wc[k].end = end;
-    d.parent[k] = parent;
-    if (parent >= 0) atomicAdd(&d.cnt[parent], 1u);
+    d.clevel[k] = (uint8_t)level;
     if (d.export_bounds) {
         double *o = d.cbounds + (size_t)k * 2 * D;
 #pragma unroll
@@ -751,12 +802,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     uint32_t gend = g + 1 < G ? d.gstart[g + 1] : nf;
     uint32_t gskip = g + 1 < G ? d.gbase[g + 1] : C;
     uint32_t idx = d.gbase[g];
-    int32_t prev = -1;
-    if (left >= 0) {
-        uint32_t f = gallop_bwd<D>(gkey, g, left);
-        int left_f = f > 0 ? key_lcp<D>(gkey[f - 1], gkey[f]) : -1;
-        prev = (int32_t)(d.gbase[f] + (uint32_t)(left - left_f - 1));
-    }
     Box<D> b = root_box<D>(st);
     int top = type == G_DEEP ? L - 1 : level;
     for (int l = 0; l <= top; l++) {
@@ -769,11 +814,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
                 skip = h + 1 < G ? d.gbase[h + 1] : C;
                 end = h + 1 < G ? d.gstart[h + 1] : nf;
             }
-            emit_cell<T, D>(d, idx, b, skip, start, end, prev);
+            emit_cell<T, D>(d, idx, b, skip, start, end, (uint32_t)l);
         } else {
-            emit_cell<T, D>(d, idx, b, idx + 1, start, gend, prev);
+            emit_cell<T, D>(d, idx, b, idx + 1, start, gend, (uint32_t)l);
         }
-        prev = (int32_t)idx;
         idx++;
     }
     if (type == G_DEEP) {
@@ -789,76 +833,203 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
                 lb.lo[a] = bk[a];
                 lb.hi[a] = bk[D + a];
             }
-            int32_t par = ik[3] >= 0 ? (int32_t)(base + (uint32_t)ik[3]) : prev;
             emit_cell<T, D>(d, base + c, lb, base + (uint32_t)ik[0], (uint32_t)ik[1],
-                            (uint32_t)ik[2], par);
+                            (uint32_t)ik[2], 255u);
+            body_final<T, D>(d, base + c, d.lc_sum + (size_t)(lc + c) * (1 + D));
         }
     }
 }
 
 // ------------------------------------------------------------------ centre of mass
 
-template <typename T, int D>
-__device__ __forceinline__ void body_final(const FaDev &d, uint32_t k, const double *s) {
-    WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
-    double *r = d.raw + (size_t)k * (1 + D);
-#pragma unroll
-    for (int a = 0; a <= D; a++) r[a] = s[a];
-    wc[k].mass = (T)s[0];
-#pragma unroll
-    for (int a = 0; a < D; a++) wc[k].c[a] = (T)(s[1 + a] / s[0]);
-}
 
-// barnes_hut.rs:62-110. A leaf sums its chain; the last child to finish sums its parent's
-// children in slot order (k+1, then each child's skip), so every sum is in the reference's
-// order whichever lane gets there last. Sums are FP64 in every precision.
+// barnes_hut.rs:62-110: a leaf sums its chain, an internal cell its children's
+// unnormalised sums in slot order (k+1, then each child's skip). Sums are FP64 in every
+// precision, and each cell's sum is formed exactly as the reference forms it, so the
+// order in which cells are visited moves no bit.
 template <typename T, int D>
-__global__ __launch_bounds__(GR_BLOCK) void k_com(const FaDev d) {
-    const GrState *st = d.st;
-    if (halted(d)) return;
-    uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (k >= st->n_cells) return;
+__device__ __forceinline__ void com_cell(const FaDev &d, uint32_t k) {
     const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
-    uint32_t skip = wc[k].skip;
-    if (skip != k + 1) return;
+    const uint32_t skip = wc[k].skip;
     double s[1 + D];
 #pragma unroll
     for (int a = 0; a <= D; a++) s[a] = 0.0;
-    const uint32_t qend = wc[k].end < d.n ? wc[k].end : d.n;
-    for (uint32_t q = wc[k].start; q < qend; q++) {
-        double m = (double)((const T *)d.sm)[q];
-        s[0] += m;
+    if (skip == k + 1) {
+        const uint32_t qend = wc[k].end < d.n ? wc[k].end : d.n;
+        for (uint32_t q = wc[k].start; q < qend; q++) {
+            double m = (double)((const T *)d.sm)[q];
+            s[0] += m;
 #pragma unroll
-        for (int a = 0; a < D; a++) s[1 + a] += m * (double)((const T *)d.sp[a])[q];
-    }
-    body_final<T, D>(d, k, s);
-    uint32_t cur = k;
-    for (;;) {
-        int32_t par = d.parent[cur];
-        if (par < 0 || (uint32_t)par >= cur) break;  // parents precede children
-        __threadfence();
-        uint32_t old = __hip_atomic_fetch_add(&d.cnt[par], (uint32_t)-1, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 1) break;
-        __threadfence();
-#pragma unroll
-        for (int a = 0; a <= D; a++) s[a] = 0.0;
-        uint32_t pskip = __hip_atomic_load(&wc[par].skip, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t c = (uint32_t)par + 1;
-        // a cell has at most 2^D children and every skip points forward: the loop is
-        // bounded even on a malformed arena
-        for (int q = 0; q < (1 << D) && c < pskip; q++) {
+            for (int a = 0; a < D; a++) s[1 + a] += m * (double)((const T *)d.sp[a])[q];
+        }
+    } else {
+        uint32_t c = k + 1;
+        // at most 2^D children, every skip points forward: bounded on any arena
+        for (int q = 0; q < (1 << D) && c < skip; q++) {
             const double *r = d.raw + (size_t)c * (1 + D);
 #pragma unroll
-            for (int a = 0; a <= D; a++)
-                s[a] += __hip_atomic_load(&r[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t nc = __hip_atomic_load(&wc[c].skip, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-            c = nc > c ? nc : pskip;
+            for (int a = 0; a <= D; a++) s[a] += r[a];
+            uint32_t nc = wc[c].skip;
+            c = nc > c ? nc : skip;
         }
-        body_final<T, D>(d, (uint32_t)par, s);
-        cur = (uint32_t)par;
+    }
+    body_final<T, D>(d, k, s);
+}
+
+constexpr uint32_t COM_TILE = 4096;
+constexpr uint32_t SPAN_SLOTS = 32;   // spanning cells are ancestors of the tile's end: <= 1 per level
+constexpr uint32_t SPAN_LDS = 12288;  // span cells k_com_span sorts in LDS (more: plain loop)
+
+// One workgroup per tile of COM_TILE consecutive cells, its sums held in LDS. Leaves (and
+// cells the deep builder already summed) are read in one parallel pass; then every cell
+// whose subtree [k, skip) stays inside the tile is summed from its children's LDS sums,
+// deepest level first, a workgroup barrier between levels. A cell whose subtree leaves the
+// tile is an ancestor of the tile's last cell, at most one per level; it goes to the
+// tile's own span slots for k_com_span.
+constexpr uint32_t COM_THREADS = 1024;
+
+template <typename T, int D>
+__global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ double sums[COM_TILE][1 + D];
+    __shared__ uint32_t skips[COM_TILE];
+    __shared__ uint8_t lv[COM_TILE];
+    __shared__ uint32_t present, nspan;
+    const uint32_t C = st->n_cells;
+    const uint32_t t0 = blockIdx.x * COM_TILE;
+    if (t0 >= C) return;
+    const uint32_t t1 = t0 + COM_TILE < C ? t0 + COM_TILE : C;
+    const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
+    if (threadIdx.x == 0) {
+        present = 0;
+        nspan = 0;
+    }
+    __syncthreads();
+    uint32_t mask = 0;
+    uint32_t *slots = d.span + (size_t)blockIdx.x * SPAN_SLOTS;
+    for (uint32_t k = t0 + threadIdx.x; k < t1; k += COM_THREADS) {
+        const uint32_t i = k - t0;
+        uint32_t l = d.clevel[k];
+        const uint32_t skip = wc[k].skip;
+        skips[i] = skip;
+        if (l == 255) {  // summed by the deep builder
+            const double *r = d.raw + (size_t)k * (1 + D);
+#pragma unroll
+            for (int a = 0; a <= D; a++) sums[i][a] = r[a];
+        } else if (skip > t1) {
+            uint32_t at = atomicAdd(&nspan, 1u);  // LDS
+            if (at < SPAN_SLOTS) slots[at] = k;
+            l = 255;
+        } else if (skip == k + 1) {
+            double s[1 + D];
+#pragma unroll
+            for (int a = 0; a <= D; a++) s[a] = 0.0;
+            const uint32_t qend = wc[k].end < d.n ? wc[k].end : d.n;
+            for (uint32_t q = wc[k].start; q < qend; q++) {
+                double m = (double)((const T *)d.sm)[q];
+                s[0] += m;
+#pragma unroll
+                for (int a = 0; a < D; a++) s[1 + a] += m * (double)((const T *)d.sp[a])[q];
+            }
+#pragma unroll
+            for (int a = 0; a <= D; a++) sums[i][a] = s[a];
+            body_final<T, D>(d, k, s);
+            l = 255;
+        }
+        lv[i] = (uint8_t)l;
+        if (l < 32) mask |= 1u << l;
+    }
+    atomicOr(&present, mask);
+    __syncthreads();
+    if (threadIdx.x == 0) d.span_n[blockIdx.x] = nspan < SPAN_SLOTS ? nspan : SPAN_SLOTS;
+    const uint32_t levels = present;
+    for (int l = 31; l >= 0; l--) {
+        if (!(levels & (1u << l))) continue;  // block-uniform
+        for (uint32_t k = t0 + threadIdx.x; k < t1; k += COM_THREADS) {
+            const uint32_t i = k - t0;
+            if (lv[i] != (uint32_t)l) continue;
+            double s[1 + D];
+#pragma unroll
+            for (int a = 0; a <= D; a++) s[a] = 0.0;
+            const uint32_t skip = skips[i];
+            uint32_t c = k + 1;
+            for (int q = 0; q < (1 << D) && c < skip; q++) {
+                const uint32_t ci = c - t0;
+#pragma unroll
+                for (int a = 0; a <= D; a++) s[a] += sums[ci][a];
+                const uint32_t nc = skips[ci];
+                c = nc > c ? nc : skip;
+            }
+#pragma unroll
+            for (int a = 0; a <= D; a++) sums[i][a] = s[a];
+            body_final<T, D>(d, k, s);
+        }
+        __syncthreads();
+    }
+}
+
+// The cells whose subtree crosses a tile (the ancestors of tile ends), finished by one
+// workgroup level by level, bucketed by level in LDS.
+template <typename T, int D>
+__global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntiles) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ uint32_t cells[SPAN_LDS];
+    __shared__ uint32_t loff[34];
+    __shared__ uint32_t lcnt[33];
+    __shared__ uint32_t total_s;
+    const uint32_t C = st->n_cells;
+    ntiles = (C + COM_TILE - 1) / COM_TILE < ntiles ? (C + COM_TILE - 1) / COM_TILE : ntiles;
+    if (threadIdx.x < 33) lcnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) total_s = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+        uint32_t cnt = d.span_n[t];
+        atomicAdd(&total_s, cnt);
+        for (uint32_t q = 0; q < cnt; q++)
+            atomicAdd(&lcnt[d.clevel[d.span[t * SPAN_SLOTS + q]]], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = total_s;
+    if (total > SPAN_LDS) {
+        // very large trees: the same levels, straight from the tile slots
+        for (int l = 31; l >= 0; l--) {
+            if (lcnt[l] == 0) continue;
+            for (uint32_t t = threadIdx.x; t < ntiles * SPAN_SLOTS; t += blockDim.x) {
+                uint32_t tile = t / SPAN_SLOTS, q = t % SPAN_SLOTS;
+                if (q >= d.span_n[tile]) continue;
+                uint32_t k = d.span[t];
+                if (d.clevel[k] == (uint32_t)l) com_cell<T, D>(d, k);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int l = 0; l < 33; l++) {
+            loff[l] = acc;
+            acc += lcnt[l];
+        }
+        loff[33] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+        uint32_t cnt = d.span_n[t];
+        for (uint32_t q = 0; q < cnt; q++) {
+            uint32_t k = d.span[t * SPAN_SLOTS + q];
+            cells[atomicAdd(&lcnt[d.clevel[k]], 1u)] = k;
+        }
+    }
+    __syncthreads();
+    for (int l = 31; l >= 0; l--) {
+        const uint32_t a = loff[l], b = loff[l + 1];
+        if (a == b) continue;  // block-uniform
+        for (uint32_t q = a + threadIdx.x; q < b; q += blockDim.x) com_cell<T, D>(d, cells[q]);
+        __syncthreads();
     }
 }
 
@@ -874,7 +1045,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
     const uint32_t n = d.n;
-    const uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
+    const uint32_t lb = gr_xcd_block(blockIdx.x, (n + GR_BLOCK - 1) / GR_BLOCK);
+    const uint32_t p = lb * GR_BLOCK + threadIdx.x;
     if (__ballot(p < n) == 0) return;
     const bool live = p < n;
     const uint32_t pc = live ? p : n - 1;
@@ -905,7 +1077,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             const bool holds = cell.start <= p && p < cell.end;
             T dx[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) dx[a] = xi[a] - cell.c[a];
+            for (int a = 0; a < D; a++) {
+                dx[a] = xi[a] - cell.c[a];
+                if constexpr (!std::is_same<T, double>::value) dx[a] = dx[a] - cell.clo[a];
+            }
             T d2 = dx[0] * dx[0] + dx[1] * dx[1];
             if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
             if (!holds && d2 > (T)0 && cell.w2 < theta2 * d2) {
@@ -980,11 +1155,15 @@ __device__ __forceinline__ void attraction_of(const FaDev &d, uint32_t v, const 
     for (int a = 0; a < D; a++) acc[a] = c[a] = (T)0;
     const uint32_t e0 = d.row_off[v], e1 = d.row_off[v + 1];
     const T *w = (const T *)d.w;
+    const T *pi = (const T *)d.pi;
     for (uint32_t e = e0; e < e1; e++) {
         uint32_t j = d.nbr[e];
+        T pj[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) pj[a] = pi[(size_t)j * D + a];
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            T dd = ((const T *)d.pos[a])[j] - pv[a];
+            T dd = pj[a] - pv[a];
             if (w) dd = dd * w[e];
             acc_add<T, KAHAN>(acc[a], c[a], dd);
         }
@@ -1077,27 +1256,96 @@ __global__ __launch_bounds__(GR_BLOCK) void k_dense(const FaDev d, uint32_t with
     }
 }
 
+constexpr uint32_t FIN_STAGE = 4096;  // CSR entries staged in LDS per round
+
 // Completes u (attraction + repulsion, then gravity, state.rs:130-148 order) and the
 // swing/traction partials (schedule.rs:16-30). mode 0: repulsion from the walk (u = attr +
 // rep); mode 1: u already holds attraction (+ dense pairs).
+//
+// Work split: block b owns the rows whose (first entry + row index) falls in its share of
+// E + n, so rows and entries are balanced together; blocks are dealt so that one XCD owns one contiguous
+// range of rows (graph-local neighbours then share its L2), and each round stages up to
+// FIN_STAGE consecutive CSR entries in LDS with coalesced loads. A row is still summed by
+// one lane in ascending entry order, the reference's order.
 template <typename T, int D, bool KAHAN>
 __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mode,
                                                       uint32_t parts) {
     const GrState *st = d.st;
     if (halted(d)) return;
+    __shared__ uint32_t s_nbr[FIN_STAGE];
+    __shared__ T s_w[FIN_STAGE];
+    __shared__ uint32_t s_rows[2];
+    const uint32_t nb = gridDim.x;
+    const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
+    const uint64_t E = d.m2;
+    const uint32_t n = d.n;
+    if (threadIdx.x < 2) {
+        // balance rows + entries: the first row r with row_off[r] + r at or past the cut
+        const uint64_t cut = ((E + n) * (uint64_t)(lb + threadIdx.x)) / nb;
+        uint32_t lo = 0, hi = n;
+        if (lb + threadIdx.x >= nb) {
+            lo = n;
+        } else {
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo) / 2;
+                if ((uint64_t)d.row_off[mid] + mid < cut)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+        }
+        s_rows[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    // rows [r0, r1): r0 = first row starting at or after this block's cut (row 0 for the
+    // first block), r1 the same for the next block
+    const uint32_t r0 = lb == 0 ? 0 : s_rows[0];
+    const uint32_t r1 = lb + 1 >= nb ? n : s_rows[1];
     double sw = 0.0, tr = 0.0;
     const double grav = d.gravity;
-    for (uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x; v < d.n; v += gridDim.x * GR_BLOCK) {
-        T pv[D], uv[D];
+    const T *pi = (const T *)d.pi;
+    const T *wgt = (const T *)d.w;
+    for (uint32_t base = r0; base < r1; base += GR_BLOCK) {
+        const uint32_t rend = base + GR_BLOCK < r1 ? base + GR_BLOCK : r1;
+        const uint32_t v = base + threadIdx.x;
+        const bool live = v < rend;
+        T pv[D], uv[D], cc[D];
 #pragma unroll
-        for (int a = 0; a < D; a++) pv[a] = ((const T *)d.pos[a])[v];
-        if (mode == 0) {
-            if (parts & PART_ATTRACTION) {
-                attraction_of<T, D, KAHAN>(d, v, pv, uv);
-            } else {
+        for (int a = 0; a < D; a++) {
+            pv[a] = live ? pi[(size_t)v * D + a] : (T)0;
+            uv[a] = cc[a] = (T)0;
+        }
+        const bool attr = mode == 0 && (parts & PART_ATTRACTION);
+        uint32_t e = live ? d.row_off[v] : 0;
+        const uint32_t e1 = live ? d.row_off[v + 1] : 0;
+        if (attr) {
+            // rounds of FIN_STAGE entries covering [row_off[base], row_off[rend])
+            const uint32_t g0 = d.row_off[base], g1 = d.row_off[rend];
+            for (uint32_t s0 = g0; s0 < g1; s0 += FIN_STAGE) {
+                const uint32_t s1 = s0 + FIN_STAGE < g1 ? s0 + FIN_STAGE : g1;
+                __syncthreads();
+                for (uint32_t q = s0 + threadIdx.x; q < s1; q += GR_BLOCK) {
+                    s_nbr[q - s0] = d.nbr[q];
+                    if (wgt) s_w[q - s0] = wgt[q];
+                }
+                __syncthreads();
+                const uint32_t lim = e1 < s1 ? e1 : s1;
+                for (; e < lim; e++) {
+                    const uint32_t j = s_nbr[e - s0];
+                    T pj[D];
 #pragma unroll
-                for (int a = 0; a < D; a++) uv[a] = (T)0;
+                    for (int a = 0; a < D; a++) pj[a] = pi[(size_t)j * D + a];
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        T dd = pj[a] - pv[a];
+                        if (wgt) dd = dd * s_w[e - s0];
+                        acc_add<T, KAHAN>(uv[a], cc[a], dd);
+                    }
+                }
             }
+        }
+        if (!live) continue;
+        if (mode == 0) {
             if (parts & PART_REPULSION) {
 #pragma unroll
                 for (int a = 0; a < D; a++) uv[a] = uv[a] + ((const T *)d.rep[a])[v];
@@ -1122,17 +1370,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
                 uv[a] = (T)((double)uv[a] - g * unit);
             }
         }
-        double nb = 0.0, nf = 0.0;
+        double nbk = 0.0, nfw = 0.0;
 #pragma unroll
         for (int a = 0; a < D; a++) {
             ((T *)d.u[a])[v] = uv[a];
             double back = (double)pv[a] - (double)uv[a];
             double fwd = (double)pv[a] + (double)uv[a];
-            nb += back * back;
-            nf += fwd * fwd;
+            nbk += back * back;
+            nfw += fwd * fwd;
         }
-        sw += mv * sqrt(nb);
-        tr += 0.5 * mv * sqrt(nf);
+        sw += mv * sqrt(nbk);
+        tr += 0.5 * mv * sqrt(nfw);
     }
     __shared__ double lds[4 * 2];
     double v2[2] = {sw, tr};
@@ -1212,6 +1460,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
             T *P = (T *)d.pos[a];
             T np = (T)((double)P[v] + dd);
             P[v] = np;
+            ((T *)d.pi)[(size_t)v * D + a] = np;
             p[a] = (double)np;
             step += fabs(dd);
         }
@@ -1250,6 +1499,7 @@ __global__ void k_tree_reset(const FaDev d) {
     if (halted(d)) return;
     st->pool_used = 0;
     st->lcell_used = 0;
+    st->span_count = 0;
     st->any_deep = 0;
 }
 
@@ -1261,7 +1511,6 @@ static void tree_impl(FaDev &d) {
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
     hipStream_t s = d.stream;
     hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL(k_tree_reset, dim3(1), dim3(1), 0, s, d);
     hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, n, 64, d.sort_tmp, s);
     hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
@@ -1270,16 +1519,16 @@ static void tree_impl(FaDev &d) {
     hipLaunchKernelGGL((k_group_info<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.gcells, d.gbase, n, &d.st->n_groups, &d.st->need_cells, d.scan_tmp, s);
     hipLaunchKernelGGL(k_cells_ready, dim3(1), dim3(1), 0, s, d);
-    uint32_t cbl = gr_blocks(d.cap, GR_BLOCK);
-    hipLaunchKernelGGL(k_zero_cnt, dim3(cbl), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL((k_com<T, D>), dim3(cbl), dim3(GR_BLOCK), 0, s, d);
+    const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
+    hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
+    hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d, ntiles);
 }
 
 template <typename T, int D, bool KAHAN>
 static void walk_impl(FaDev &d) {
-    hipLaunchKernelGGL((k_walk<T, D, KAHAN>), dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
-                       d.stream, d);
+    hipLaunchKernelGGL((k_walk<T, D, KAHAN>), dim3(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK))),
+                       dim3(GR_BLOCK), 0, d.stream, d);
 }
 
 void pm_repulsion(FaDev &d);  // pm.hip

@@ -150,8 +150,10 @@ struct gr_fa2_session {
         d.cap = cap;
         gr_status s;
         if ((s = alloc(&d.wc, (size_t)cap * gr::walkcell_bytes(d.precision, d.dim)))) return s;
-        if ((s = alloc(&d.parent, (size_t)cap * 4))) return s;
-        if ((s = alloc(&d.cnt, (size_t)cap * 4))) return s;
+        if ((s = alloc(&d.clevel, (size_t)cap))) return s;
+        const size_t tiles = (cap + 4095) / 4096;
+        if ((s = alloc(&d.span, tiles * 32 * 4))) return s;
+        if ((s = alloc(&d.span_n, tiles * 4))) return s;
         if ((s = alloc(&d.raw, (size_t)cap * (1 + d.dim) * 8))) return s;
         if (d.export_bounds && (s = alloc(&d.cbounds, (size_t)cap * 2 * d.dim * 8))) return s;
         return GR_OK;
@@ -189,16 +191,19 @@ struct gr_fa2_session {
         if (st.pool_overflow) {
             grew = true;
             uint32_t grow = d.pool_cap * 2;
-            void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+            void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr;
             HIP_TRY(hipMalloc(&p1, (size_t)grow * 4));
             HIP_TRY(hipMalloc(&p2, (size_t)d.lcell_cap * 2 * 16));
             HIP_TRY(hipMalloc(&p3, (size_t)d.lcell_cap * 2 * 2 * d.dim * 8));
+            HIP_TRY(hipMalloc(&p4, (size_t)d.lcell_cap * 2 * (1 + d.dim) * 8));
             allocs.push_back(p1);
             allocs.push_back(p2);
             allocs.push_back(p3);
+            allocs.push_back(p4);
             d.pool = (int32_t *)p1;
             d.lc_int = (int32_t *)p2;
             d.lc_dbl = (double *)p3;
+            d.lc_sum = (double *)p4;
             d.pool_cap = grow;
             d.lcell_cap *= 2;
         }
@@ -387,6 +392,7 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
         TRY_ALLOC(d.rep[a], nn * es);
         TRY_ALLOC(d.sp[a], nn * es);
     }
+    TRY_ALLOC(d.pi, nn * d.dim * es);
     TRY_ALLOC(d.mass, nn * es);
     TRY_ALLOC(d.sm, nn * es);
     TRY_ALLOC(d.row_off, (nn + 1) * 4);
@@ -414,7 +420,8 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     TRY_ALLOC(d.pool, (size_t)d.pool_cap * 4);
     TRY_ALLOC(d.lc_int, (size_t)d.lcell_cap * 16);
     TRY_ALLOC(d.lc_dbl, (size_t)d.lcell_cap * 2 * d.dim * 8);
-    d.nparts = gr_blocks(nn, GR_BLOCK);
+    TRY_ALLOC(d.lc_sum, (size_t)d.lcell_cap * (1 + d.dim) * 8);
+    d.nparts = gr_xcd_grid(gr_blocks(nn, GR_BLOCK));  // a multiple of the XCD count
     if (d.nparts > 2048) d.nparts = 2048;
     TRY_ALLOC(d.bpart, (size_t)d.nparts * 10 * 8);
     TRY_ALLOC(d.spart, (size_t)d.nparts * 2 * 8);
@@ -598,7 +605,8 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
         std::vector<uint8_t> buf((size_t)C * wb);
         HIP_TRY(hipMemcpy(buf.data(), s->d.wc, buf.size(), hipMemcpyDeviceToHost));
         size_t es = s->esize();
-        size_t off = es * (dim + 2);  // c[dim], mass, w2, then skip, start, end
+        // [clo[dim] in the FP32 precisions], c[dim], mass, w2, then skip, start, end
+        size_t off = es * (dim + 2) + (s->d.precision == gr::PREC_FP64 ? 0 : es * dim);
         for (uint32_t k = 0; k < C; k++) {
             const uint8_t *c = buf.data() + (size_t)k * wb + off;
             uint32_t v[3];

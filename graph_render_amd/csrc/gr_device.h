@@ -28,7 +28,7 @@ struct GrState {
     uint32_t any_deep;
     uint32_t has_root;    // at least one finite point
     uint32_t lcell_used;  // deep-group local cells used
-    uint32_t pad;
+    uint32_t span_count;  // cells whose subtree crosses a centre-of-mass tile
 };
 
 __device__ __forceinline__ uint64_t gr_ballot(bool p) { return __ballot(p); }
@@ -73,6 +73,18 @@ __device__ __forceinline__ double gr_wave_max(double v) {
 
 __device__ __forceinline__ uint32_t gr_uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8). This
+// maps them so that XCD x owns logical blocks [x*q, (x+1)*q): consecutive bodies or rows
+// share one XCD's L2. Launch gr_xcd_grid(nb) workgroups; a logical index >= nb is idle.
+#define GR_XCDS 8u
+__device__ __forceinline__ uint32_t gr_xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = (nb + GR_XCDS - 1) / GR_XCDS;
+    return (b % GR_XCDS) * q + b / GR_XCDS;
+}
+static inline uint32_t gr_xcd_grid(uint32_t nb) {
+    return ((nb + GR_XCDS - 1) / GR_XCDS) * GR_XCDS;
 }
 
 static inline uint32_t gr_blocks(uint64_t n, uint32_t per) {

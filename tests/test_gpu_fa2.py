@@ -59,8 +59,8 @@ def test_tree_is_bit_exact(oracle, seed, dim):
     assert np.array_equal(tr["bodies"], rb, equal_nan=True)
 
 
-def test_tree_on_a_layout_is_bit_exact(oracle):
-    n = 20000
+@pytest.mark.parametrize("n", [20000, 200000])
+def test_tree_on_a_layout_is_bit_exact(oracle, n):
     s, t = bh_graph(n)
     ref = oracle_state(oracle, n, s, t)
     ref.run(3)

@@ -159,6 +159,17 @@ __device__ __forceinline__ bool halted(const FaDev &d) {
     return st->halt != 0 || (!d.ignore_done && st->done != 0);
 }
 
+// m_a*m_b/d2*k. FP64 divides exactly as the reference does; the FP32 precisions use the
+// hardware reciprocal (about 1 ulp), well inside their 1e-5 force tolerance.
+template <typename T>
+__device__ __forceinline__ T pair_force(T mm, T d2, T kr) {
+    if constexpr (std::is_same<T, double>::value) {
+        return mm / d2 * kr;
+    } else {
+        return mm * __builtin_amdgcn_rcpf(d2) * kr;
+    }
+}
+
 // Compensated (Kahan) accumulation, used by the FP32+Kahan precision.
 template <typename T, bool KAHAN>
 __device__ __forceinline__ void acc_add(T &s, T &c, T v) {
@@ -396,30 +407,29 @@ __global__ __launch_bounds__(GR_BLOCK) void k_gather(const FaDev d) {
     uint32_t i = d.vals[src];
     uint64_t key = d.keys[src];
     d.order[p] = i;
+    const T *pi = (const T *)d.pi;
 #pragma unroll
-    for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = ((const T *)d.pos[a])[i];
+    for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = pi[(size_t)i * D + a];
     ((T *)d.sm)[p] = ((const T *)d.mass)[i];
-    uint32_t newgroup = 0, distinct = 0;
-    if (p < nf) {
-        if (p == 0 || d.keys[p - 1] != key) {
-            newgroup = 1;
-        } else {
-            uint32_t q = d.vals[p - 1];
-#pragma unroll
-            for (int a = 0; a < D; a++)
-                if (((const T *)d.pos[a])[i] != ((const T *)d.pos[a])[q]) distinct = 1;
-        }
-    }
-    d.flag[p] = newgroup;
-    d.dflag[p] = distinct;
+    d.flag[p] = (p < nf && (p == 0 || d.keys[p - 1] != key)) ? 1u : 0u;
 }
 
-template <int D>
+template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_group_compact(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
     uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (p >= st->n_finite || !d.flag[p]) return;
+    if (p >= st->n_finite) return;
+    if (!d.flag[p]) {
+        // a later point of a group: does its position differ from the previous one's?
+        uint32_t distinct = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            if (((const T *)d.sp[a])[p] != ((const T *)d.sp[a])[p - 1]) distinct = 1;
+        d.dflag[p] = distinct;
+        return;
+    }
+    d.dflag[p] = 0;
     uint32_t g = d.gid[p];
     d.gstart[g] = p;
     d.gkey[g] = d.keys[p];
@@ -908,25 +918,40 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
     __syncthreads();
     uint32_t mask = 0;
     uint32_t *slots = d.span + (size_t)blockIdx.x * SPAN_SLOTS;
-    for (uint32_t k = t0 + threadIdx.x; k < t1; k += COM_THREADS) {
-        const uint32_t i = k - t0;
-        uint32_t l = d.clevel[k];
-        const uint32_t skip = wc[k].skip;
+    constexpr int PER = COM_TILE / COM_THREADS;
+    // independent loads first (level, skip, leaf range of every cell this thread owns), so
+    // their latencies overlap
+    uint32_t lvl[PER], skp[PER], qs[PER], qe[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t kk = t0 + threadIdx.x + j * COM_THREADS;
+        lvl[j] = kk < t1 ? d.clevel[kk] : 254u;
+        skp[j] = kk < t1 ? wc[kk].skip : 0u;
+        qs[j] = kk < t1 ? wc[kk].start : 0u;
+        qe[j] = kk < t1 ? wc[kk].end : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t kk = t0 + threadIdx.x + j * COM_THREADS;
+        if (kk >= t1) continue;
+        const uint32_t i = kk - t0;
+        uint32_t l = lvl[j];
+        const uint32_t skip = skp[j];
         skips[i] = skip;
         if (l == 255) {  // summed by the deep builder
-            const double *r = d.raw + (size_t)k * (1 + D);
+            const double *r = d.raw + (size_t)kk * (1 + D);
 #pragma unroll
             for (int a = 0; a <= D; a++) sums[i][a] = r[a];
         } else if (skip > t1) {
             uint32_t at = atomicAdd(&nspan, 1u);  // LDS
-            if (at < SPAN_SLOTS) slots[at] = k;
+            if (at < SPAN_SLOTS) slots[at] = kk;
             l = 255;
-        } else if (skip == k + 1) {
+        } else if (skip == kk + 1) {
             double s[1 + D];
 #pragma unroll
             for (int a = 0; a <= D; a++) s[a] = 0.0;
-            const uint32_t qend = wc[k].end < d.n ? wc[k].end : d.n;
-            for (uint32_t q = wc[k].start; q < qend; q++) {
+            const uint32_t qend = qe[j] < d.n ? qe[j] : d.n;
+            for (uint32_t q = qs[j]; q < qend; q++) {
                 double m = (double)((const T *)d.sm)[q];
                 s[0] += m;
 #pragma unroll
@@ -934,7 +959,7 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
             }
 #pragma unroll
             for (int a = 0; a <= D; a++) sums[i][a] = s[a];
-            body_final<T, D>(d, k, s);
+            body_final<T, D>(d, kk, s);
             l = 255;
         }
         lv[i] = (uint8_t)l;
@@ -1025,10 +1050,44 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
         }
     }
     __syncthreads();
+    // each span cell's children, read once (structure only), into a small global table
+    constexpr int KIDS = 1 << D;
+    {
+        const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
+        for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+            const uint32_t k = cells[q], skip = wc[k].skip;
+            uint32_t c = k + 1;
+            for (int x = 0; x < KIDS; x++) {
+                uint32_t kid = 0xFFFFFFFFu;
+                if (c < skip) {
+                    kid = c;
+                    const uint32_t nc = wc[c].skip;
+                    c = nc > c ? nc : skip;
+                }
+                d.span_kids[(size_t)q * KIDS + x] = kid;
+            }
+        }
+    }
+    __syncthreads();
     for (int l = 31; l >= 0; l--) {
         const uint32_t a = loff[l], b = loff[l + 1];
         if (a == b) continue;  // block-uniform
-        for (uint32_t q = a + threadIdx.x; q < b; q += blockDim.x) com_cell<T, D>(d, cells[q]);
+        for (uint32_t q = a + threadIdx.x; q < b; q += blockDim.x) {
+            uint32_t kid[KIDS];
+#pragma unroll
+            for (int x = 0; x < KIDS; x++) kid[x] = d.span_kids[(size_t)q * KIDS + x];
+            double s[1 + D];
+#pragma unroll
+            for (int y = 0; y <= D; y++) s[y] = 0.0;
+#pragma unroll
+            for (int x = 0; x < KIDS; x++) {
+                if (kid[x] == 0xFFFFFFFFu) break;
+                const double *r = d.raw + (size_t)kid[x] * (1 + D);
+#pragma unroll
+                for (int y = 0; y <= D; y++) s[y] += r[y];
+            }
+            body_final<T, D>(d, cells[q], s);
+        }
         __syncthreads();
     }
 }
@@ -1084,7 +1143,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             T d2 = dx[0] * dx[0] + dx[1] * dx[1];
             if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
             if (!holds && d2 > (T)0 && cell.w2 < theta2 * d2) {
-                const T ff = mi * cell.mass / d2 * kr;
+                const T ff = pair_force<T>(mi * cell.mass, d2, kr);
 #pragma unroll
                 for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], dx[a] * ff);
                 next = cell.skip;
@@ -1125,7 +1184,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
                     }
                     T d2 = dd[0] * dd[0] + dd[1] * dd[1];
                     if (D == 3) d2 = d2 + dd[D - 1] * dd[D - 1];
-                    const T ff = mi * mj / d2 * kr;
+                    const T ff = pair_force<T>(mi * mj, d2, kr);
 #pragma unroll
                     for (int a = 0; a < D; a++) acc_add<T, KAHAN>(lf[a], lc[a], dd[a] * ff);
                 }
@@ -1515,7 +1574,7 @@ static void tree_impl(FaDev &d) {
     radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, n, 64, d.sort_tmp, s);
     hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.flag, d.gid, n, nullptr, &d.st->n_groups, d.scan_tmp, s);
-    hipLaunchKernelGGL((k_group_compact<D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_group_compact<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_group_info<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.gcells, d.gbase, n, &d.st->n_groups, &d.st->need_cells, d.scan_tmp, s);
     hipLaunchKernelGGL(k_cells_ready, dim3(1), dim3(1), 0, s, d);

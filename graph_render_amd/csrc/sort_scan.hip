@@ -152,57 +152,77 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scan(uint32_t *hist, uint32_t n
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-// Stable scatter: keys of a tile are ranked round by round (256 consecutive keys per
-// round), inside a wave by ballot matching on the 8 digit bits, across the block's four
-// waves through LDS counts, so equal digits keep their input order.
+// Stable scatter. A tile's keys are ranked round by round (256 consecutive keys per round),
+// inside a wave by ballot matching on the 8 digit bits and across the block's four waves by
+// LDS counts, into the tile's digit-sorted order in LDS; the tile is then written out as one
+// contiguous run per digit, so global stores are coalesced. The tile's per-digit counts come
+// from the (already scanned) block histogram.
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin,
                                                          uint64_t *kout, uint32_t *vout,
                                                          uint32_t n, int shift,
                                                          const uint32_t *hist, uint32_t nb,
                                                          const uint32_t *totals) {
-    __shared__ uint32_t base[RS_BUCKETS];
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
+    __shared__ uint32_t gbase[RS_BUCKETS];  // global position of the tile's first key per digit
+    __shared__ uint32_t lbase[RS_BUCKETS];  // tile-local position of the same
+    __shared__ uint32_t run[RS_BUCKETS];
     __shared__ uint32_t wcnt[GR_BLOCK / GR_WAVE][RS_BUCKETS];
     __shared__ uint32_t lds[8];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    const uint32_t b = blockIdx.x;
     {
-        uint32_t t = totals[tid];
-        uint32_t inc = block_inclusive_scan(t, lds);
-        base[tid] = inc - t + hist[(size_t)tid * nb + blockIdx.x];
+        const uint32_t t = totals[tid];
+        const uint32_t inc = block_inclusive_scan(t, lds);
+        const uint32_t h = hist[(size_t)tid * nb + b];
+        gbase[tid] = inc - t + h;
+        const uint32_t next = b + 1 < nb ? hist[(size_t)tid * nb + b + 1] : t;
+        const uint32_t cnt = next - h;
+        const uint32_t linc = block_inclusive_scan(cnt, lds);
+        lbase[tid] = linc - cnt;
+        run[tid] = linc - cnt;
     }
     __syncthreads();
-    const uint64_t tile = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t tile = (uint64_t)b * RS_TILE;
+    const uint32_t tn = tile + RS_TILE <= n ? RS_TILE : (uint32_t)(n - tile);
     const uint64_t below = gr_lanemask_lt();
     for (int r = 0; r < RS_ITEMS; r++) {
-        uint64_t i = tile + (uint64_t)r * GR_BLOCK + tid;
-        if (tile + (uint64_t)r * GR_BLOCK >= n) break;  // block-uniform
-        bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        uint32_t val = valid ? vin[i] : 0;
-        uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
+        if ((uint32_t)r * GR_BLOCK >= tn) break;  // block-uniform
+        const uint32_t li = (uint32_t)r * GR_BLOCK + tid;
+        const bool valid = li < tn;
+        const uint64_t key = valid ? kin[tile + li] : 0;
+        const uint32_t val = valid ? vin[tile + li] : 0;
+        const uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
 #pragma unroll
         for (int q = 0; q < GR_BLOCK / GR_WAVE; q++) wcnt[q][tid] = 0;
         __syncthreads();
         uint64_t peers = gr_ballot(valid);
 #pragma unroll
-        for (int b = 0; b < RS_BITS; b++) {
-            uint64_t bb = gr_ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        for (int bit = 0; bit < RS_BITS; bit++) {
+            uint64_t bb = gr_ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & below);
-        uint32_t cnt = (uint32_t)__popcll(peers);
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
         if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
-            uint32_t pos = base[d] + rank;
+            uint32_t pos = run[d] + rank;
             for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
-            kout[pos] = key;
-            vout[pos] = val;
+            skey[pos] = key;
+            sval[pos] = val;
         }
         __syncthreads();
-        base[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
         __syncthreads();
     }
-    (void)lane;
+    for (uint32_t i = tid; i < tn; i += GR_BLOCK) {
+        const uint64_t key = skey[i];
+        const uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
+        const uint32_t pos = gbase[d] + (i - lbase[d]);
+        kout[pos] = key;
+        vout[pos] = sval[i];
+    }
 }
 
 size_t radix_tmp_bytes(uint32_t n) {

@@ -130,6 +130,23 @@ def main():
     node_err, field_err = errors(f32, f64)
     knode_err, kfield_err = errors(fk, f64)
 
+    # particle-mesh variant on a 2048^2 mesh, same graph and start, FP32
+    pm = G.Session(topo, params, G.Fa2Ext(repulsion="pm", pm_grid=2048, precision="fp32",
+                                          start=start, edge_weights=w))
+    pm.step(2)
+    t0 = time.perf_counter()
+    dpm = pm.step(args.steps)
+    pm_its = dpm / (time.perf_counter() - t0)
+    pm_ms = pm.time("pm", 5)
+    pm.set_positions(pos32)
+    fpm = pm.forces(2)
+    pm.close()
+    s64r = make("fp64")
+    s64r.set_positions(pos32)
+    f64r = s64r.forces(2)
+    s64r.close()
+    pm_node, pm_field = errors(fpm, f64r)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(topo, params, start, w, args.cpu_seconds)
@@ -159,6 +176,9 @@ def main():
         "fp32_kahan_iters_per_s": round(kahan_its, 3),
         "force_err_fp32_kahan_vs_fp64": {"per_node_rms": knode_err,
                                          "whole_field": kfield_err},
+        "pm_2048_iters_per_s": round(pm_its, 3),
+        "pm_2048_repulsion_ms": round(pm_ms, 4),
+        "pm_vs_bh_fp64_repulsion_err": {"per_node_rms": pm_node, "whole_field": pm_field},
         "walk_ms": round(walk_ms, 4),
         "tree_build_ms": round(tree_ms, 4),
         "tree_cells": int(ncells),

@@ -74,9 +74,15 @@ struct FaDev {
     void *scan_tmp = nullptr;
     void *sort_tmp = nullptr;
 
-    // PM grid (see pm.hip)
+    // particle mesh (pm.hip): side x side complex FP32 grids
     uint32_t pm_side = 0;
-    void *pm = nullptr;
+    float2 *pm_a = nullptr;   // density, then field, x along rows
+    float2 *pm_b = nullptr;   // transposed grid (y along rows)
+    float2 *pm_k = nullptr;   // kernel spectrum, transposed layout, 1/side^2 folded in
+    float2 *pm_tw = nullptr;  // twiddles exp(-2 pi i t / side), t < side/2
+    unsigned long long *pm_acc = nullptr;  // fixed-point deposit
+    void *pm_fr = nullptr;    // frame and the rung the spectrum was built for
+    double pm_fix = 0.0;      // fixed-point scale of the deposit (a power of two)
 
     int ignore_done = 0;  // set on force evaluations, which run after an early exit too
 
@@ -84,6 +90,8 @@ struct FaDev {
 };
 
 size_t walkcell_bytes(int precision, uint32_t dim);
+size_t pm_frame_bytes();
+void pm_twiddles(uint32_t side, float2 *host_out);
 
 // Positions changed from the host: recompute the bounds/mean partials.
 void launch_bounds(FaDev &d);

@@ -343,6 +343,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
         st->any_deep = 0;
         uint32_t nfin = (uint32_t)vsum[3];
         st->n_finite = nfin;
+        for (int a = 0; a < 3; a++) {
+            st->box_lo[a] = vmin[a];
+            st->box_hi[a] = vmax[a];
+        }
         st->has_root = nfin > 0;
         for (int a = 0; a < D; a++) st->mean[a] = vsum[a] / (double)d.n;
         if (nfin > 0) cover_root<D>(st, vmin, vmax);

@@ -430,9 +430,38 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     TRY_ALLOC(d.scan_tmp, gr::scan_tmp_bytes((uint32_t)nn));
     TRY_ALLOC(d.sort_tmp, gr::radix_tmp_bytes((uint32_t)nn));
 #undef TRY_ALLOC
+#define TRY_ALLOC2(ptr, bytes)                     \
+    if ((st = s->alloc(&(ptr), (bytes))) != GR_OK) { \
+        delete s;                                  \
+        return nullptr;                            \
+    }
     if (s->alloc_cells(2 * (uint32_t)nn + 64) != GR_OK) {
         delete s;
         return nullptr;
+    }
+    if (d.repulsion == gr::REP_PM) {
+        const size_t P2 = (size_t)d.pm_side * d.pm_side;
+        TRY_ALLOC2(d.pm_a, P2 * sizeof(float2));
+        TRY_ALLOC2(d.pm_b, P2 * sizeof(float2));
+        TRY_ALLOC2(d.pm_k, P2 * sizeof(float2));
+        TRY_ALLOC2(d.pm_acc, P2 * sizeof(unsigned long long));
+        TRY_ALLOC2(d.pm_tw, (d.pm_side / 2) * sizeof(float2));
+        TRY_ALLOC2(d.pm_fr, gr::pm_frame_bytes());
+        std::vector<float2> tw(d.pm_side / 2);
+        gr::pm_twiddles(d.pm_side, tw.data());
+        std::vector<uint8_t> zero(gr::pm_frame_bytes(), 0);
+        if (hipMemcpy(d.pm_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(d.pm_fr, zero.data(), zero.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(d.pm_a, 0, P2 * sizeof(float2)) != hipSuccess) {
+            fail(GR_ERR_DEVICE, "hipMemcpy", "particle-mesh setup failed");
+            delete s;
+            return nullptr;
+        }
+        // fixed-point deposit: every cell sum stays below 2^60
+        double total = 0.0;
+        for (double m : s->mass) total += m;
+        d.pm_fix = std::ldexp(1.0, 60 - (int)std::ceil(std::log2(total + 1.0)));
     }
 
     auto upload = [&](void *dst, const void *src, size_t bytes) -> bool {

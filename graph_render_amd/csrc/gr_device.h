@@ -15,6 +15,7 @@ struct GrState {
     double moved;         // last applied step's sum |dp| (schedule.rs:62-77)
     double mean[3];       // centroid of the current positions (state.rs:208-220)
     double root_lo[3], root_hi[3];  // root square of the current tree (quadtree.rs:157-195)
+    double box_lo[3], box_hi[3];    // extent of the finite positions
     uint32_t iter_no;     // jiggle tick, incremented once per iteration
     uint32_t iters_done;  // iterations applied by this session
     uint32_t done;        // set once an iteration moved < 1e-10 (state.rs:122-128)

@@ -1,6 +1,327 @@
-// Particle-mesh repulsion (placeholder until the PM kernels land; the session refuses PM).
+// Particle-mesh FA2 repulsion on gfx950: F_i = k * m_i * E(p_i), E = mass density (*) G,
+// G(r) = r / |r|^2, solved on a side x side periodic mesh in FP32.
+//
+// Reference machinery (paths relative to crates/graph-core/src/layout/force/particle_mesh/):
+//   frame    frame.rs:50-89 (rung ladder h = 2^(step/4), cells = floor(span/h) + 3, origin
+//            snapped to h); the FA2 law has infinite reach, so reach = cells - 1 and
+//            cells + reach <= side keeps occupied cells free of periodic images
+//   deposit  deposit.rs (CIC weights); here an exact fixed-point sum, so the density does not
+//            depend on the order the atomics land in
+//   solve    mesh.rs:151-185 + kernel.rs: packed kernel Gx + i*Gy, one forward transform of
+//            the density, multiply, one inverse; Ex = re, Ey = im; 1/side^2 folded into G
+//   read     mesh.rs:276-288 (CIC gather)
+// The FFT is a hand-written radix-2 Stockham transform per line in LDS; 2D transforms are
+// row passes with tiled LDS transposes between them. Only the rows that hold density (and
+// the rows that are read back) are transformed along x.
+#include <math.h>
+
 #include "fa2_kernels.h"
 
 namespace gr {
-void pm_repulsion(FaDev &d) { (void)d; }
+
+struct PmFrame {
+    double h, ox, oy;
+    int32_t step, cells, reach, valid;
+    int32_t kstep, kreach, kvalid, rebuild;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// 2^(q/4) for q = 0..3, correctly rounded: h = 2^(step/4) built exactly from these
+__device__ __forceinline__ double pm_h(int step) {
+    const double q[4] = {1.0, 1.189207115002721, 1.4142135623730951, 1.681792830507429};
+    return ldexp(q[step & 3], step >> 2);
+}
+
+__global__ void k_pm_frame(const FaDev d) {
+    const GrState *st = d.st;
+    PmFrame *f = (PmFrame *)d.pm_fr;
+    f->valid = 0;
+    f->rebuild = 0;
+    if (st->halt) return;
+    const int side = (int)d.pm_side;
+    const double span = fmax(st->box_hi[0] - st->box_lo[0], st->box_hi[1] - st->box_lo[1]);
+    if (d.n < 2 || st->n_finite == 0 || !isfinite(span)) return;
+    double s0 = floor(4.0 * log2(span / (double)side));
+    int step = s0 < -32.0 ? -32 : (int)s0;
+    for (int guard = 0; guard < 4096; guard++, step++) {
+        const double h = pm_h(step);
+        const double cells = floor(span / h) + 3.0;
+        if (cells > (double)side) continue;
+        const int c = (int)cells, reach = c - 1;
+        if (c + reach > side) continue;
+        f->step = step;
+        f->h = h;
+        f->cells = c;
+        f->reach = reach;
+        f->ox = floor(st->box_lo[0] / h) * h;
+        f->oy = floor(st->box_lo[1] / h) * h;
+        f->valid = 1;
+        f->rebuild = !(f->kvalid && f->kstep == step && f->kreach == reach);
+        return;
+    }
+}
+
+__device__ __forceinline__ void pm_stencil(const PmFrame *f, double px, double py, int &cx,
+                                           int &cy, double &fx, double &fy) {
+    const double ux = (px - f->ox) / f->h, uy = (py - f->oy) / f->h;
+    long ix = ux > 0.0 ? (long)ux : 0, iy = uy > 0.0 ? (long)uy : 0;
+    if (ix > f->cells - 2) ix = f->cells - 2;
+    if (iy > f->cells - 2) iy = f->cells - 2;
+    const double gx = ux - (double)ix, gy = uy - (double)iy;
+    fx = gx < 0.0 ? 0.0 : (gx > 1.0 ? 1.0 : gx);
+    fy = gy < 0.0 ? 0.0 : (gy > 1.0 ? 1.0 : gy);
+    cx = (int)ix;
+    cy = (int)iy;
+}
+
+// ---------------------------------------------------------------- kernel spectrum
+
+// G sampled at every cell offset within reach, wrapped modulo side, scaled by 1/side^2.
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_sample(const FaDev d) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid || !f->rebuild) return;
+    const uint32_t P = d.pm_side;
+    const double scale = 1.0 / ((double)P * (double)P);
+    const int reach = f->reach;
+    for (uint64_t q = (uint64_t)blockIdx.x * GR_BLOCK + threadIdx.x; q < (uint64_t)P * P;
+         q += (uint64_t)gridDim.x * GR_BLOCK) {
+        const int r = (int)(q / P), c = (int)(q % P);
+        const int dy = r <= reach ? r : r - (int)P;
+        const int dx = c <= reach ? c : c - (int)P;
+        float2 g = make_float2(0.f, 0.f);
+        if (dy >= -reach && dy <= reach && dx >= -reach && dx <= reach) {
+            const double rx = dx * f->h, ry = dy * f->h;
+            const double l = rx * rx + ry * ry;
+            if (l != 0.0) g = make_float2((float)(rx / l * scale), (float)(ry / l * scale));
+        }
+        d.pm_b[q] = g;
+    }
+}
+
+__global__ void k_pm_commit(const FaDev d) {
+    PmFrame *f = (PmFrame *)d.pm_fr;
+    if (!f->valid || !f->rebuild) return;
+    f->kstep = f->step;
+    f->kreach = f->reach;
+    f->kvalid = 1;
+}
+
+// ---------------------------------------------------------------- FFT
+
+// Radix-2 Stockham over one line of P points held in LDS (x, y: P float2 each). Returns the
+// buffer holding the result. tw[t] = exp(-2 pi i t / P), t < P/2; dir +1 is the inverse
+// (conjugate twiddles, unnormalised).
+__device__ float2 *stockham(float2 *x, float2 *y, uint32_t P, const float2 *__restrict__ tw,
+                            int dir) {
+    float2 *in = x, *out = y;
+    const uint32_t half = P >> 1;
+    for (uint32_t ns = 1; ns < P; ns <<= 1) {
+        const uint32_t tstride = P / (2 * ns);
+        for (uint32_t j = threadIdx.x; j < half; j += blockDim.x) {
+            const uint32_t k = j & (ns - 1);
+            float2 w = tw[k * tstride];
+            if (dir > 0) w.y = -w.y;
+            const float2 v0 = in[j];
+            const float2 v1 = cmul(in[j + half], w);
+            const uint32_t idx = ((j - k) << 1) + k;
+            out[idx] = make_float2(v0.x + v1.x, v0.y + v1.y);
+            out[idx + ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
+        }
+        __syncthreads();
+        float2 *t = in;
+        in = out;
+        out = t;
+    }
+    return in;
+}
+
+// One workgroup per line. mode 0: forward; 1: inverse; 2: forward, times the kernel
+// spectrum line, inverse. Lines >= limit are skipped (limit -1 means the frame's cells).
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_lines(const FaDev d, float2 *buf, int mode,
+                                                       int limit) {
+    extern __shared__ float2 lds_f2[];
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    if (limit == -2 && !f->rebuild) return;
+    const uint32_t P = d.pm_side;
+    const uint32_t lim = limit < 0 ? (limit == -1 ? (uint32_t)f->cells : P) : (uint32_t)limit;
+    const uint32_t line = blockIdx.x;
+    if (line >= lim) return;
+    float2 *x = lds_f2, *y = lds_f2 + P;
+    float2 *row = buf + (size_t)line * P;
+    for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) x[j] = row[j];
+    __syncthreads();
+    float2 *r = stockham(x, y, P, d.pm_tw, mode == 1 ? 1 : -1);
+    if (mode == 2) {
+        const float2 *kl = d.pm_k + (size_t)line * P;
+        for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) r[j] = cmul(r[j], kl[j]);
+        __syncthreads();
+        float2 *other = r == x ? y : x;
+        r = stockham(r, other, P, d.pm_tw, 1);
+    }
+    for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) row[j] = r[j];
+}
+
+// dst[c][r] = src[r][c] through 32x32 LDS tiles. rows_src: rows of src that hold data
+// (others read as zero); rows_dst: rows of dst to write (-1: the frame's cells; 0: all).
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_transpose(const FaDev d, const float2 *src,
+                                                           float2 *dst, int src_cells,
+                                                           int dst_cells, int only_rebuild) {
+    __shared__ float2 tile[32][33];
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    if (only_rebuild && !f->rebuild) return;
+    const uint32_t P = d.pm_side;
+    const uint32_t rs = src_cells ? (uint32_t)f->cells : P;  // src rows with data
+    const uint32_t rd = dst_cells ? (uint32_t)f->cells : P;  // dst rows needed
+    const uint32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;  // src col tile, src row tile
+    if (bx >= rd) return;  // dst row = src col
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (uint32_t k = ty; k < 32; k += 8) {
+        const uint32_t r = by + k, c = bx + tx;
+        tile[k][tx] = r < rs ? src[(size_t)r * P + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (uint32_t k = ty; k < 32; k += 8) {
+        const uint32_t r = bx + k, c = by + tx;  // dst row r = src col, dst col = src row
+        if (r < rd) dst[(size_t)r * P + c] = tile[tx][k];
+    }
+}
+
+// ---------------------------------------------------------------- deposit / read
+
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_clear(const FaDev d) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    const uint32_t P = d.pm_side, c = (uint32_t)f->cells;
+    for (uint64_t q = (uint64_t)blockIdx.x * GR_BLOCK + threadIdx.x; q < (uint64_t)c * P;
+         q += (uint64_t)gridDim.x * GR_BLOCK)
+        d.pm_acc[q] = 0ull;
+}
+
+template <typename T>
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_deposit(const FaDev d) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= d.n) return;
+    const T *pi = (const T *)d.pi;
+    const double px = (double)pi[2 * (size_t)i], py = (double)pi[2 * (size_t)i + 1];
+    if (!isfinite(px) || !isfinite(py)) return;
+    int cx, cy;
+    double fx, fy;
+    pm_stencil(f, px, py, cx, cy, fx, fy);
+    const double m = (double)((const T *)d.mass)[i] * d.pm_fix;
+    const uint32_t P = d.pm_side;
+    const size_t at = (size_t)cy * P + cx;
+    atomicAdd(&d.pm_acc[at], (unsigned long long)llrint(m * ((1.0 - fx) * (1.0 - fy))));
+    atomicAdd(&d.pm_acc[at + 1], (unsigned long long)llrint(m * (fx * (1.0 - fy))));
+    atomicAdd(&d.pm_acc[at + P], (unsigned long long)llrint(m * ((1.0 - fx) * fy)));
+    atomicAdd(&d.pm_acc[at + P + 1], (unsigned long long)llrint(m * (fx * fy)));
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_to_grid(const FaDev d) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    const uint32_t P = d.pm_side, c = (uint32_t)f->cells;
+    const double inv = 1.0 / d.pm_fix;
+    for (uint64_t q = (uint64_t)blockIdx.x * GR_BLOCK + threadIdx.x; q < (uint64_t)c * P;
+         q += (uint64_t)gridDim.x * GR_BLOCK) {
+        const uint32_t col = (uint32_t)(q % P);
+        const float v = col < c ? (float)((double)d.pm_acc[q] * inv) : 0.f;
+        d.pm_a[q] = make_float2(v, 0.f);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_read(const FaDev d) {
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= d.n) return;
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    T *rx = (T *)d.rep[0], *ry = (T *)d.rep[1];
+    if (!f->valid) {
+        rx[i] = (T)0;
+        ry[i] = (T)0;
+        return;
+    }
+    const T *pi = (const T *)d.pi;
+    const double px = (double)pi[2 * (size_t)i], py = (double)pi[2 * (size_t)i + 1];
+    if (!isfinite(px) || !isfinite(py)) {
+        rx[i] = (T)NAN;
+        ry[i] = (T)NAN;
+        return;
+    }
+    int cx, cy;
+    double fx, fy;
+    pm_stencil(f, px, py, cx, cy, fx, fy);
+    const uint32_t P = d.pm_side;
+    const size_t at = (size_t)cy * P + cx;
+    const double w[4] = {(1.0 - fx) * (1.0 - fy), fx * (1.0 - fy), (1.0 - fx) * fy, fx * fy};
+    const size_t cell[4] = {at, at + 1, at + P, at + P + 1};
+    double ex = 0.0, ey = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float2 e = d.pm_a[cell[q]];
+        ex += (double)e.x * w[q];
+        ey += (double)e.y * w[q];
+    }
+    const double km = d.scaling_ratio * (double)((const T *)d.mass)[i];
+    rx[i] = (T)(km * ex);
+    ry[i] = (T)(km * ey);
+}
+
+// ---------------------------------------------------------------- driver
+
+void pm_repulsion(FaDev &d) {
+    hipStream_t s = d.stream;
+    const uint32_t P = d.pm_side;
+    const uint32_t tiles = P / 32;
+    const size_t lds = 2 * (size_t)P * sizeof(float2);
+    const uint32_t fill = 2048;
+    hipLaunchKernelGGL(k_pm_frame, dim3(1), dim3(1), 0, s, d);
+    // kernel spectrum, only when the rung or reach changed: sample, FFT rows, transpose,
+    // FFT rows again, so it sits in the transposed layout the column pass multiplies in
+    hipLaunchKernelGGL(k_pm_sample, dim3(fill), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_b, 0, -2);
+    hipLaunchKernelGGL(k_pm_transpose, dim3(tiles, tiles), dim3(GR_BLOCK), 0, s, d,
+                       (const float2 *)d.pm_b, d.pm_k, 0, 0, 1);
+    hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_k, 0, -2);
+    hipLaunchKernelGGL(k_pm_commit, dim3(1), dim3(1), 0, s, d);
+    // density, forward along x (rows with data), transpose, y-lines forward * K inverse,
+    // transpose back (rows read), inverse along x, CIC read
+    hipLaunchKernelGGL(k_pm_clear, dim3(fill), dim3(GR_BLOCK), 0, s, d);
+    if (d.precision == PREC_FP64)
+        hipLaunchKernelGGL(k_pm_deposit<double>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK),
+                           0, s, d);
+    else
+        hipLaunchKernelGGL(k_pm_deposit<float>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK),
+                           0, s, d);
+    hipLaunchKernelGGL(k_pm_to_grid, dim3(fill), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_a, 0, -1);
+    hipLaunchKernelGGL(k_pm_transpose, dim3(tiles, tiles), dim3(GR_BLOCK), 0, s, d,
+                       (const float2 *)d.pm_a, d.pm_b, 1, 0, 0);
+    hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_b, 2, (int)P);
+    hipLaunchKernelGGL(k_pm_transpose, dim3(tiles, tiles), dim3(GR_BLOCK), 0, s, d,
+                       (const float2 *)d.pm_b, d.pm_a, 0, 1, 0);
+    hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_a, 1, -1);
+    if (d.precision == PREC_FP64)
+        hipLaunchKernelGGL(k_pm_read<double>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                           s, d);
+    else
+        hipLaunchKernelGGL(k_pm_read<float>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                           s, d);
+}
+
+size_t pm_frame_bytes() { return sizeof(PmFrame); }
+
+void pm_twiddles(uint32_t P, float2 *host_out) {
+    for (uint32_t t = 0; t < P / 2; t++) {
+        const double a = -2.0 * M_PI * (double)t / (double)P;
+        host_out[t] = make_float2((float)cos(a), (float)sin(a));
+    }
+}
+
 }  // namespace gr

@@ -78,6 +78,12 @@ void gro_tree_order(const gro_tree *t, uint32_t *order);
 void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, const double *ys,
                      const double *zs, double *out);
 
+/* ---- particle mesh (pm_oracle.c) ---- */
+double gro_pm_h(int step);
+void gro_fft2(double *re, double *im, int side, int sign);
+int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, int side,
+                  double *out, double *frame_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,11 @@ def lib():
         L.gro_tree_cells.argtypes = [C.c_void_p, f64p, u32p, u32p, u32p]
         L.gro_tree_order.argtypes = [C.c_void_p, u32p]
         L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
+        L.gro_pm_h.restype = C.c_double
+        L.gro_pm_h.argtypes = [C.c_int]
+        L.gro_fft2.argtypes = [f64p, f64p, C.c_int, C.c_int]
+        L.gro_pm_forces.restype = C.c_int
+        L.gro_pm_forces.argtypes = [C.c_uint32, f64p, f64p, C.c_double, C.c_int, f64p, f64p]
         _lib = L
     return _lib
 
@@ -218,3 +223,27 @@ class Tree:
         if h:
             lib().gro_tree_free(h)
             self._h = None
+
+
+def fft2(z: np.ndarray, sign: int = -1) -> np.ndarray:
+    side = z.shape[0]
+    re = np.ascontiguousarray(z.real, np.float64).copy()
+    im = np.ascontiguousarray(z.imag, np.float64).copy()
+    lib().gro_fft2(re, im, side, sign)
+    return re + 1j * im
+
+
+def pm_forces(xy: np.ndarray, mass: np.ndarray, k: float, side: int):
+    """FA2-law particle-mesh repulsion: (forces (n,2), frame dict or None)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    fr = np.zeros(6)
+    ok = lib().gro_pm_forces(n, xy.reshape(-1) if n else np.zeros(2),
+                             np.ascontiguousarray(mass, np.float64) if n else np.zeros(1),
+                             k, side, out, fr)
+    frame = None
+    if ok:
+        frame = dict(h=fr[0], ox=fr[1], oy=fr[2], step=int(fr[3]), cells=int(fr[4]),
+                     reach=int(fr[5]))
+    return out[: 2 * n].reshape(n, 2), frame

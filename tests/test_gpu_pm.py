@@ -1,0 +1,59 @@
+"""GPU particle-mesh repulsion (FP32 mesh) against the FP64 PM oracle, through the C ABI.
+
+Bar: the device solves the same frame, deposit, kernel and read as the oracle, in FP32 with
+an exact fixed-point deposit, so the forces agree to FP32 FFT accuracy: whole-field relative
+error <= 1e-4 and per-node rms <= 1e-3 (both metrics of barnes_hut/tests.rs:34-47)."""
+import numpy as np
+import pytest
+
+import graph_render_amd as G
+from tests.refgraphs import bh_graph, errors
+
+pytestmark = pytest.mark.gpu
+
+
+def pm_session(n, s, t, start, side, precision="fp32"):
+    return G.Session(G.Topology(n, s, t), G.Fa2Params(),
+                     G.Fa2Ext(repulsion="pm", pm_grid=side, start=start, precision=precision))
+
+
+@pytest.mark.parametrize("side,n,spread", [(128, 300, 10.0), (256, 3000, 50.0),
+                                          (1024, 20000, 200.0), (2048, 60000, 500.0)])
+def test_pm_matches_the_oracle(oracle, side, n, spread):
+    rng = np.random.default_rng(side)
+    xy = np.concatenate([rng.normal(0, spread / 4, (n // 2, 2)),
+                         rng.uniform(-spread, spread, (n - n // 2, 2))])
+    s, t = bh_graph(n)
+    print(f"pm case side={side} n={n}", flush=True)
+    with pm_session(n, s, t, xy.astype(np.float32).astype(np.float64), side) as sess:
+        f = sess.forces(2)
+        _, _, mass = sess.graph()
+    ref, frame = oracle.pm_forces(xy.astype(np.float32).astype(np.float64), mass, 2.0, side)
+    assert frame is not None
+    node, total = errors(f, ref)
+    print("pm vs oracle", side, n, node, total)
+    assert total <= 1e-4 and node <= 1e-3, (node, total)
+
+
+def test_pm_far_field_follows_the_pair_sum():
+    rng = np.random.default_rng(2)
+    a = rng.normal(0, 1.0, (200, 2))
+    b = rng.normal(0, 1.0, (200, 2)) + [60.0, 0.0]
+    xy = np.concatenate([a, b])
+    e = np.zeros(0, np.uint32)
+    with pm_session(400, e, e, xy, 512) as sess:
+        f = sess.forces(2)
+    with G.Session(G.Topology(400, e, e), G.Fa2Params(),
+                   G.Fa2Ext(repulsion="dense", start=xy)) as dense:
+        ex = dense.forces(2)
+    for sl in (slice(0, 200), slice(200, 400)):
+        assert np.linalg.norm(f[sl].sum(0) - ex[sl].sum(0)) <= 2e-2 * np.linalg.norm(ex[sl].sum(0))
+
+
+def test_pm_layout_runs_and_stays_finite():
+    n = 5000
+    s, t = bh_graph(n)
+    with pm_session(n, s, t, None, 512) as sess:
+        assert sess.step(30) == 30
+        p = sess.positions()
+    assert np.isfinite(p).all()

@@ -40,6 +40,18 @@ def walk_bytes(n: int, ncells: int, precision: str) -> int:
     return n * (3 * es + 4 + 2 * es) + ncells * cell
 
 
+def walk_traffic(n: int):
+    """HBM bytes per k_walk<float,2> launch on this workload, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/walk_1M_pmc.md), or None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "walk_1M_pmc.md")
+    if n != 1_000_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_walk_fp32_1M"):
+            return int(line.split("=")[1])
+    return None
+
+
 def cpu_baseline(topo, params, start, weights, budget_s: float):
     """The oracle (the reference's FA2-BH restated in C, FP64, one thread) on the same graph
     and start, iterated until about budget_s seconds of CPU time are spent."""
@@ -184,7 +196,7 @@ def main():
         "tree_cells": int(ncells),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "kernel": "k_walk<float,2>",
+                     "traffic": walk_traffic(n), "kernel": "k_walk<float,2>",
                      "algorithmic_bytes": int(wb)},
         "cpu_baseline": cpu,
     }

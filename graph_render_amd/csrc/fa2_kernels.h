@@ -36,6 +36,16 @@ struct FaDev {
     void *w = nullptr;
     GrState *st = nullptr;
 
+    // batch layout: graph b owns nodes [gnode[b], gnode[b+1]) and node chunks
+    // [gchunk[b], gchunk[b+1]); chunk c holds nodes [chunk[c], chunk[c+1]) of graph chunk_g[c]
+    uint32_t ngraphs = 1;
+    uint32_t levels = 31;  // key levels below the root (the graph id sits above them)
+    GrGraph *gr = nullptr;
+    uint32_t *gnode = nullptr, *gidn = nullptr;  // gidn: node -> graph (null for one graph)
+    uint32_t nchunks = 0;
+    uint32_t *chunk = nullptr, *chunk_g = nullptr, *gchunk = nullptr;
+    double *nodeval = nullptr;  // per node: swing and traction terms
+
     // tree
     uint64_t *keys = nullptr, *keys_alt = nullptr;
     uint32_t *vals = nullptr, *vals_alt = nullptr;

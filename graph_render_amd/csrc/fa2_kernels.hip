@@ -101,34 +101,40 @@ __device__ __forceinline__ void box_quad(Box<D> &b, int slot) {
 }
 
 template <int D>
-__device__ __forceinline__ Box<D> root_box(const GrState *st) {
+__device__ __forceinline__ Box<D> root_box(const GrGraph &g) {
     Box<D> b;
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        b.lo[a] = st->root_lo[a];
-        b.hi[a] = st->root_hi[a];
+        b.lo[a] = g.root_lo[a];
+        b.hi[a] = g.root_hi[a];
     }
     return b;
 }
 
-template <int D>
-__device__ __forceinline__ int key_slot(uint64_t key, int level) {
-    return (int)((key >> (D * (KeyFmt<D>::LEVELS - level))) & ((1u << D) - 1));
+__device__ __forceinline__ uint32_t graph_of(const FaDev &d, uint32_t node) {
+    return d.gidn ? d.gidn[node] : 0u;
 }
 
-// Levels two finite keys share below the root.
+// Key layout: [63: non-finite][graph id][L levels of D slot bits], level 1 highest.
 template <int D>
-__device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b) {
-    uint64_t x = a ^ b;
-    if (x == 0) return KeyFmt<D>::LEVELS;
-    int lz = __clzll((long long)x) - (64 - KeyFmt<D>::USED);
+__device__ __forceinline__ int key_slot(uint64_t key, int level, int L) {
+    return (int)((key >> (D * (L - level))) & ((1u << D) - 1));
+}
+
+// Levels two finite keys share below their root; -1 when they are in different graphs.
+template <int D>
+__device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b, int L) {
+    const uint64_t x = a ^ b;
+    if (x >> (D * L)) return -1;
+    if (x == 0) return L;
+    const int lz = __clzll((long long)x) - (64 - D * L);
     return lz / D;
 }
 
+// Same graph and the same first `level` slots (level 0: the same graph).
 template <int D>
-__device__ __forceinline__ bool key_same(uint64_t a, uint64_t b, int level) {
-    if (level <= 0) return true;
-    return ((a ^ b) >> (D * (KeyFmt<D>::LEVELS - level))) == 0;
+__device__ __forceinline__ bool key_same(uint64_t a, uint64_t b, int level, int L) {
+    return ((a ^ b) >> (D * (L - level))) == 0;
 }
 
 // rng.rs:27-84: MurmurHash3 fmix64, the three-round fold and the 53-bit nudge.
@@ -269,7 +275,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_bounds(const FaDev d) {
         hi[a] = -INFINITY;
         sum[a] = 0.0;
     }
-    for (uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * GR_BLOCK) {
+    const uint32_t c0 = d.chunk[blockIdx.x], c1 = d.chunk[blockIdx.x + 1];
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += GR_BLOCK) {
         double p[D];
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -284,7 +291,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_bounds(const FaDev d) {
 // quadtree.rs:157-195 (octree.rs:220-254): the root square from the floor of the first
 // corner, doubled until it holds the second; a square that stops growing is left as is.
 template <int D>
-__device__ void cover_root(GrState *st, const double *lo, const double *hi) {
+__device__ void cover_root(GrGraph *st, const double *lo, const double *hi) {
     double blo[D], bhi[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
@@ -316,16 +323,19 @@ __device__ void cover_root(GrState *st, const double *lo, const double *hi) {
     }
 }
 
+// Per graph (one workgroup each): its extent, centroid and root square from the bounds
+// partials of its node chunks, reduced in a fixed order.
 template <int D>
-__global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
-    GrState *st = d.st;
+__global__ __launch_bounds__(GR_BLOCK) void k_head_graph(const FaDev d) {
     if (halted(d)) return;
+    const uint32_t b = blockIdx.x;
+    GrGraph *g = d.gr + b;
     __shared__ double lds[4 * 4];
     double vmin[3] = {INFINITY, INFINITY, INFINITY};
     double vmax[3] = {-INFINITY, -INFINITY, -INFINITY};
     double vsum[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) {
-        const double *o = d.bpart + (size_t)b * NB;
+    for (uint32_t c = d.gchunk[b] + threadIdx.x; c < d.gchunk[b + 1]; c += GR_BLOCK) {
+        const double *o = d.bpart + (size_t)c * NB;
         for (int a = 0; a < 3; a++) {
             vmin[a] = fmin(vmin[a], o[a]);
             vmax[a] = fmax(vmax[a], o[3 + a]);
@@ -337,19 +347,45 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
     block_reduce<3, 2>(vmax, lds);
     block_reduce<4, 0>(vsum, lds);
     if (threadIdx.x == 0) {
+        const uint32_t nfin = (uint32_t)vsum[3];
+        const double nb = (double)(d.gnode[b + 1] - d.gnode[b]);
+        g->n_finite = nfin;
+        g->has_root = nfin > 0;
+        g->cell_lo = g->cell_hi = 0;
+        for (int a = 0; a < 3; a++) {
+            g->box_lo[a] = vmin[a];
+            g->box_hi[a] = vmax[a];
+        }
+        for (int a = 0; a < D; a++) g->mean[a] = vsum[a] / nb;
+        if (nfin > 0) cover_root<D>(g, vmin, vmax);
+    }
+}
+
+// Session-wide: counters of this iteration's build, points in trees, graphs still live.
+__global__ __launch_bounds__(GR_BLOCK) void k_head_global(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ uint32_t s_fin, s_live;
+    if (threadIdx.x == 0) {
+        s_fin = 0;
+        s_live = 0;
+    }
+    __syncthreads();
+    uint32_t fin = 0, live = 0;
+    for (uint32_t b = threadIdx.x; b < d.ngraphs; b += GR_BLOCK) {
+        fin += d.gr[b].n_finite;
+        live += d.gr[b].done ? 0u : 1u;
+    }
+    atomicAdd(&s_fin, fin);
+    atomicAdd(&s_live, live);
+    __syncthreads();
+    if (threadIdx.x == 0) {
         st->pool_used = 0;
         st->lcell_used = 0;
         st->span_count = 0;
         st->any_deep = 0;
-        uint32_t nfin = (uint32_t)vsum[3];
-        st->n_finite = nfin;
-        for (int a = 0; a < 3; a++) {
-            st->box_lo[a] = vmin[a];
-            st->box_hi[a] = vmax[a];
-        }
-        st->has_root = nfin > 0;
-        for (int a = 0; a < D; a++) st->mean[a] = vsum[a] / (double)d.n;
-        if (nfin > 0) cover_root<D>(st, vmin, vmax);
+        st->n_finite = s_fin;
+        st->live = s_live;
     }
 }
 
@@ -361,7 +397,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head(const FaDev d) {
 // instead (build.rs:76-85); the key stops there and the level is kept as the point's stall.
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
-    const GrState *st = d.st;
     if (halted(d)) return;
     uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (i >= d.n) return;
@@ -375,9 +410,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
     uint64_t key = 1ull << 63;
     uint32_t stall = 0xFF;
     if (fin) {
-        key = 0;
-        Box<D> b = root_box<D>(st);
-        for (int l = 1; l <= KeyFmt<D>::LEVELS; l++) {
+        const int L = (int)d.levels;
+        const uint32_t gb = graph_of(d, i);
+        key = (uint64_t)gb << (D * L);
+        Box<D> b = root_box<D>(d.gr[gb]);
+        for (int l = 1; l <= L; l++) {
             double before = b.hi[0] - b.lo[0];
             int slot = box_narrow<D>(b, p);
             double after = b.hi[0] - b.lo[0];
@@ -385,7 +422,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
                 stall = (uint32_t)(l - 1);
                 break;
             }
-            key |= (uint64_t)slot << (D * (KeyFmt<D>::LEVELS - l));
+            key |= (uint64_t)slot << (D * (L - l));
         }
     }
     // Reverse input order: the stable sort then leaves equal keys in descending point
@@ -665,10 +702,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_group_info(const FaDev d) {
     uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
     const uint32_t G = st->n_groups;
     if (g >= G) return;
-    constexpr int L = KeyFmt<D>::LEVELS;
+    const int L = (int)d.levels;
     uint64_t key = d.gkey[g];
-    int left = g > 0 ? key_lcp<D>(d.gkey[g - 1], key) : -1;
-    int right = g + 1 < G ? key_lcp<D>(key, d.gkey[g + 1]) : -1;
+    int left = g > 0 ? key_lcp<D>(d.gkey[g - 1], key, L) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, d.gkey[g + 1], L) : -1;
     uint32_t s = d.gstart[g];
     uint32_t e = g + 1 < G ? d.gstart[g + 1] : st->n_finite;
     bool distinct = false;
@@ -691,8 +728,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_group_info(const FaDev d) {
         } else {
             type = G_DEEP;
             level = L;
-            Box<D> b = root_box<D>(st);
-            for (int l = 1; l <= L; l++) box_quad<D>(b, key_slot<D>(key, l));
+            Box<D> b = root_box<D>(d.gr[graph_of(d, d.order[s])]);
+            for (int l = 1; l <= L; l++) box_quad<D>(b, key_slot<D>(key, l, L));
             uint32_t lc = 0, lcount = 0;
             if (!local_build<T, D>(d, s, e, b, &lc, &lcount)) {
                 st->halt = 1;
@@ -720,43 +757,22 @@ __global__ void k_cells_ready(const FaDev d) {
 
 // The last group h >= g whose key shares `level` levels with group g's.
 template <int D>
-__device__ uint32_t gallop_fwd(const uint64_t *gkey, uint32_t G, uint32_t g, int level) {
+__device__ uint32_t gallop_fwd(const uint64_t *gkey, uint32_t G, uint32_t g, int level, int L) {
     uint64_t key = gkey[g];
     uint32_t lo = g, step = 1;
-    while (lo + step < G && key_same<D>(key, gkey[lo + step], level)) {
+    while (lo + step < G && key_same<D>(key, gkey[lo + step], level, L)) {
         lo += step;
         step <<= 1;
     }
     uint32_t hi = lo + step < G ? lo + step : G;  // first index known (or assumed) not to match
     while (hi - lo > 1) {
         uint32_t mid = lo + (hi - lo) / 2;
-        if (key_same<D>(key, gkey[mid], level))
+        if (key_same<D>(key, gkey[mid], level, L))
             lo = mid;
         else
             hi = mid;
     }
     return lo;
-}
-
-// The first group f <= g whose key shares `level` levels with group g's.
-template <int D>
-__device__ uint32_t gallop_bwd(const uint64_t *gkey, uint32_t g, int level) {
-    uint64_t key = gkey[g];
-    uint32_t hi = g, step = 1;
-    while (hi >= step && key_same<D>(key, gkey[hi - step], level)) {
-        hi -= step;
-        step <<= 1;
-    }
-    int64_t lo = (int64_t)hi - (int64_t)step;  // last index known (or assumed) not to match
-    if (lo < -1) lo = -1;
-    while ((int64_t)hi - lo > 1) {
-        int64_t mid = lo + ((int64_t)hi - lo) / 2;
-        if (key_same<D>(key, gkey[mid], level))
-            hi = (uint32_t)mid;
-        else
-            lo = mid;
-    }
-    return hi;
 }
 
 template <typename T, int D>
@@ -803,34 +819,42 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
     const uint32_t G = st->n_groups;
     if (g >= G) return;
-    constexpr int L = KeyFmt<D>::LEVELS;
+    const int L = (int)d.levels;
     const uint32_t C = st->n_cells, nf = st->n_finite;
     const uint64_t *gkey = d.gkey;
     uint64_t key = gkey[g];
-    int left = g > 0 ? key_lcp<D>(gkey[g - 1], key) : -1;
-    int right = g + 1 < G ? key_lcp<D>(key, gkey[g + 1]) : -1;
+    int left = g > 0 ? key_lcp<D>(gkey[g - 1], key, L) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, gkey[g + 1], L) : -1;
     uint32_t info = d.ginfo[g], type = info & 0xFF;
     int level = (int)(info >> 8);
     int last_internal = type == G_SIMPLE ? level - 1 : (type == G_STALL ? level - 1 : L - 1);
-    uint32_t start = d.gstart[g];
+    const uint32_t start = d.gstart[g];
     uint32_t gend = g + 1 < G ? d.gstart[g + 1] : nf;
     uint32_t gskip = g + 1 < G ? d.gbase[g + 1] : C;
     uint32_t idx = d.gbase[g];
-    Box<D> b = root_box<D>(st);
+    const uint32_t gb = graph_of(d, d.order[start]);
+    Box<D> b = root_box<D>(d.gr[gb]);
     int top = type == G_DEEP ? L - 1 : level;
     for (int l = 0; l <= top; l++) {
-        if (l > 0) box_quad<D>(b, key_slot<D>(key, l));
+        if (l > 0) box_quad<D>(b, key_slot<D>(key, l, L));
         if (l <= left) continue;
+        uint32_t skip;
         if (l <= last_internal) {
-            uint32_t skip = gskip, end = gend;
+            uint32_t end = gend;
+            skip = gskip;
             if (l <= right) {
-                uint32_t h = gallop_fwd<D>(gkey, G, g, l);
+                uint32_t h = gallop_fwd<D>(gkey, G, g, l, L);
                 skip = h + 1 < G ? d.gbase[h + 1] : C;
                 end = h + 1 < G ? d.gstart[h + 1] : nf;
             }
             emit_cell<T, D>(d, idx, b, skip, start, end, (uint32_t)l);
         } else {
-            emit_cell<T, D>(d, idx, b, idx + 1, start, gend, (uint32_t)l);
+            skip = idx + 1;
+            emit_cell<T, D>(d, idx, b, skip, start, gend, (uint32_t)l);
+        }
+        if (l == 0) {  // the graph's root: its tree is cells [idx, skip)
+            d.gr[gb].cell_lo = idx;
+            d.gr[gb].cell_hi = skip;
         }
         idx++;
     }
@@ -1124,13 +1148,21 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const T mi = sm[pc];
     const uint32_t i = order[pc];
     const T theta2 = (T)d.theta2, kr = (T)d.scaling_ratio;
-    const uint32_t seed = d.seed, tick = st->iter_no;
+    // each lane walks its own graph's tree, cells [lo, hi) of the arena
+    const GrGraph &gg = d.gr[graph_of(d, i)];
+    const uint32_t seed = d.seed, tick = gg.iter_no;
+    const uint32_t lo = gg.cell_lo, hi = gg.cell_hi;
     const uint32_t ncells = st->n_cells;
     T f[D], fc[D];
 #pragma unroll
     for (int a = 0; a < D; a++) f[a] = fc[a] = (T)0;
-    uint32_t next = live ? 0u : 0xFFFFFFFFu;
-    uint32_t k = 0;
+    uint32_t next = (live && lo < hi) ? lo : 0xFFFFFFFFu;
+    uint32_t k = next;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t o = __shfl_xor(k, m, GR_WAVE);
+        k = o < k ? o : k;
+    }
     while (k < ncells) {
         k = gr_uniform(k);
         const WalkCell<T, D> cell = wc[k];
@@ -1155,6 +1187,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
                 next = k + 1;
                 open = true;
             }
+            if (next >= hi) next = 0xFFFFFFFFu;  // past its graph's tree: done
         }
         const bool leaf = cell.skip == k + 1;
         const bool pairs = open && leaf;
@@ -1276,21 +1309,36 @@ __global__ __launch_bounds__(GR_BLOCK) void k_dense(const FaDev d, uint32_t with
     }
     const T mv = ((const T *)d.mass)[vc];
     const T kr = (T)d.scaling_ratio;
-    const uint32_t seed = d.seed, tick = st->iter_no;
-    for (uint32_t j0 = 0; j0 < n; j0 += GR_BLOCK) {
+    const uint32_t gv = graph_of(d, vc);
+    const uint32_t seed = d.seed, tick = d.gr[gv].iter_no;
+    const uint32_t jlo = d.gnode[gv], jhi = d.gnode[gv + 1];
+    // a block's nodes can span graphs: it sweeps the union of their node ranges
+    __shared__ uint32_t s_lo, s_hi;
+    if (threadIdx.x == 0) {
+        s_lo = 0xFFFFFFFFu;
+        s_hi = 0;
+    }
+    __syncthreads();
+    if (live) {
+        atomicMin(&s_lo, jlo);
+        atomicMax(&s_hi, jhi);
+    }
+    __syncthreads();
+    const uint32_t blo = s_lo, bhi = s_hi;
+    for (uint32_t j0 = blo; j0 < bhi; j0 += GR_BLOCK) {
         __syncthreads();
         uint32_t jj = j0 + threadIdx.x;
-        if (jj < n) {
+        if (jj < bhi) {
 #pragma unroll
             for (int a = 0; a < D; a++) tp[a][threadIdx.x] = ((const T *)d.pos[a])[jj];
             tm[threadIdx.x] = ((const T *)d.mass)[jj];
         }
         __syncthreads();
-        uint32_t cnt = n - j0 < GR_BLOCK ? n - j0 : GR_BLOCK;
+        uint32_t cnt = bhi - j0 < GR_BLOCK ? bhi - j0 : GR_BLOCK;
         if (!live) continue;
         for (uint32_t t = 0; t < cnt; t++) {
             uint32_t j = j0 + t;
-            if (j == v) continue;
+            if (j == v || j < jlo || j >= jhi) continue;
             T dd[D];
             bool zero = true;
 #pragma unroll
@@ -1419,10 +1467,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         }
         const double mv = (double)((const T *)d.mass)[v];
         if (parts & PART_GRAVITY) {
+            const GrGraph &gg = d.gr[graph_of(d, v)];
             double cen[D], nn = 0.0;
 #pragma unroll
             for (int a = 0; a < D; a++) {
-                cen[a] = (double)pv[a] - st->mean[a];
+                cen[a] = (double)pv[a] - gg.mean[a];
                 nn += cen[a] * cen[a];
             }
             double norm = sqrt(nn);
@@ -1442,11 +1491,23 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
             nbk += back * back;
             nfw += fwd * fwd;
         }
-        sw += mv * sqrt(nbk);
-        tr += 0.5 * mv * sqrt(nfw);
+        d.nodeval[2 * (size_t)v] = mv * sqrt(nbk);
+        d.nodeval[2 * (size_t)v + 1] = 0.5 * mv * sqrt(nfw);
+    }
+    (void)sw;
+    (void)tr;
+}
+
+// swing and traction partial sums per node chunk (chunks never cross graphs), fixed order
+__global__ __launch_bounds__(GR_BLOCK) void k_stats(const FaDev d) {
+    if (halted(d)) return;
+    const uint32_t c0 = d.chunk[blockIdx.x], c1 = d.chunk[blockIdx.x + 1];
+    double v2[2] = {0.0, 0.0};
+    for (uint32_t v = c0 + threadIdx.x; v < c1; v += GR_BLOCK) {
+        v2[0] = v2[0] + d.nodeval[2 * (size_t)v];
+        v2[1] = v2[1] + d.nodeval[2 * (size_t)v + 1];
     }
     __shared__ double lds[4 * 2];
-    double v2[2] = {sw, tr};
     block_reduce<2, 0>(v2, lds);
     if (threadIdx.x == 0) {
         d.spart[2 * blockIdx.x] = v2[0];
@@ -1456,20 +1517,22 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
 
 // schedule.rs:33-60 on the device, after the swing/traction sums are reduced.
 __global__ __launch_bounds__(GR_BLOCK) void k_estimate(const FaDev d) {
-    GrState *st = d.st;
     if (halted(d)) return;
+    GrGraph *st = d.gr + blockIdx.x;
+    if (st->done) return;  // block-uniform
     __shared__ double lds[4 * 2];
     double v[2] = {0.0, 0.0};
-    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) {
-        v[0] = v[0] + d.spart[2 * b];
-        v[1] = v[1] + d.spart[2 * b + 1];
+    for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
+         c += GR_BLOCK) {
+        v[0] = v[0] + d.spart[2 * c];
+        v[1] = v[1] + d.spart[2 * c + 1];
     }
     block_reduce<2, 0>(v, lds);
     if (threadIdx.x != 0) return;
     st->swing += v[0];
     st->traction += v[1];
     const double swing = st->swing, traction = st->traction;
-    const double n = (double)d.n;
+    const double n = (double)(d.gnode[blockIdx.x + 1] - d.gnode[blockIdx.x]);
     const double jt = d.jitter_tolerance;
     const double opt_jitter = 0.05 * sqrt(n);
     const double min_jitter = sqrt(opt_jitter);
@@ -1495,9 +1558,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_estimate(const FaDev d) {
 // schedule.rs:62-77, fused with the next iteration's bounds/centroid partials.
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
-    const GrState *st = d.st;
     if (halted(d)) return;
-    const double speed = st->speed;
+    const GrGraph &gg = d.gr[d.chunk_g[blockIdx.x]];
+    const double speed = gg.speed;
+    const bool frozen = gg.done != 0;  // an exited graph keeps its positions
     double moved = 0.0;
     double lo[D], hi[D], sum[D], nfin = 0.0;
 #pragma unroll
@@ -1506,7 +1570,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
         hi[a] = -INFINITY;
         sum[a] = 0.0;
     }
-    for (uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x; v < d.n; v += gridDim.x * GR_BLOCK) {
+    const uint32_t c0 = d.chunk[blockIdx.x], c1 = d.chunk[blockIdx.x + 1];
+    for (uint32_t v = c0 + threadIdx.x; v < c1; v += GR_BLOCK) {
+        if (frozen) {
+            double p[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) p[a] = (double)((const T *)d.pos[a])[v];
+            bounds_take<D>(p, lo, hi, sum, nfin);
+            continue;
+        }
         double uv[D], nn = 0.0;
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -1539,11 +1611,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
 
 // state.rs:122-128: the iteration counts, and one that moved less than 1e-10 ends the run.
 __global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
-    GrState *st = d.st;
     if (halted(d)) return;
+    GrGraph *st = d.gr + blockIdx.x;
+    if (st->done) return;  // block-uniform
     __shared__ double lds[4];
     double v[1] = {0.0};
-    for (uint32_t b = threadIdx.x; b < d.nparts; b += GR_BLOCK) v[0] = v[0] + d.mpart[b];
+    for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
+         c += GR_BLOCK)
+        v[0] = v[0] + d.mpart[c];
     block_reduce<1, 0>(v, lds);
     if (threadIdx.x == 0) {
         st->moved = v[0];
@@ -1552,28 +1627,44 @@ __global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
     }
 }
 
+// The session's iteration count and whether every graph has exited.
+__global__ __launch_bounds__(GR_BLOCK) void k_tail_global(const FaDev d) {
+    GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ uint32_t s_live;
+    if (threadIdx.x == 0) s_live = 0;
+    __syncthreads();
+    uint32_t live = 0;
+    for (uint32_t b = threadIdx.x; b < d.ngraphs; b += GR_BLOCK) live += d.gr[b].done ? 0u : 1u;
+    atomicAdd(&s_live, live);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (st->live > 0) st->iters_done += 1;
+        st->live = s_live;
+        if (s_live == 0) st->done = 1;
+    }
+}
+
 __global__ void k_clear_halt(const FaDev d) {
     d.st->halt = 0;
     d.st->pool_overflow = 0;
 }
 
-__global__ void k_tree_reset(const FaDev d) {
-    GrState *st = d.st;
-    if (halted(d)) return;
-    st->pool_used = 0;
-    st->lcell_used = 0;
-    st->span_count = 0;
-    st->any_deep = 0;
-}
 
 // ------------------------------------------------------------------ launchers
+
+template <int D>
+static void head_impl(FaDev &d) {
+    hipLaunchKernelGGL((k_head_graph<D>), dim3(d.ngraphs), dim3(GR_BLOCK), 0, d.stream, d);
+    hipLaunchKernelGGL(k_head_global, dim3(1), dim3(GR_BLOCK), 0, d.stream, d);
+}
 
 template <typename T, int D>
 static void tree_impl(FaDev &d) {
     const uint32_t n = d.n;
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
     hipStream_t s = d.stream;
-    hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+    head_impl<D>(d);
     hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, n, 64, d.sort_tmp, s);
     hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
@@ -1605,7 +1696,7 @@ static void forces_impl(FaDev &d, uint32_t parts) {
         tree_impl<T, D>(d);
         if (parts & PART_REPULSION) walk_impl<T, D, KAHAN>(d);
     } else if (d.repulsion == REP_DENSE) {
-        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        head_impl<D>(d);
         if (parts & PART_ATTRACTION)
             hipLaunchKernelGGL((k_attract_only<T, D, KAHAN>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
         if (parts & PART_REPULSION) {
@@ -1616,23 +1707,25 @@ static void forces_impl(FaDev &d, uint32_t parts) {
         }
         mode = 1;
     } else if (d.repulsion == REP_PM) {
-        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        head_impl<D>(d);
         if (parts & PART_REPULSION) pm_repulsion(d);
     } else {
-        hipLaunchKernelGGL((k_head<D>), dim3(1), dim3(GR_BLOCK), 0, s, d);
+        head_impl<D>(d);
         parts &= ~(uint32_t)PART_REPULSION;
     }
     hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d, mode,
                        parts);
+    hipLaunchKernelGGL(k_stats, dim3(d.nchunks), dim3(GR_BLOCK), 0, s, d);
 }
 
 template <typename T, int D, bool KAHAN>
 static void iteration_impl(FaDev &d) {
     hipStream_t s = d.stream;
     forces_impl<T, D, KAHAN>(d, PART_ATTRACTION | PART_REPULSION | PART_GRAVITY);
-    hipLaunchKernelGGL(k_estimate, dim3(1), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL((k_apply<T, D>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_estimate, dim3(d.ngraphs), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_apply<T, D>), dim3(d.nchunks), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_tail, dim3(d.ngraphs), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_tail_global, dim3(1), dim3(GR_BLOCK), 0, s, d);
 }
 
 #define GR_DISPATCH(FN, ...)                                                                   \
@@ -1657,7 +1750,7 @@ static void iteration_impl(FaDev &d) {
 
 template <typename T, int D, bool KAHAN>
 static void bounds_impl(FaDev &d) {
-    hipLaunchKernelGGL((k_bounds<T, D>), dim3(d.nparts), dim3(GR_BLOCK), 0, d.stream, d);
+    hipLaunchKernelGGL((k_bounds<T, D>), dim3(d.nchunks), dim3(GR_BLOCK), 0, d.stream, d);
 }
 
 template <typename T, int D, bool KAHAN>

@@ -275,9 +275,28 @@ gr_status gr_fa2_params_decode(const uint8_t *bytes, size_t len, gr_fa2_params *
     return ok();
 }
 
-gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
-                                      const uint32_t *target, size_t m,
-                                      const gr_fa2_params *params, const gr_fa2_ext *ext) {
+// A session over `ngraphs` independent graphs laid out one after another: graph b has
+// node_counts[b] nodes and edge_counts[b] raw edges whose endpoints are graph-local. One graph
+// is the reference's Stage; many are the batched layout, all in the same launches.
+static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts,
+                                   const size_t *edge_counts, const uint32_t *source,
+                                   const uint32_t *target, const gr_fa2_params *params,
+                                   const gr_fa2_ext *ext) {
+    if (ngraphs == 0 || !node_counts || !edge_counts) {
+        fail(GR_ERR_PARAM, "graphs", "at least one graph with its node and edge counts");
+        return nullptr;
+    }
+    uint64_t n64 = 0, m64 = 0;
+    for (uint32_t b = 0; b < ngraphs; b++) {
+        n64 += node_counts[b];
+        m64 += edge_counts[b];
+    }
+    if (n64 >= 0xFFFFFFFFull) {  // index.rs: the u32 index space
+        fail(GR_ERR_CAPACITY, "nodes", "node count fits the u32 index space");
+        return nullptr;
+    }
+    const uint32_t n = (uint32_t)n64;
+    const size_t m = (size_t)m64;
     gr_fa2_params p;
     if (params)
         p = *params;
@@ -304,11 +323,34 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
         fail(GR_ERR_PARAM, "edges", "source and target columns are required");
         return nullptr;
     }
-    for (size_t e = 0; e < m; e++)
-        if (source[e] >= n || target[e] >= n) {
-            fail(GR_ERR_PARAM, "edges", "every endpoint is a node index below n");
-            return nullptr;
+    // graph-local endpoints, checked, then offset into the session's node space
+    std::vector<uint32_t> gsrc, gdst;
+    const uint32_t *src_g = source, *dst_g = target;
+    {
+        size_t e = 0;
+        uint32_t off = 0;
+        if (ngraphs > 1) {
+            gsrc.resize(m);
+            gdst.resize(m);
         }
+        for (uint32_t b = 0; b < ngraphs; b++) {
+            for (size_t q = 0; q < edge_counts[b]; q++, e++) {
+                if (source[e] >= node_counts[b] || target[e] >= node_counts[b]) {
+                    fail(GR_ERR_PARAM, "edges", "every endpoint is a node index below n");
+                    return nullptr;
+                }
+                if (ngraphs > 1) {
+                    gsrc[e] = source[e] + off;
+                    gdst[e] = target[e] + off;
+                }
+            }
+            off += node_counts[b];
+        }
+        if (ngraphs > 1) {
+            src_g = gsrc.data();
+            dst_g = gdst.data();
+        }
+    }
     if (x.precision < GR_PREC_FP64 || x.precision > GR_PREC_FP32_KAHAN) {
         fail(GR_ERR_PARAM, "precision", "GR_PREC_FP64, GR_PREC_FP32 or GR_PREC_FP32_KAHAN");
         return nullptr;
@@ -335,8 +377,8 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
             fail(GR_ERR_PARAM, "pm_grid", "a power of two from 64 to 4096");
             return nullptr;
         }
-        if (p.dim != 2) {
-            fail(GR_ERR_UNSUPPORTED, "pm_grid", "the particle mesh runs in 2D");
+        if (p.dim != 2 || ngraphs != 1) {
+            fail(GR_ERR_UNSUPPORTED, "pm_grid", "the particle mesh runs on one 2D graph");
             return nullptr;
         }
         x.pm_grid = side;
@@ -370,7 +412,7 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     }
 
     SimpleGraph g;
-    build_simple(n, source, target, m, x.edge_weights, g);
+    build_simple(n, src_g, dst_g, m, x.edge_weights, g);
     s->m_simple = (uint32_t)g.lo.size();
     s->lo = g.lo;
     s->hi = g.hi;
@@ -400,6 +442,37 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     TRY_ALLOC(d.nbr, (g.nbr.size() + 1) * 4);
     if (d.weighted) TRY_ALLOC(d.w, (g.nbr.size() + 1) * es);
     TRY_ALLOC(d.st, sizeof(GrState));
+    // batch tables: node offsets, graph-aligned node chunks, node -> graph
+    d.ngraphs = ngraphs;
+    {
+        const uint32_t gbits = ngraphs > 1 ? (uint32_t)std::ceil(std::log2((double)ngraphs)) : 0;
+        const uint32_t maxl = d.dim == 2 ? 31 : 21;
+        const uint32_t fit = (63 - gbits) / d.dim;
+        d.levels = fit < maxl ? fit : maxl;
+    }
+    std::vector<uint32_t> gnode(ngraphs + 1, 0), gchunk(ngraphs + 1, 0), chunk(1, 0), chunk_g;
+    const uint32_t CHUNK = 2048;
+    for (uint32_t b = 0; b < ngraphs; b++) {
+        gnode[b + 1] = gnode[b] + node_counts[b];
+        for (uint32_t v = gnode[b]; v < gnode[b + 1]; v += CHUNK) {
+            chunk.push_back(std::min(v + CHUNK, gnode[b + 1]));
+            chunk_g.push_back(b);
+        }
+        gchunk[b + 1] = (uint32_t)chunk_g.size();
+    }
+    if (chunk_g.empty()) {  // no nodes at all: one empty chunk keeps every grid non-empty
+        chunk.push_back(0);
+        chunk_g.push_back(0);
+        gchunk[ngraphs] = 1;
+    }
+    d.nchunks = (uint32_t)chunk_g.size();
+    TRY_ALLOC(d.gr, (size_t)ngraphs * sizeof(GrGraph));
+    TRY_ALLOC(d.gnode, (ngraphs + 1) * 4);
+    TRY_ALLOC(d.gchunk, (ngraphs + 1) * 4);
+    TRY_ALLOC(d.chunk, (d.nchunks + 1) * 4);
+    TRY_ALLOC(d.chunk_g, d.nchunks * 4);
+    TRY_ALLOC(d.nodeval, nn * 2 * 8);
+    if (ngraphs > 1) TRY_ALLOC(d.gidn, nn * 4);
     TRY_ALLOC(d.keys, nn * 8);
     TRY_ALLOC(d.keys_alt, nn * 8);
     TRY_ALLOC(d.vals, nn * 4);
@@ -424,9 +497,9 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     TRY_ALLOC(d.lc_sum, (size_t)d.lcell_cap * (1 + d.dim) * 8);
     d.nparts = gr_xcd_grid(gr_blocks(nn, GR_BLOCK));  // a multiple of the XCD count
     if (d.nparts > 2048) d.nparts = 2048;
-    TRY_ALLOC(d.bpart, (size_t)d.nparts * 10 * 8);
-    TRY_ALLOC(d.spart, (size_t)d.nparts * 2 * 8);
-    TRY_ALLOC(d.mpart, (size_t)d.nparts * 8);
+    TRY_ALLOC(d.bpart, (size_t)d.nchunks * 10 * 8);
+    TRY_ALLOC(d.spart, (size_t)d.nchunks * 2 * 8);
+    TRY_ALLOC(d.mpart, (size_t)d.nchunks * 8);
     TRY_ALLOC(d.scan_tmp, gr::scan_tmp_bytes((uint32_t)nn));
     TRY_ALLOC(d.sort_tmp, gr::radix_tmp_bytes((uint32_t)nn));
 #undef TRY_ALLOC
@@ -483,8 +556,29 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     }
     GrState init;
     std::memset(&init, 0, sizeof(init));
-    init.swing = init.traction = init.speed = init.speed_efficiency = 1.0;
+    std::vector<GrGraph> ginit(ngraphs);
+    uint32_t live = 0;
+    for (uint32_t b = 0; b < ngraphs; b++) {
+        GrGraph &gg = ginit[b];
+        std::memset(&gg, 0, sizeof(gg));
+        gg.swing = gg.traction = gg.speed = gg.speed_efficiency = 1.0;  // state.rs:90-93
+        gg.done = node_counts[b] == 0 ? 1u : 0u;  // an empty graph has nothing to run
+        live += gg.done ? 0u : 1u;
+    }
+    init.live = live;
+    init.done = live == 0 ? 1u : 0u;
     good = good && upload(d.st, &init, sizeof(init));
+    good = good && upload(d.gr, ginit.data(), ginit.size() * sizeof(GrGraph)) &&
+           upload(d.gnode, gnode.data(), gnode.size() * 4) &&
+           upload(d.gchunk, gchunk.data(), gchunk.size() * 4) &&
+           upload(d.chunk, chunk.data(), chunk.size() * 4) &&
+           upload(d.chunk_g, chunk_g.data(), chunk_g.size() * 4);
+    if (ngraphs > 1) {
+        std::vector<uint32_t> gidn(n);
+        for (uint32_t b = 0; b < ngraphs; b++)
+            for (uint32_t v = gnode[b]; v < gnode[b + 1]; v++) gidn[v] = b;
+        good = good && upload(d.gidn, gidn.data(), (size_t)n * 4);
+    }
     if (!good) {
         fail(GR_ERR_DEVICE, "hipMemcpy", "upload failed");
         delete s;
@@ -493,9 +587,12 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     std::vector<double> start((size_t)n * d.dim);
     if (x.start) {
         std::memcpy(start.data(), x.start, start.size() * 8);
-    } else {  // state.rs:272-283
-        uint32_t rs = p.seed;
-        for (size_t q = 0; q < start.size(); q++) start[q] = mulberry32_next(rs);
+    } else {  // state.rs:272-283, each graph its own stream from the same seed
+        for (uint32_t b = 0; b < ngraphs; b++) {
+            uint32_t rs = p.seed;
+            for (size_t q = (size_t)gnode[b] * d.dim; q < (size_t)gnode[b + 1] * d.dim; q++)
+                start[q] = mulberry32_next(rs);
+        }
     }
     if (n > 0 && s->upload_positions(start.data()) != GR_OK) {
         delete s;
@@ -503,6 +600,19 @@ gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
     }
     ok();
     return s;
+}
+
+gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
+                                      const uint32_t *target, size_t m,
+                                      const gr_fa2_params *params, const gr_fa2_ext *ext) {
+    return create_impl(1, &n, &m, source, target, params, ext);
+}
+
+gr_fa2_session *gr_fa2_batch_create(uint32_t ngraphs, const uint32_t *node_counts,
+                                    const size_t *edge_counts, const uint32_t *source,
+                                    const uint32_t *target, const gr_fa2_params *params,
+                                    const gr_fa2_ext *ext) {
+    return create_impl(ngraphs, node_counts, edge_counts, source, target, params, ext);
 }
 
 void gr_fa2_session_release(gr_fa2_session *s) { delete s; }
@@ -586,19 +696,24 @@ gr_status gr_fa2_session_forces(gr_fa2_session *s, uint32_t parts, double *out) 
     return r ? r : ok();
 }
 
-gr_status gr_fa2_session_scalars(gr_fa2_session *s, double *out7) {
+gr_status gr_fa2_session_graph_scalars(gr_fa2_session *s, uint32_t graph, double *out7) {
     if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
-    GrState st;
-    gr_status r = s->read_state(st);
-    if (r) return r;
-    out7[0] = st.swing;
-    out7[1] = st.traction;
-    out7[2] = st.speed;
-    out7[3] = st.speed_efficiency;
-    out7[4] = st.iter_no;
-    out7[5] = st.iters_done;
-    out7[6] = st.done;
+    if (graph >= s->d.ngraphs) return fail(GR_ERR_PARAM, "graph", "a graph of the session");
+    GrGraph g;
+    HIP_TRY(hipMemcpyAsync(&g, s->d.gr + graph, sizeof(g), hipMemcpyDeviceToHost, s->d.stream));
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
+    out7[0] = g.swing;
+    out7[1] = g.traction;
+    out7[2] = g.speed;
+    out7[3] = g.speed_efficiency;
+    out7[4] = g.iter_no;
+    out7[5] = g.iters_done;
+    out7[6] = g.done;
     return ok();
+}
+
+gr_status gr_fa2_session_scalars(gr_fa2_session *s, double *out7) {
+    return gr_fa2_session_graph_scalars(s, 0, out7);
 }
 
 gr_status gr_fa2_session_graph(gr_fa2_session *s, uint32_t *m_out, uint32_t *lo, uint32_t *hi,

@@ -6,30 +6,40 @@
 #define GR_WAVE 64
 #define GR_BLOCK 256
 
-// Per-session scalars kept resident on the device: the adaptive-speed state of
-// layout/forceatlas2/state.rs:151-171 (swing, traction, speed, speed_efficiency, iter_no) plus
-// the bookkeeping a launch-ahead iteration loop needs (early-exit flag, overflow flags, the
-// tree sizes of the current iteration).
-struct GrState {
+// One graph's scalars, resident on the device: the adaptive-speed state of
+// layout/forceatlas2/state.rs:151-171 (swing, traction, speed, speed_efficiency, iter_no),
+// its centroid, extent and root square, and where its tree sits in the session's arena.
+// A single-graph session is a batch of one.
+struct GrGraph {
     double swing, traction, speed, speed_efficiency;
-    double moved;         // last applied step's sum |dp| (schedule.rs:62-77)
-    double mean[3];       // centroid of the current positions (state.rs:208-220)
+    double moved;                   // last applied step's sum |dp| (schedule.rs:62-77)
+    double mean[3];                 // centroid of the current positions (state.rs:208-220)
     double root_lo[3], root_hi[3];  // root square of the current tree (quadtree.rs:157-195)
     double box_lo[3], box_hi[3];    // extent of the finite positions
-    uint32_t iter_no;     // jiggle tick, incremented once per iteration
-    uint32_t iters_done;  // iterations applied by this session
-    uint32_t done;        // set once an iteration moved < 1e-10 (state.rs:122-128)
-    uint32_t halt;        // set when a capacity check fails: the iteration is not applied
-    uint32_t n_finite;    // points inside the tree (finite coordinates)
-    uint32_t n_groups;    // distinct-key groups of the current tree
-    uint32_t n_cells;     // cells of the current tree
+    uint32_t iter_no;               // jiggle tick, incremented once per iteration
+    uint32_t iters_done;            // iterations applied to this graph
+    uint32_t done;                  // set once an iteration moved < 1e-10 (state.rs:122-128)
+    uint32_t n_finite;              // its points inside the tree
+    uint32_t cell_lo, cell_hi;      // its cells in the arena: [cell_lo, cell_hi)
+    uint32_t has_root, pad;
+};
+
+// Session-wide bookkeeping of the launch-ahead iteration loop.
+struct GrState {
+    uint32_t iters_done;  // iteration launches that moved at least one graph
+    uint32_t done;        // every graph has exited early
+    uint32_t halt;        // a capacity check failed: the iteration is not applied
+    uint32_t n_finite;    // points inside the trees (all graphs)
+    uint32_t n_groups;    // distinct-key groups of the current trees
+    uint32_t n_cells;     // cells of the current trees
     uint32_t need_cells;  // cells the current iteration asked for (capacity check)
     uint32_t pool_used;   // deep-group scratch used
     uint32_t pool_overflow;
     uint32_t any_deep;
-    uint32_t has_root;    // at least one finite point
     uint32_t lcell_used;  // deep-group local cells used
-    uint32_t span_count;  // cells whose subtree crosses a centre-of-mass tile
+    uint32_t span_count;  // unused
+    uint32_t live;        // graphs not yet done
+    uint32_t pad[3];
 };
 
 __device__ __forceinline__ uint64_t gr_ballot(bool p) { return __ballot(p); }

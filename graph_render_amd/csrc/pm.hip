@@ -37,13 +37,14 @@ __device__ __forceinline__ double pm_h(int step) {
 
 __global__ void k_pm_frame(const FaDev d) {
     const GrState *st = d.st;
+    const GrGraph *g = d.gr;  // the particle mesh runs on single-graph sessions
     PmFrame *f = (PmFrame *)d.pm_fr;
     f->valid = 0;
     f->rebuild = 0;
     if (st->halt) return;
     const int side = (int)d.pm_side;
-    const double span = fmax(st->box_hi[0] - st->box_lo[0], st->box_hi[1] - st->box_lo[1]);
-    if (d.n < 2 || st->n_finite == 0 || !isfinite(span)) return;
+    const double span = fmax(g->box_hi[0] - g->box_lo[0], g->box_hi[1] - g->box_lo[1]);
+    if (d.n < 2 || g->n_finite == 0 || !isfinite(span)) return;
     double s0 = floor(4.0 * log2(span / (double)side));
     int step = s0 < -32.0 ? -32 : (int)s0;
     for (int guard = 0; guard < 4096; guard++, step++) {
@@ -56,8 +57,8 @@ __global__ void k_pm_frame(const FaDev d) {
         f->h = h;
         f->cells = c;
         f->reach = reach;
-        f->ox = floor(st->box_lo[0] / h) * h;
-        f->oy = floor(st->box_lo[1] / h) * h;
+        f->ox = floor(g->box_lo[0] / h) * h;
+        f->oy = floor(g->box_lo[1] / h) * h;
         f->valid = 1;
         f->rebuild = !(f->kvalid && f->kstep == step && f->kreach == reach);
         return;

@@ -313,6 +313,38 @@ class Session:
     def __exit__(self, *exc):
         self.close()
 
+    @classmethod
+    def batch(cls, topologies, params: Optional[Fa2Params] = None,
+              ext: Optional[Fa2Ext] = None) -> "Session":
+        """Many independent graphs in one session, laid out by the same launches. Positions,
+        weights and starts are the graphs' concatenated in order."""
+        self = cls.__new__(cls)
+        self.params = params or Fa2Params()
+        self.ext = ext or Fa2Ext()
+        self.dim = int(self.params.dim)
+        counts = np.array([t.n for t in topologies], np.uint32)
+        edges = np.array([t.m for t in topologies], np.uint64)
+        self.n = int(counts.sum())
+        self.node_counts = counts
+        src = np.concatenate([t.source for t in topologies] + [np.zeros(1, np.uint32)])
+        dst = np.concatenate([t.target for t in topologies] + [np.zeros(1, np.uint32)])
+        m = int(edges.sum())
+        xc = self.ext._c(self.n, m, self.dim)
+        cp = self.params._c()
+        self._h = A.lib().gr_fa2_batch_create(len(topologies), counts.ctypes.data,
+                                              edges.ctypes.data, src.ctypes.data,
+                                              dst.ctypes.data, C.byref(cp), C.addressof(xc))
+        if not self._h:
+            _raise()
+        return self
+
+    def graph_scalars(self, graph: int) -> dict:
+        out = np.zeros(7)
+        _check(A.lib().gr_fa2_session_graph_scalars(self._h, int(graph), out.ctypes.data))
+        keys = ("swing", "traction", "speed", "speed_efficiency", "iter_no", "iters_done",
+                "done")
+        return dict(zip(keys, out.tolist()))
+
     def step(self, iters: int) -> int:
         done = C.c_uint32(0)
         _check(A.lib().gr_fa2_session_step(self._h, int(iters), C.byref(done)))

@@ -111,6 +111,16 @@ typedef struct gr_fa2_session gr_fa2_session;
 gr_fa2_session *gr_fa2_session_create(uint32_t n, const uint32_t *source,
                                       const uint32_t *target, size_t m,
                                       const gr_fa2_params *params, const gr_fa2_ext *ext);
+/* Batched layout: `ngraphs` independent graphs in one session, run by the same launches.
+ * Graph b has node_counts[b] nodes and edge_counts[b] raw edges; source/target hold every
+ * graph's edges one graph after another, with graph-local endpoints. Edge weights and start
+ * positions (gr_fa2_ext) are concatenated the same way. Each graph keeps its own swing,
+ * traction, speed and early exit; positions() returns all graphs' nodes in order. No
+ * reference counterpart (the reference lays out one graph per call). */
+gr_fa2_session *gr_fa2_batch_create(uint32_t ngraphs, const uint32_t *node_counts,
+                                    const size_t *edge_counts, const uint32_t *source,
+                                    const uint32_t *target, const gr_fa2_params *params,
+                                    const gr_fa2_ext *ext);
 void gr_fa2_session_release(gr_fa2_session *s);
 
 /* Runs up to `iters` iterations of iterate() (state.rs:130-148), stopping early once an
@@ -126,8 +136,10 @@ gr_status gr_fa2_session_set_positions(gr_fa2_session *s, const double *in);
  * 1 attraction, 2 repulsion, 4 gravity, summed in iterate()'s order. out: n*dim f64. */
 gr_status gr_fa2_session_forces(gr_fa2_session *s, uint32_t parts, double *out);
 
-/* swing, traction, speed, speed_efficiency, iter_no, iterations applied, early-exit flag */
+/* swing, traction, speed, speed_efficiency, iter_no, iterations applied, early-exit flag
+ * (graph 0; gr_fa2_session_graph_scalars for graph b of a batch) */
 gr_status gr_fa2_session_scalars(gr_fa2_session *s, double *out7);
+gr_status gr_fa2_session_graph_scalars(gr_fa2_session *s, uint32_t graph, double *out7);
 
 /* Simple edges (lo < hi, first raw edge kept) and per-node mass, as the session holds them. */
 gr_status gr_fa2_session_graph(gr_fa2_session *s, uint32_t *m_out, uint32_t *lo, uint32_t *hi,

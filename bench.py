@@ -73,25 +73,177 @@ def cpu_baseline(topo, params, start, weights, budget_s: float):
                       f"(oracle/fa2_oracle.c, gcc -O2, one thread; {setup:.1f}s setup untimed)"}
 
 
+def batch_cpu_baseline(topos, starts, offs, params, budget_s: float):
+    """The oracle on every 32nd graph of the batch (FP64, one thread, theta 1.0), eight
+    iterations each after an untimed set-up; the batch's CPU time per iteration is the
+    sample's time scaled by total nodes over sampled nodes."""
+    from oracle import oracle as O
+    sample = list(range(0, len(topos), 32))
+    spent, nodes, its = 0.0, 0, 8
+    for b in sample:
+        t = topos[b]
+        st = O.Fa2(t.n, t.source, t.target, start=starts[offs[b]:offs[b] + t.n], tree=True,
+                   theta2=1.0, scaling_ratio=params.scaling_ratio, gravity=params.gravity,
+                   jitter_tolerance=params.jitter_tolerance)
+        t0 = time.perf_counter()
+        for _ in range(its):
+            st.iterate()
+        spent += (time.perf_counter() - t0) / its
+        nodes += t.n
+        if spent * its > budget_s:
+            break
+    total = sum(t.n for t in topos)
+    per_batch_iter = spent * total / nodes
+    return {"value": 1.0 / per_batch_iter, "unit": "batch iters/s", "cores": 1, "kind": "port",
+            "sample": f"{its} FA2-BH FP64 iterations on each of {len(sample)} graphs (every "
+                      f"32nd, {nodes} nodes), scaled by {total}/{nodes} nodes "
+                      f"(oracle/fa2_oracle.c, gcc -O2, one thread)"}
+
+
+def bench_batch(args):
+    """4096 graphs (log-uniform 50..20000 nodes, half preferential attachment, half
+    Erdos-Renyi) laid out together in one session, FP32, theta 1.0: batch iterations/s
+    (one iteration advances every graph), graph-iterations/s and node-iterations/s."""
+    import graph_render_amd as G
+    from graph_render_amd import synth
+    t0 = time.perf_counter()
+    sizes, srcs, dsts, starts = synth.batch_workload(args.graphs)
+    topos = [G.Topology(n, s, t) for n, s, t in zip(sizes, srcs, dsts)]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    gen_s = time.perf_counter() - t0
+    params = G.Fa2Params(max_iter=args.warmup + args.steps)
+    t0 = time.perf_counter()
+    bs = G.Session.batch(topos, params, G.Fa2Ext(theta=1.0, precision="fp32", start=starts))
+    create_s = time.perf_counter() - t0
+    bs.step(args.warmup)
+    t0 = time.perf_counter()
+    done = bs.step(args.steps)
+    el = time.perf_counter() - t0
+    live = sum(1 for b in range(len(topos)) if not bs.graph_scalars(b)["done"])
+    walk_ms = bs.time("walk", 5)
+    tree_ms = bs.time("tree", 3)
+    it_ms = bs.time("iteration", 3)
+    bs.close()
+    its = done / el
+    n = int(offs[-1])
+    cpu = None if args.no_cpu else batch_cpu_baseline(topos, starts, offs, params,
+                                                      args.cpu_seconds)
+    line = {
+        "metric": "FA2-BH batched layout, batch iterations/s (every graph advances one "
+                  "iteration)",
+        "value": round(its, 3), "unit": "batch iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 / its, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(topos)} graphs, log-uniform 50..20000 nodes, even "
+                               "graphs preferential attachment m 1..6, odd Erdos-Renyi mean "
+                               "degree 4..12, seeds 0..4095, starts U[-100,100]^2, 2D FP32, "
+                               "theta 1.0",
+                   "graphs": len(topos), "nodes": n,
+                   "raw_edges": int(sum(len(s) for s in srcs))},
+        "graph_iters_per_s": round(its * len(topos), 1),
+        "node_iters_per_s": round(its * n, 1),
+        "graphs_still_moving": live, "walk_ms": round(walk_ms, 4),
+        "tree_build_ms": round(tree_ms, 4), "iteration_ms": round(it_ms, 4),
+        "generate_s": round(gen_s, 2), "session_create_s": round(create_s, 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def rel_errors(approx, exact):
+    """Per-node relative error |a - e| / |e| (max and mean over nodes with e != 0) and the
+    whole-field ratio sqrt(sum |a - e|^2 / sum |e|^2)."""
+    d = np.sqrt(((approx - exact) ** 2).sum(1))
+    e = np.sqrt((exact ** 2).sum(1))
+    ok = e > 0
+    r = d[ok] / e[ok]
+    return {"max": float(r.max()), "mean": float(r.mean()),
+            "whole_field": float(np.sqrt((d ** 2).sum() / (e ** 2).sum()))}
+
+
+def bench_precision(args):
+    """3D FA2-BH (octree) precision study on a 250K-node random geometric graph (unit cube,
+    mean degree 8, seed 2): FP64, FP32 and FP32-Kahan forces at theta 0 and 0.8 against the
+    exact O(n^2) FP64 pair sum, and the position drift after `steps` iterations against the
+    exact FP64 layout from the same start (drift in units of the exact layout's extent)."""
+    import graph_render_amd as G
+    from graph_render_amd import synth
+    n = args.nodes or 250_000
+    s, t, _ = synth.random_geometric_3d(n, 8.0, 2)
+    topo = G.Topology(n, s, t)
+    iters = args.steps
+    params = G.Fa2Params(max_iter=iters, dim=3, seed=2)
+    with G.Session(topo, params, G.Fa2Ext(repulsion="dense", precision="fp64")) as ex:
+        start = ex.positions()
+        f_exact = ex.forces(7)
+        t0 = time.perf_counter()
+        ex.step(iters)
+        exact_its = iters / (time.perf_counter() - t0)
+        p_exact = ex.positions()
+    extent = float((p_exact.max(0) - p_exact.min(0)).max())
+    rows = []
+    for theta in (0.0, 0.8):
+        f64 = None
+        for prec in ("fp64", "fp32", "fp32_kahan"):
+            with G.Session(topo, params, G.Fa2Ext(theta=theta, precision=prec,
+                                                  start=start)) as ss:
+                f = ss.forces(7)
+                t0 = time.perf_counter()
+                d = ss.step(iters)
+                its = d / (time.perf_counter() - t0)
+                p = ss.positions()
+            if prec == "fp64":
+                f64 = f
+            drift = np.sqrt(((p - p_exact) ** 2).sum(1)) / extent
+            rows.append({"theta": theta, "precision": prec, "iters_per_s": round(its, 3),
+                         "force_err_vs_exact_fp64": rel_errors(f, f_exact),
+                         "force_err_vs_fp64_same_theta": rel_errors(f, f64),
+                         "drift_after_iters": {"max": float(drift.max()),
+                                               "mean": float(drift.mean())}})
+    head = next(r for r in rows if r["theta"] == 0.8 and r["precision"] == "fp32")
+    line = {
+        "metric": "FA2-BH 3D precision study: force error vs exact FP64 and position drift",
+        "value": head["iters_per_s"], "unit": "iters/s", "n_gpus": 1, "steps": iters,
+        "warmup": 0, "ms_per_step": round(1e3 / head["iters_per_s"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"FA2-BH 3D octree, random geometric graph in the unit cube, "
+                               f"{n} nodes, mean degree 8, seed 2, Mulberry32 start seed 2",
+                   "nodes": n, "edges": int(s.size), "edges_sha256": synth.digest(s, t)},
+        "exact_fp64_dense_iters_per_s": round(exact_its, 3),
+        "exact_layout_extent": extent,
+        "rows": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nodes", type=int, default=1_000_000)
+    ap.add_argument("--nodes", type=int, default=None)
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["sbm1m", "batch", "precision"], default="sbm1m")
+    ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    if args.workload == "precision":
+        bench_precision(args)
+        return
 
     import graph_render_amd as G
     from graph_render_amd import synth
 
-    n = args.nodes
+    n = args.nodes or 1_000_000
     s, t, w = synth.sbm_zipf(n, 64, 1.1, 10.0, seed=1)
     start = synth.uniform_start(n, 2, seed=1)
     topo = G.Topology(n, s, t)

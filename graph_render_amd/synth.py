@@ -109,36 +109,67 @@ def sbm_equal(n: int, blocks: int, p_in: float, p_out: float, seed: int):
 
 
 def random_geometric_3d(n: int, mean_degree: float, seed: int):
-    """Random geometric graph in the unit cube: radius r with n*(4/3)*pi*r^3 = mean_degree."""
+    """Random geometric graph in the unit cube: points uniform (seeded), an edge for every
+    pair closer than r, with n*(4/3)*pi*r^3 = mean_degree. Edges sorted by (lo, hi)."""
+    from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
     p = rng.random((n, 3))
     r = (mean_degree / (n * 4.0 / 3.0 * np.pi)) ** (1.0 / 3.0)
-    g = max(1, int(1.0 / r))
-    cell = np.minimum((p * g).astype(np.int64), g - 1)
-    cid = (cell[:, 0] * g + cell[:, 1]) * g + cell[:, 2]
-    order = np.argsort(cid, kind="stable")
-    cid_sorted = cid[order]
-    starts = np.searchsorted(cid_sorted, np.arange(g ** 3))
-    ends = np.searchsorted(cid_sorted, np.arange(g ** 3), side="right")
-    src, dst = [], []
-    offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
-    for cx in range(g):
-        for cy in range(g):
-            for cz in range(g):
-                a = order[starts[(cx * g + cy) * g + cz]:ends[(cx * g + cy) * g + cz]]
-                if a.size == 0:
-                    continue
-                for dx, dy, dz in offs:
-                    nx_, ny_, nz_ = cx + dx, cy + dy, cz + dz
-                    if not (0 <= nx_ < g and 0 <= ny_ < g and 0 <= nz_ < g):
-                        continue
-                    c = (nx_ * g + ny_) * g + nz_
-                    b = order[starts[c]:ends[c]]
-                    if b.size == 0:
-                        continue
-                    d2 = ((p[a, None, :] - p[None, b, :]) ** 2).sum(-1)
-                    ii, jj = np.nonzero((d2 <= r * r))
-                    sel = a[ii] < b[jj]
-                    src.append(a[ii][sel])
-                    dst.append(b[jj][sel])
-    return (np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32), p)
+    pairs = cKDTree(p).query_pairs(r, output_type="ndarray").astype(np.int64)
+    pairs.sort(axis=1)
+    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    return pairs[:, 0].astype(np.uint32), pairs[:, 1].astype(np.uint32), p
+
+
+def barabasi_albert_fast(n: int, m: int, rng) -> tuple:
+    """Preferential attachment, vectorised: node t >= m adds m edges whose targets are
+    uniform picks from the endpoint list of all earlier edges (degree-proportional), the
+    first new node linking to nodes 0..m-1. Picks that repeat a target are dropped later by
+    the simple-graph dedup. Endpoint picks that land on an earlier edge's target are
+    resolved by pointer doubling."""
+    m = max(1, min(m, n - 1)) if n > 1 else 0
+    if n <= 1 or m == 0:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    E = (n - m) * m
+    e = np.arange(E, dtype=np.int64)
+    t = m + e // m
+    first = t == m
+    pool = 2 * (t - m) * m
+    u = np.floor(rng.random(E) * np.maximum(pool, 1)).astype(np.int64)
+    half = u // 2
+    even = (u % 2) == 0
+    val = np.where(first, e % m, np.where(even, m + half // m, -1)).astype(np.int64)
+    ptr = np.where(~first & ~even, half, -1).astype(np.int64)
+    todo = np.nonzero(ptr >= 0)[0]
+    while todo.size:
+        p = ptr[todo]
+        got = val[p] >= 0
+        jump = ptr[p[~got]]  # taken before this round clears the pointers it resolves
+        val[todo[got]] = val[p[got]]
+        ptr[todo[got]] = -1
+        rest = todo[~got]
+        ptr[rest] = jump
+        todo = rest
+    return t.astype(np.uint32), val.astype(np.uint32)
+
+
+def batch_workload(count: int = 4096, lo: int = 50, hi: int = 20000):
+    """The batched benchmark: graph b (seed b) has log-uniform node count in [lo, hi]; even b
+    are preferential-attachment graphs with m uniform in 1..6, odd b Erdos-Renyi graphs with
+    mean degree uniform in [4, 12]; starts uniform in [-100, 100]^2."""
+    sizes, srcs, dsts, starts = [], [], [], []
+    for b in range(count):
+        rng = np.random.default_rng(b)
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        if b % 2 == 0:
+            s, t = barabasi_albert_fast(n, int(rng.integers(1, 7)), rng)
+        else:
+            mdeg = rng.uniform(4.0, 12.0)
+            k = int(round(n * mdeg / 2))
+            s = rng.integers(0, n, k).astype(np.uint32)
+            t = rng.integers(0, n, k).astype(np.uint32)
+        sizes.append(n)
+        srcs.append(s)
+        dsts.append(t)
+        starts.append(rng.uniform(-100.0, 100.0, (n, 2)))
+    return sizes, srcs, dsts, np.concatenate(starts)

@@ -217,6 +217,114 @@ def bench_precision(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args):
+    """Reverse mode through unrolled FA2-BH: 10K-node SBM (16 equal blocks, p_in 0.01,
+    p_out 0.0005, seed 3), iterations recorded with a checkpoint every 20, loss = Kruskal
+    stress-1 against BFS hops from 256 evenly spaced sources. Times forward and reverse over
+    `steps` iterations in FP32 and FP64. Then, at horizons of 10 .. `steps` iterations,
+    compares d loss / d weight on 64 random edges, d scaling ratio and d gravity with
+    central differences of the FP64 forward at h 1e-3 and 1e-6: the two step sizes agreeing
+    with each other is what says the finite difference measures a derivative at all."""
+    import torch
+    import graph_render_amd as G
+    from graph_render_amd import autograd as AG
+    from graph_render_amd import synth
+    n = args.nodes or 10_000
+    s, t = synth.sbm_equal(n, 16, 0.01, 0.0005, seed=3)
+    topo = G.Topology(n, s, t)
+    every = 20
+    srcs = AG.stress_sources(n, 256)
+    hops = AG.bfs_hops(n, s, t, srcs)
+    w0 = np.ones(s.size)
+
+    def reverse(precision, iters):
+        spec = AG.UnrolledSpec(topology=topo, iters=iters, checkpoint_every=every,
+                               precision=precision, repulsion="tree")
+        t0 = time.perf_counter()
+        sess, pos = AG.run_forward(spec, w0, 2.0, 1.0, record=True)
+        t_fwd = time.perf_counter() - t0
+        pt = torch.tensor(pos, requires_grad=True)
+        loss = AG.stress1(pt, srcs, hops)
+        loss.backward()
+        t0 = time.perf_counter()
+        g = sess.adjoint(pt.grad.numpy())
+        t_bwd = time.perf_counter() - t0
+        sess.close()
+        return float(loss.detach()), g, t_fwd, t_bwd, spec.applied[-1]
+
+    L32, g32, f32, b32, a32 = reverse("fp32", args.steps)
+    L64, g64, f64, b64, a64 = reverse("fp64", args.steps)
+
+    # raw edges the simple graph keeps (first occurrence of each unordered pair)
+    lo, hi = np.minimum(s, t).astype(np.int64), np.maximum(s, t).astype(np.int64)
+    _, first = np.unique(lo * n + hi, return_index=True)
+    first = first[lo[first] != hi[first]]
+    rng = np.random.default_rng(3)
+    pick = np.sort(rng.choice(first, 64, replace=False))
+
+    def vec_err(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    horizons = [h for h in (10, 25, 50, 100, 200) if h < args.steps] + [args.steps]
+    rows = []
+    t_fd = time.perf_counter()
+    for H in horizons:
+        def loss64(w, k=2.0, g=1.0):
+            spec = AG.UnrolledSpec(topology=topo, iters=H, precision="fp64", repulsion="tree")
+            _, pos = AG.run_forward(spec, w, k, g)
+            return AG.stress1_np(pos, srcs, hops)
+
+        fds = {}
+        for h in (1e-3, 1e-6):
+            fw = []
+            for e in pick:
+                wp, wm = w0.copy(), w0.copy()
+                wp[e] += h
+                wm[e] -= h
+                fw.append((loss64(wp) - loss64(wm)) / (2 * h))
+            fk = (loss64(w0, k=2.0 + h) - loss64(w0, k=2.0 - h)) / (2 * h)
+            fg = (loss64(w0, g=1.0 + h) - loss64(w0, g=1.0 - h)) / (2 * h)
+            fds[h] = (np.array(fw), fk, fg)
+        ga64 = g64 if H == args.steps else reverse("fp64", H)[1]
+        ga32 = g32 if H == args.steps else reverse("fp32", H)[1]
+        fw3, fk3, fg3 = fds[1e-3]
+        fw6 = fds[1e-6][0]
+        row = {"iterations": H,
+               "grad_norm_fp64_adjoint": float(np.linalg.norm(ga64["weights"][pick])),
+               "fd_h1e-3_vs_fd_h1e-6": vec_err(fw3, fw6)}
+        for name, ga in (("fp64", ga64), ("fp32", ga32)):
+            row[f"adjoint_{name}_vs_fd_h1e-3"] = {
+                "weights_vector_rel_err": vec_err(ga["weights"][pick], fw3),
+                "weights_max_abs_err_over_max_fd":
+                    float(np.abs(ga["weights"][pick] - fw3).max() / np.abs(fw3).max()),
+                "scaling_ratio_rel_err": float(abs(ga["scaling_ratio"] - fk3) / abs(fk3)),
+                "gravity_rel_err": float(abs(ga["gravity"] - fg3) / abs(fg3))}
+        rows.append(row)
+        print(f"horizon {H}: fd(1e-3) vs fd(1e-6) {row['fd_h1e-3_vs_fd_h1e-6']:.2e}, "
+              f"fp64 adjoint vs fd(1e-3) "
+              f"{row['adjoint_fp64_vs_fd_h1e-3']['weights_vector_rel_err']:.2e}",
+              file=sys.stderr, flush=True)
+    fd_s = time.perf_counter() - t_fd
+
+    its = args.steps / (f32 + b32)
+    line = {
+        "metric": "FA2-BH adjoint: unrolled iterations/s, forward + reverse (checkpoint 20)",
+        "value": round(its, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 0, "ms_per_step": round(1e3 / its, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"SBM {n} nodes, 16 equal blocks, p_in 0.01, p_out 0.0005, "
+                               f"seed 3; FA2-BH theta 0.8, {args.steps} iterations, checkpoint "
+                               "every 20; loss Kruskal stress-1 vs BFS hops from 256 sources",
+                   "nodes": n, "raw_edges": int(s.size), "edges_sha256": synth.digest(s, t)},
+        "loss_fp32": L32, "loss_fp64": L64, "iterations_applied": [a32, a64],
+        "fp32": {"forward_s": round(f32, 4), "reverse_s": round(b32, 4)},
+        "fp64": {"forward_s": round(f64, 4), "reverse_s": round(b64, 4)},
+        "fd": {"edges": 64, "step_sizes": [1e-3, 1e-6], "seconds": round(fd_s, 2)},
+        "horizons": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +334,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["sbm1m", "batch", "precision"], default="sbm1m")
+    ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint"],
+                    default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -238,6 +347,9 @@ def main():
         return
     if args.workload == "precision":
         bench_precision(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
 
     import graph_render_amd as G

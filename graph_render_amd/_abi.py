@@ -65,6 +65,10 @@ SIGNATURES = {
     "gr_fa2_session_tree": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "gr_fa2_session_record": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_uint32)]),
+    "gr_fa2_session_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
     "gr_fa2_session_time": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32,
                                       C.POINTER(C.c_double)]),
     "gr_last_error": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),

@@ -117,4 +117,40 @@ void launch_walk(FaDev &d);
 // Clears the halt flag left by a failed capacity check.
 void launch_clear_halt(FaDev &d);
 
+// ---- reverse mode (adjoint.hip) ----
+
+// Scalar adjoints carried backward through a recording, resident on the device.
+struct AdjScal {
+    double sbar;        // d loss / d speed entering the step being reversed
+    double Sbar, Tbar;  // d loss / d cumulative swing and traction totals
+    double kbar, gbar;  // d loss / d scaling_ratio, d gravity, summed over steps
+    double mubar[3];    // d loss / d centroid of the step being reversed
+    double pad[8];
+};
+
+constexpr int ADJ_NPART = 8;  // partial sums per block
+
+struct AdjDev {
+    double *pb[3] = {};  // d loss / d positions, one column per axis
+    double *ub[3] = {};  // d loss / d force u of the step being reversed
+    double *cacc = nullptr;  // per tree cell: sum over accepting bodies of -k*m_i*J(r)*ub_i
+    double *pscat = nullptr; // per tree position: pair terms for bodies of many-body leaves
+    double *wb = nullptr;    // d loss / d weight, per simple edge, summed over steps
+    uint32_t *elo = nullptr, *ehi = nullptr;  // simple edges (lo < hi)
+    uint32_t m = 0;
+    AdjScal *sc = nullptr;
+    double *part = nullptr;  // ADJ_NPART doubles per block
+    uint32_t nblk = 0;       // blocks of GR_BLOCK nodes (>= node chunks)
+    const GrGraph *g0 = nullptr;  // the graph's scalars before the step being reversed
+    const GrGraph *g1 = nullptr;  // and after it
+};
+
+// Copies n*dim planar positions (one column per axis, the session's element type) into
+// the state (planar and interleaved copies).
+void launch_adjoint_load(FaDev &d, const void *planar);
+// One reversed iteration. Needs the state of the iteration's start with its force u
+// evaluated (launch_forces with every part) and a.g0 / a.g1 set; updates a.pb, a.wb and the
+// scalar adjoints in a.sc.
+void launch_adjoint_step(FaDev &d, AdjDev &a);
+
 }  // namespace gr

@@ -16,6 +16,7 @@
 
 #include <type_traits>
 
+#include "fa2_common.h"
 #include "fa2_kernels.h"
 #include "sort_scan.h"
 
@@ -28,24 +29,6 @@ struct KeyFmt {
     // Levels held in one 64-bit key (D bits each); bit 63 marks a non-finite point.
     static constexpr int LEVELS = D == 2 ? 31 : 21;
     static constexpr int USED = D * LEVELS;
-};
-
-// The FP32 precisions keep each centre as an unevaluated pair hi + lo of floats (about 48
-// bits), so the offset of a nearby body from it is exact to float rounding even where
-// coordinates are large and the cell is small. FP64 keeps the reference's one double.
-template <typename T, int D>
-struct CentreLo {
-    T clo[D];
-};
-template <int D>
-struct CentreLo<double, D> {};
-
-template <typename T, int D>
-struct WalkCell : CentreLo<T, D> {
-    T c[D];
-    T mass;
-    T w2;
-    uint32_t skip, start, end;
 };
 
 size_t walkcell_bytes(int precision, uint32_t dim) {
@@ -111,10 +94,6 @@ __device__ __forceinline__ Box<D> root_box(const GrGraph &g) {
     return b;
 }
 
-__device__ __forceinline__ uint32_t graph_of(const FaDev &d, uint32_t node) {
-    return d.gidn ? d.gidn[node] : 0u;
-}
-
 // Key layout: [63: non-finite][graph id][L levels of D slot bits], level 1 highest.
 template <int D>
 __device__ __forceinline__ int key_slot(uint64_t key, int level, int L) {
@@ -137,45 +116,6 @@ __device__ __forceinline__ bool key_same(uint64_t a, uint64_t b, int level, int 
     return ((a ^ b) >> (D * (L - level))) == 0;
 }
 
-// rng.rs:27-84: MurmurHash3 fmix64, the three-round fold and the 53-bit nudge.
-__device__ __forceinline__ uint64_t d_fmix64(uint64_t k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ull;
-    k ^= k >> 33;
-    return k;
-}
-
-__device__ __noinline__ double d_jiggle(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t i,
-                                        uint32_t j) {
-    uint32_t lo = i <= j ? i : j, hi = i <= j ? j : i;
-    uint64_t a = d_fmix64((uint64_t)seed ^ ((uint64_t)tick << 32));
-    uint64_t b = d_fmix64(a ^ ((uint64_t)pass << 32) ^ (uint64_t)lo);
-    uint64_t w = d_fmix64(b ^ ((uint64_t)hi << 32) ^ a) >> 11;
-    if (w == (1ull << 52)) w = (1ull << 52) + 1;
-    double u = (double)w * (1.0 / (double)(1ull << 53));
-    return (u - 0.5) * 1e-6;
-}
-
-// Iteration kernels stop once the run has exited early or a capacity check failed; a
-// force evaluation (ignore_done) still runs on an exited state.
-__device__ __forceinline__ bool halted(const FaDev &d) {
-    const GrState *st = d.st;
-    return st->halt != 0 || (!d.ignore_done && st->done != 0);
-}
-
-// m_a*m_b/d2*k. FP64 divides exactly as the reference does; the FP32 precisions use the
-// hardware reciprocal (about 1 ulp), well inside their 1e-5 force tolerance.
-template <typename T>
-__device__ __forceinline__ T pair_force(T mm, T d2, T kr) {
-    if constexpr (std::is_same<T, double>::value) {
-        return mm / d2 * kr;
-    } else {
-        return mm * __builtin_amdgcn_rcpf(d2) * kr;
-    }
-}
-
 // Compensated (Kahan) accumulation, used by the FP32+Kahan precision.
 template <typename T, bool KAHAN>
 __device__ __forceinline__ void acc_add(T &s, T &c, T v) {
@@ -190,34 +130,6 @@ __device__ __forceinline__ void acc_add(T &s, T &c, T v) {
 }
 
 constexpr int NB = 10;  // bounds partial: lo[3], hi[3], sum[3], finite count
-
-// Block reduction of NV values (min, max or sum) in a fixed order; thread 0 gets the result.
-template <int NV, int OP>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double *lds) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        if (OP == 0) v[q] = gr_wave_sum(v[q]);
-        if (OP == 1) v[q] = gr_wave_min(v[q]);
-        if (OP == 2) v[q] = gr_wave_max(v[q]);
-        if (lane == 0) lds[wave * NV + q] = v[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double r = lds[q];
-            for (int w = 1; w < GR_BLOCK / GR_WAVE; w++) {
-                double o = lds[w * NV + q];
-                if (OP == 0) r = r + o;
-                if (OP == 1) r = fmin(r, o);
-                if (OP == 2) r = fmax(r, o);
-            }
-            v[q] = r;
-        }
-    }
-    __syncthreads();
-}
 
 // One point's contribution to the bounds/centroid partials (quadtree.rs:46-53, the finite
 // extent; state.rs:208-220, the centroid over every point).

@@ -58,6 +58,7 @@ uint64_t mix64(uint64_t k) {
 
 struct SimpleGraph {
     std::vector<uint32_t> lo, hi;
+    std::vector<size_t> raw;  // the raw edge each simple edge keeps (its first occurrence)
     std::vector<double> w;
     std::vector<uint32_t> row_off, nbr;
     std::vector<double> roww;
@@ -92,6 +93,7 @@ void build_simple(uint32_t n, const uint32_t *src, const uint32_t *dst, size_t m
         table[h] = key;
         g.lo.push_back(lo);
         g.hi.push_back(hi);
+        g.raw.push_back(e);
         if (weights) g.w.push_back(weights[e]);
     }
     size_t ms = g.lo.size();
@@ -123,14 +125,70 @@ struct gr_fa2_session {
     gr_fa2_params params{};
     uint32_t m_simple = 0;
     std::vector<uint32_t> lo, hi;
+    std::vector<size_t> simple_raw;
+    size_t m_raw = 0;
     std::vector<double> mass;
     std::vector<void *> allocs;
     int device = 0;
     bool tree_valid = false;
 
+    // The last gr_fa2_session_record(): state checkpoints at the start of each segment of
+    // `every` iterations, and the state it ended in.
+    struct Recording {
+        bool valid = false;
+        uint32_t every = 0, nseg = 0;
+        std::vector<uint32_t> seg_len;  // iterations applied in each segment
+        void *ck_pos = nullptr;         // nseg snapshots of the planar positions
+        GrGraph *ck_g = nullptr;
+        GrState *ck_st = nullptr;
+        void *fin_pos = nullptr;
+        GrGraph *fin_g = nullptr;
+        GrState *fin_st = nullptr;
+        std::vector<void *> allocs;
+    } rec;
+
     ~gr_fa2_session() {
+        drop_recording();
         for (void *p : allocs) (void)hipFree(p);
         if (d.stream) (void)hipStreamDestroy(d.stream);
+    }
+
+    void drop_recording() {
+        if (d.stream) (void)hipStreamSynchronize(d.stream);
+        for (void *p : rec.allocs) (void)hipFree(p);
+        rec = Recording();
+    }
+
+    template <typename P>
+    gr_status rec_alloc(P **out, size_t bytes) {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+        if (e != hipSuccess) return fail(GR_ERR_DEVICE, "hipMalloc", hipGetErrorString(e));
+        rec.allocs.push_back(p);
+        *out = (P *)p;
+        return GR_OK;
+    }
+
+    size_t pos_bytes() const { return (size_t)d.n * d.dim * esize(); }
+
+    // state -> snapshot (planar positions, graph scalars, session bookkeeping)
+    gr_status save_state(void *pos, GrGraph *g, GrState *st) {
+        const size_t col = (size_t)d.n * esize();
+        for (uint32_t a = 0; a < d.dim; a++)
+            HIP_TRY(hipMemcpyAsync((char *)pos + a * col, d.pos[a], col,
+                                   hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(g, d.gr, sizeof(GrGraph), hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(st, d.st, sizeof(GrState), hipMemcpyDeviceToDevice, d.stream));
+        return GR_OK;
+    }
+
+    gr_status load_state(const void *pos, const GrGraph *g, const GrState *st) {
+        gr::launch_adjoint_load(d, pos);
+        HIP_TRY(hipMemcpyAsync(d.gr, g, sizeof(GrGraph), hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(d.st, st, sizeof(GrState), hipMemcpyDeviceToDevice, d.stream));
+        gr::launch_bounds(d);
+        tree_valid = false;
+        return GR_OK;
     }
 
     template <typename P>
@@ -416,6 +474,8 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     s->m_simple = (uint32_t)g.lo.size();
     s->lo = g.lo;
     s->hi = g.hi;
+    s->simple_raw = g.raw;
+    s->m_raw = m;
     s->mass.resize(n);
     for (uint32_t v = 0; v < n; v++) s->mass[v] = (double)(g.row_off[v + 1] - g.row_off[v]) + 1.0;
     d.m2 = (uint32_t)g.nbr.size();
@@ -617,10 +677,10 @@ gr_fa2_session *gr_fa2_batch_create(uint32_t ngraphs, const uint32_t *node_count
 
 void gr_fa2_session_release(gr_fa2_session *s) { delete s; }
 
-gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_out) {
-    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
-    if (done_out) *done_out = 0;
-    if (s->d.n == 0 || iters == 0) return ok();
+// Runs iterations until `iters` more are applied or the run exits early, growing capacity
+// and re-running after a halted iteration. *applied: iterations applied.
+static gr_status run_iters(gr_fa2_session *s, uint32_t iters, uint32_t *applied) {
+    *applied = 0;
     GrState st;
     gr_status r = s->read_state(st);
     if (r) return r;
@@ -640,8 +700,210 @@ gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_
         }
         remaining = iters - (st.iters_done - start);
     }
+    *applied = st.iters_done - start;
+    return GR_OK;
+}
+
+gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (done_out) *done_out = 0;
+    if (s->d.n == 0 || iters == 0) return ok();
+    s->drop_recording();
+    uint32_t applied = 0;
+    gr_status r = run_iters(s, iters, &applied);
+    if (r) return r;
     s->tree_valid = s->d.repulsion == gr::REP_TREE;
-    if (done_out) *done_out = st.iters_done - start;
+    if (done_out) *done_out = applied;
+    return ok();
+}
+
+gr_status gr_fa2_session_record(gr_fa2_session *s, uint32_t iters, uint32_t checkpoint_every,
+                                uint32_t *done_out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (done_out) *done_out = 0;
+    if (checkpoint_every == 0)
+        return fail(GR_ERR_PARAM, "checkpoint_every", "at least one iteration per segment");
+    if (s->d.ngraphs != 1)
+        return fail(GR_ERR_UNSUPPORTED, "adjoint", "a session of one graph");
+    if (s->d.repulsion != gr::REP_TREE && s->d.repulsion != gr::REP_DENSE)
+        return fail(GR_ERR_UNSUPPORTED, "adjoint", "tree or dense repulsion");
+    s->drop_recording();
+    auto &rec = s->rec;
+    rec.every = checkpoint_every;
+    if (s->d.n == 0 || iters == 0) {
+        rec.valid = true;
+        return ok();
+    }
+    const uint32_t maxseg = (uint32_t)(((uint64_t)iters + checkpoint_every - 1) / checkpoint_every);
+    const size_t pb = s->pos_bytes();
+    gr_status r;
+    if ((r = s->rec_alloc(&rec.ck_pos, (size_t)maxseg * pb)) ||
+        (r = s->rec_alloc(&rec.ck_g, (size_t)maxseg * sizeof(GrGraph))) ||
+        (r = s->rec_alloc(&rec.ck_st, (size_t)maxseg * sizeof(GrState))) ||
+        (r = s->rec_alloc(&rec.fin_pos, pb)) || (r = s->rec_alloc(&rec.fin_g, sizeof(GrGraph))) ||
+        (r = s->rec_alloc(&rec.fin_st, sizeof(GrState)))) {
+        s->drop_recording();
+        return r;
+    }
+    uint32_t total = 0;
+    for (uint32_t seg = 0; seg < maxseg; seg++) {
+        if ((r = s->save_state((char *)rec.ck_pos + (size_t)seg * pb, rec.ck_g + seg,
+                               rec.ck_st + seg))) {
+            s->drop_recording();
+            return r;
+        }
+        const uint32_t want = iters - total < checkpoint_every ? iters - total : checkpoint_every;
+        uint32_t got = 0;
+        if ((r = run_iters(s, want, &got))) {
+            s->drop_recording();
+            return r;
+        }
+        rec.seg_len.push_back(got);
+        rec.nseg++;
+        total += got;
+        if (got < want) break;  // the run exited early
+    }
+    if ((r = s->save_state(rec.fin_pos, rec.fin_g, rec.fin_st))) {
+        s->drop_recording();
+        return r;
+    }
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
+    rec.valid = true;
+    s->tree_valid = s->d.repulsion == gr::REP_TREE;
+    if (done_out) *done_out = total;
+    return ok();
+}
+
+gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, double *weight_bar,
+                                 double *param_bar, double *start_bar) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    auto &rec = s->rec;
+    if (!rec.valid)
+        return fail(GR_ERR_UNSUPPORTED, "adjoint", "a recording (gr_fa2_session_record)");
+    gr::FaDev &d = s->d;
+    const uint32_t n = d.n, dim = d.dim;
+    if (n > 0 && !pos_bar) return fail(GR_ERR_PARAM, "pos_bar", "n*dim loss gradients");
+    if (weight_bar) std::memset(weight_bar, 0, s->m_raw * sizeof(double));
+    if (param_bar) param_bar[0] = param_bar[1] = 0.0;
+    if (n == 0 || rec.nseg == 0) {
+        if (start_bar && n) std::memcpy(start_bar, pos_bar, (size_t)n * dim * 8);
+        return ok();
+    }
+    // temporaries of this call
+    std::vector<void *> tmp;
+    struct Free {
+        std::vector<void *> &v;
+        hipStream_t st;
+        ~Free() {
+            (void)hipStreamSynchronize(st);
+            for (void *p : v) (void)hipFree(p);
+        }
+    } release{tmp, d.stream};
+    auto get = [&](void **out, size_t bytes) -> gr_status {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+        if (e != hipSuccess) return fail(GR_ERR_DEVICE, "hipMalloc", hipGetErrorString(e));
+        tmp.push_back(p);
+        *out = p;
+        return GR_OK;
+    };
+    gr::AdjDev a;
+    const uint32_t nbl = gr_blocks(n, GR_BLOCK);
+    const size_t every = rec.every;
+    const size_t pb = s->pos_bytes();
+    void *tape_pos = nullptr, *tape_g = nullptr, *tape_st = nullptr, *cacc = nullptr;
+    uint32_t cacc_cap = 0;
+    gr_status r;
+    for (uint32_t x = 0; x < dim; x++) {
+        if ((r = get((void **)&a.pb[x], (size_t)n * 8)) ||
+            (r = get((void **)&a.ub[x], (size_t)n * 8)))
+            return r;
+    }
+    a.m = s->m_simple;
+    if ((r = get((void **)&a.pscat, (size_t)n * dim * 8)) ||
+        (r = get((void **)&a.wb, (size_t)a.m * 8)) || (r = get((void **)&a.elo, (size_t)a.m * 4)) ||
+        (r = get((void **)&a.ehi, (size_t)a.m * 4)) ||
+        (r = get((void **)&a.sc, sizeof(gr::AdjScal))) ||
+        (r = get((void **)&a.part, (size_t)nbl * gr::ADJ_NPART * 8)) ||
+        (r = get(&tape_pos, every * pb)) || (r = get(&tape_g, (every + 1) * sizeof(GrGraph))) ||
+        (r = get(&tape_st, (every + 1) * sizeof(GrState))))
+        return r;
+    {
+        std::vector<double> col(n);
+        for (uint32_t x = 0; x < dim; x++) {
+            for (uint32_t i = 0; i < n; i++) col[i] = pos_bar[(size_t)i * dim + x];
+            HIP_TRY(hipMemcpy(a.pb[x], col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        }
+        if (a.m) {
+            HIP_TRY(hipMemcpy(a.elo, s->lo.data(), (size_t)a.m * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(a.ehi, s->hi.data(), (size_t)a.m * 4, hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipMemset(a.wb, 0, (size_t)a.m * 8));
+        HIP_TRY(hipMemset(a.sc, 0, sizeof(gr::AdjScal)));
+    }
+    GrGraph *tg = (GrGraph *)tape_g;
+    GrState *ts = (GrState *)tape_st;
+    GrState st;
+    for (uint32_t seg = rec.nseg; seg-- > 0;) {
+        const uint32_t L = rec.seg_len[seg];
+        if (L == 0) continue;
+        // recompute the segment from its checkpoint, keeping every iteration's start state
+        int tries = 0;
+        for (;;) {
+            if ((r = s->load_state((char *)rec.ck_pos + (size_t)seg * pb, rec.ck_g + seg,
+                                   rec.ck_st + seg)))
+                return r;
+            for (uint32_t j = 0; j < L; j++) {
+                if ((r = s->save_state((char *)tape_pos + (size_t)j * pb, tg + j, ts + j)))
+                    return r;
+                gr::launch_iteration(d);
+            }
+            HIP_TRY(hipMemcpyAsync(tg + L, d.gr, sizeof(GrGraph), hipMemcpyDeviceToDevice,
+                                   d.stream));
+            HIP_TRY(hipGetLastError());
+            if ((r = s->read_state(st))) return r;
+            if (!st.halt) break;
+            if (++tries > 8) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
+            if ((r = s->recover(st))) return r;
+        }
+        if (cacc_cap < d.cap) {
+            if ((r = get(&cacc, (size_t)d.cap * dim * 8))) return r;
+            cacc_cap = d.cap;
+            a.cacc = (double *)cacc;
+        }
+        for (uint32_t j = L; j-- > 0;) {
+            if ((r = s->load_state((char *)tape_pos + (size_t)j * pb, tg + j, ts + j))) return r;
+            gr::launch_forces(d, gr::PART_ATTRACTION | gr::PART_REPULSION | gr::PART_GRAVITY);
+            a.g0 = tg + j;
+            a.g1 = tg + j + 1;
+            gr::launch_adjoint_step(d, a);
+        }
+        HIP_TRY(hipGetLastError());
+        if ((r = s->read_state(st))) return r;
+        if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "halted while reversing");
+    }
+    // outputs
+    if (start_bar) {
+        std::vector<double> col(n);
+        for (uint32_t x = 0; x < dim; x++) {
+            HIP_TRY(hipMemcpy(col.data(), a.pb[x], (size_t)n * 8, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; i++) start_bar[(size_t)i * dim + x] = col[i];
+        }
+    }
+    if (weight_bar && a.m) {
+        std::vector<double> wb(a.m);
+        HIP_TRY(hipMemcpy(wb.data(), a.wb, (size_t)a.m * 8, hipMemcpyDeviceToHost));
+        for (uint32_t e = 0; e < a.m; e++) weight_bar[s->simple_raw[e]] = wb[e];
+    }
+    if (param_bar) {
+        gr::AdjScal sc;
+        HIP_TRY(hipMemcpy(&sc, a.sc, sizeof(sc), hipMemcpyDeviceToHost));
+        param_bar[0] = sc.kbar;
+        param_bar[1] = sc.gbar;
+    }
+    // leave the session as the recording left it
+    if ((r = s->load_state(rec.fin_pos, rec.fin_g, rec.fin_st))) return r;
+    HIP_TRY(hipStreamSynchronize(d.stream));
     return ok();
 }
 
@@ -672,6 +934,7 @@ gr_status gr_fa2_session_set_positions(gr_fa2_session *s, const double *in) {
     if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
     HIP_TRY(hipStreamSynchronize(s->d.stream));
     if (s->d.n == 0) return ok();
+    s->drop_recording();
     gr_status r = s->upload_positions(in);
     return r ? r : ok();
 }
@@ -787,6 +1050,7 @@ gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t rep
         return fail(GR_ERR_UNSUPPORTED, "stage", "walk/tree timing needs a tree session");
     if (pm && s->d.repulsion != gr::REP_PM)
         return fail(GR_ERR_UNSUPPORTED, "stage", "pm timing needs a PM session");
+    if (it) s->drop_recording();  // timed iterations move the state
     if (walk && !s->tree_valid) {
         double *tmp = (double *)std::malloc((size_t)s->d.n * s->d.dim * 8 + 8);
         gr_status r = gr_fa2_session_forces(s, 7, tmp);

@@ -290,6 +290,7 @@ class Session:
         self.params = params or Fa2Params()
         self.ext = ext or Fa2Ext()
         self.n, self.dim = int(topology.n), int(self.params.dim)
+        self.m_raw = int(topology.m)
         xc = self.ext._c(self.n, topology.m, self.dim)
         cp = self.params._c()
         src = topology.source if topology.m else np.zeros(1, np.uint32)
@@ -329,6 +330,7 @@ class Session:
         src = np.concatenate([t.source for t in topologies] + [np.zeros(1, np.uint32)])
         dst = np.concatenate([t.target for t in topologies] + [np.zeros(1, np.uint32)])
         m = int(edges.sum())
+        self.m_raw = m
         xc = self.ext._c(self.n, m, self.dim)
         cp = self.params._c()
         self._h = A.lib().gr_fa2_batch_create(len(topologies), counts.ctypes.data,
@@ -362,6 +364,30 @@ class Session:
         p = np.ascontiguousarray(p, np.float64).reshape(-1)
         assert p.size == self.n * self.dim
         _check(A.lib().gr_fa2_session_set_positions(self._h, p.ctypes.data))
+
+    def record(self, iters: int, checkpoint_every: int = 20) -> int:
+        """Runs up to `iters` iterations like step(), keeping a checkpoint every
+        `checkpoint_every` iterations for adjoint(). Returns the iterations applied."""
+        done = C.c_uint32(0)
+        _check(A.lib().gr_fa2_session_record(self._h, int(iters), int(checkpoint_every),
+                                             C.byref(done)))
+        return int(done.value)
+
+    def adjoint(self, pos_bar) -> dict:
+        """Reverse sweep of the last record(): from d loss / d final positions (n, dim) to
+        d loss / d raw edge weight, d scaling_ratio, d gravity and d start positions."""
+        pb = np.ascontiguousarray(pos_bar, np.float64).reshape(-1)
+        if pb.size != self.n * self.dim:
+            raise ValueError("pos_bar must hold n*dim values")
+        wb = np.zeros(max(self.m_raw, 1))
+        par = np.zeros(2)
+        sb = np.zeros(max(self.n * self.dim, 1))
+        _check(A.lib().gr_fa2_session_adjoint(self._h, pb.ctypes.data if pb.size else None,
+                                              wb.ctypes.data, par.ctypes.data,
+                                              sb.ctypes.data))
+        return {"weights": wb[: self.m_raw], "scaling_ratio": float(par[0]),
+                "gravity": float(par[1]),
+                "start": sb[: self.n * self.dim].reshape(self.n, self.dim)}
 
     def forces(self, parts: int = 7) -> np.ndarray:
         out = np.zeros(max(self.n, 1) * self.dim)

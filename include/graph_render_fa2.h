@@ -152,6 +152,25 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
                               double *bounds, uint32_t *skip, uint32_t *start, uint32_t *end,
                               double *bodies, uint32_t *order);
 
+/* ---- reverse mode: gradients through unrolled iterations (no reference counterpart; the
+ * reference has no derivatives) ---- */
+
+/* Runs up to `iters` iterations as gr_fa2_session_step does, keeping a device checkpoint of
+ * the state at the start of every `checkpoint_every` iterations for gr_fa2_session_adjoint.
+ * One graph, tree or dense repulsion. *done_out (may be NULL): iterations applied. */
+gr_status gr_fa2_session_record(gr_fa2_session *s, uint32_t iters, uint32_t checkpoint_every,
+                                uint32_t *done_out);
+
+/* Reverse sweep of the last recording: given d loss / d final positions (n*dim f64,
+ * row-major), writes d loss / d edge weight per raw edge (a repeated or self-loop raw edge,
+ * which the simple graph drops, gets 0; unweighted sessions are differentiated at weight 1),
+ * d loss / d (scaling_ratio, gravity), and d loss / d start positions (n*dim). Any output
+ * may be NULL. Each segment is recomputed from its checkpoint and reversed iteration by
+ * iteration; the tree's shape and the accept/open decisions are held as the forward made
+ * them. The session is left in the state the recording ended in. */
+gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, double *weight_bar,
+                                 double *param_bar2, double *start_bar);
+
 /* Times `reps` launches of one stage on the session's stream with HIP events: "walk",
  * "tree", "iteration", "pm". Writes the mean milliseconds per launch. */
 gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,

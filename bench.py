@@ -221,10 +221,11 @@ def bench_adjoint(args):
     """Reverse mode through unrolled FA2-BH: 10K-node SBM (16 equal blocks, p_in 0.01,
     p_out 0.0005, seed 3), iterations recorded with a checkpoint every 20, loss = Kruskal
     stress-1 against BFS hops from 256 evenly spaced sources. Times forward and reverse over
-    `steps` iterations in FP32 and FP64. Then, at horizons of 10 .. `steps` iterations,
-    compares d loss / d weight on 64 random edges, d scaling ratio and d gravity with
-    central differences of the FP64 forward at h 1e-3 and 1e-6: the two step sizes agreeing
-    with each other is what says the finite difference measures a derivative at all."""
+    `steps` iterations in FP32 and FP64. Then, at horizons of 10 .. `steps` iterations (and
+    with dense repulsion, whose map has no accept/open switches, at 10 .. 50), compares
+    d loss / d weight on 64 random edges, d scaling ratio and d gravity with central
+    differences of the FP64 forward at h 1e-3 and 1e-6: the two step sizes agreeing with
+    each other is what says the finite difference measures a derivative at all."""
     import torch
     import graph_render_amd as G
     from graph_render_amd import autograd as AG
@@ -237,9 +238,9 @@ def bench_adjoint(args):
     hops = AG.bfs_hops(n, s, t, srcs)
     w0 = np.ones(s.size)
 
-    def reverse(precision, iters):
+    def reverse(precision, iters, repulsion="tree"):
         spec = AG.UnrolledSpec(topology=topo, iters=iters, checkpoint_every=every,
-                               precision=precision, repulsion="tree")
+                               precision=precision, repulsion=repulsion)
         t0 = time.perf_counter()
         sess, pos = AG.run_forward(spec, w0, 2.0, 1.0, record=True)
         t_fwd = time.perf_counter() - t0
@@ -265,12 +266,13 @@ def bench_adjoint(args):
     def vec_err(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-    horizons = [h for h in (10, 25, 50, 100, 200) if h < args.steps] + [args.steps]
+    checks = [("tree", h) for h in (10, 25, 50) if h < args.steps] + [("tree", args.steps)]
+    checks += [("dense", h) for h in (10, 25, 50) if h <= args.steps]
     rows = []
     t_fd = time.perf_counter()
-    for H in horizons:
+    for rep, H in checks:
         def loss64(w, k=2.0, g=1.0):
-            spec = AG.UnrolledSpec(topology=topo, iters=H, precision="fp64", repulsion="tree")
+            spec = AG.UnrolledSpec(topology=topo, iters=H, precision="fp64", repulsion=rep)
             _, pos = AG.run_forward(spec, w, k, g)
             return AG.stress1_np(pos, srcs, hops)
 
@@ -285,24 +287,28 @@ def bench_adjoint(args):
             fk = (loss64(w0, k=2.0 + h) - loss64(w0, k=2.0 - h)) / (2 * h)
             fg = (loss64(w0, g=1.0 + h) - loss64(w0, g=1.0 - h)) / (2 * h)
             fds[h] = (np.array(fw), fk, fg)
-        ga64 = g64 if H == args.steps else reverse("fp64", H)[1]
-        ga32 = g32 if H == args.steps else reverse("fp32", H)[1]
-        fw3, fk3, fg3 = fds[1e-3]
-        fw6 = fds[1e-6][0]
-        row = {"iterations": H,
+        if rep == "tree" and H == args.steps:
+            ga64, ga32 = g64, g32
+        else:
+            ga64 = reverse("fp64", H, rep)[1]
+            ga32 = reverse("fp32", H, rep)[1]
+        row = {"repulsion": rep, "iterations": H,
                "grad_norm_fp64_adjoint": float(np.linalg.norm(ga64["weights"][pick])),
-               "fd_h1e-3_vs_fd_h1e-6": vec_err(fw3, fw6)}
+               "fd_h1e-3_vs_fd_h1e-6": vec_err(fds[1e-3][0], fds[1e-6][0])}
         for name, ga in (("fp64", ga64), ("fp32", ga32)):
-            row[f"adjoint_{name}_vs_fd_h1e-3"] = {
-                "weights_vector_rel_err": vec_err(ga["weights"][pick], fw3),
-                "weights_max_abs_err_over_max_fd":
-                    float(np.abs(ga["weights"][pick] - fw3).max() / np.abs(fw3).max()),
-                "scaling_ratio_rel_err": float(abs(ga["scaling_ratio"] - fk3) / abs(fk3)),
-                "gravity_rel_err": float(abs(ga["gravity"] - fg3) / abs(fg3))}
+            for h in (1e-3, 1e-6):
+                fw, fk, fg = fds[h]
+                row[f"adjoint_{name}_vs_fd_h{h:.0e}"] = {
+                    "weights_vector_rel_err": vec_err(ga["weights"][pick], fw),
+                    "weights_max_abs_err_over_max_fd":
+                        float(np.abs(ga["weights"][pick] - fw).max() / np.abs(fw).max()),
+                    "scaling_ratio_rel_err": float(abs(ga["scaling_ratio"] - fk) / abs(fk)),
+                    "gravity_rel_err": float(abs(ga["gravity"] - fg) / abs(fg))}
         rows.append(row)
-        print(f"horizon {H}: fd(1e-3) vs fd(1e-6) {row['fd_h1e-3_vs_fd_h1e-6']:.2e}, "
+        print(f"{rep} horizon {H}: fd(1e-3) vs fd(1e-6) {row['fd_h1e-3_vs_fd_h1e-6']:.2e}; "
               f"fp64 adjoint vs fd(1e-3) "
-              f"{row['adjoint_fp64_vs_fd_h1e-3']['weights_vector_rel_err']:.2e}",
+              f"{row['adjoint_fp64_vs_fd_h1e-03']['weights_vector_rel_err']:.2e}, vs fd(1e-6) "
+              f"{row['adjoint_fp64_vs_fd_h1e-06']['weights_vector_rel_err']:.2e}",
               file=sys.stderr, flush=True)
     fd_s = time.perf_counter() - t_fd
 
@@ -320,7 +326,7 @@ def bench_adjoint(args):
         "fp32": {"forward_s": round(f32, 4), "reverse_s": round(b32, 4)},
         "fp64": {"forward_s": round(f64, 4), "reverse_s": round(b64, 4)},
         "fd": {"edges": 64, "step_sizes": [1e-3, 1e-6], "seconds": round(fd_s, 2)},
-        "horizons": rows,
+        "checks": rows,
     }
     print(json.dumps(line), flush=True)
 

@@ -71,6 +71,10 @@ SIGNATURES = {
                                          C.c_void_p]),
     "gr_fa2_session_time": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32,
                                       C.POINTER(C.c_double)]),
+    "gr_snapshot_encode": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_size_t, C.POINTER(C.c_size_t)]),
     "gr_last_error": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),
     "gr_version": (C.c_char_p, []),
 }

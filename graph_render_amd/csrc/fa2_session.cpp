@@ -280,6 +280,13 @@ struct gr_fa2_session {
 
 extern "C" {
 
+// Sets the calling thread's last error from another translation unit of this library.
+__attribute__((visibility("hidden"))) gr_status gr_internal_fail(gr_status code,
+                                                                 const char *name,
+                                                                 const char *rule) {
+    return code == GR_OK ? ok() : fail(code, name, rule);
+}
+
 void gr_fa2_params_default(gr_fa2_params *out) {
     out->max_iter = 100;
     out->jitter_tolerance = 1.0;

@@ -176,6 +176,21 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
 gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,
                               double *ms_out);
 
+/* The graph-motor snapshot's binary face (GMSN; the reference's docs/contract/binary-layout.md,
+ * written by graph-contract's Snapshot::to_bytes, binary.rs:170-197, from the geometry
+ * layout::snapshot builds, layout/mod.rs:205-235): Point nodes, Line edges, the node and edge
+ * id tables, endpoints as node positions, f32 x, y (and z, which makes a 0.4 3D snapshot;
+ * NULL for 2D, labelled 0.3) and `notes` (code, index) pairs in ascending order. Writes the
+ * bytes to `out` when `cap` is large enough; *len_out always receives the size. Refuses what
+ * Snapshot::new refuses: duplicate ids, endpoints outside the node table, non-finite
+ * coordinates, unknown or unsorted notes. */
+gr_status gr_snapshot_encode(uint32_t n, const char *const *node_ids, uint32_t m,
+                             const char *const *edge_ids, const uint32_t *source,
+                             const uint32_t *target, const float *x, const float *y,
+                             const float *z, uint32_t notes, const uint32_t *note_code,
+                             const uint32_t *note_index, uint8_t *out, size_t cap,
+                             size_t *len_out);
+
 /* Last failure of the calling thread: code, parameter/column name, rule text. */
 gr_status gr_last_error(const char **name, const char **rule);
 

@@ -22,3 +22,27 @@ def errors(approx, exact):
     d2 = ((approx - exact) ** 2).sum(1)
     e2 = (exact ** 2).sum(1)
     return float(np.sqrt((d2 / e2).mean())), float(np.sqrt(d2.sum() / e2.sum()))
+
+
+def gate_model(oracle, seed):
+    """The reference gate model seeded_model(seed, 2 + seed % 600, 8) as graph-cli's
+    snapshot builds it (stage/topology.rs:19-35, synthetic.rs:92-170): node ids
+    bench:db-{i % 8}:{i}, edge ids bench-e-{k}, endpoints from the oracle's restatement of
+    synthetic_edges (the seed's remix redraws sources and kinds, not endpoints)."""
+    n = 2 + seed % 600
+    s, t = oracle.gate_edges(n)
+    nodes = [f"bench:db-{i % 8}:{i}" for i in range(n)]
+    edges = [f"bench-e-{k}" for k in range(len(s))]
+    return n, nodes, edges, s, t
+
+
+def fa2_snapshot_pins():
+    """seed -> sha256 of layout.forceatlas2's snapshot (tests/golden/fa2-snapshot-sha256.txt)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "fa2-snapshot-sha256.txt")
+    pins = {}
+    for line in open(path):
+        if line.startswith("layout.forceatlas2 "):
+            _, seed, digest = line.split()
+            pins[int(seed)] = digest
+    return pins

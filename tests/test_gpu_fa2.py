@@ -276,3 +276,16 @@ def test_params_are_refused_not_clamped():
         G.run_params("layout.nope", topo, b"")
     geo = G.run_params("layout.forceatlas2.barnes_hut", topo, b"")
     assert geo.x.size == 5
+
+
+def test_layout_forceatlas2_matches_the_reference_snapshot_pins(oracle):
+    """The registry entry layout.forceatlas2 on the device, default parameters (100
+    iterations), encoded as the reference's snapshot: the sha256 the reference itself pins
+    for seeds 0..7 (harness/golden-2d-before.txt)."""
+    from graph_render_amd import snapshot as S
+    from tests.refgraphs import fa2_snapshot_pins, gate_model
+    for seed, want in fa2_snapshot_pins().items():
+        n, nodes, edges, s, t = gate_model(oracle, seed)
+        print(f"snapshot pin seed={seed} n={n}", flush=True)
+        geo = G.ForceAtlas2.run(G.Topology(n, s, t), G.Fa2Params())
+        assert S.sha256(S.of_geometry(nodes, edges, s, t, geo)) == want, seed

@@ -377,12 +377,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_walk(const FaDev d, const AdjD
     const uint32_t ncells = d.st->n_cells;
     uint32_t next = (live && lo < hi) ? lo : 0xFFFFFFFFu;
     uint32_t k = next;
+    // the scan ends at the last cell of the wave's last graph: later trees are others'
+    uint32_t end = live ? hi : 0u;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const uint32_t o = __shfl_xor(k, m, GR_WAVE);
         k = o < k ? o : k;
+        const uint32_t e = __shfl_xor(end, m, GR_WAVE);
+        end = e > end ? e : end;
     }
-    while (k < ncells) {
+    end = gr_uniform(end < ncells ? end : ncells);
+    while (k < end) {
         k = gr_uniform(k);
         const WalkCell<T, D> cell = wc[k];
         const bool act = next == k;

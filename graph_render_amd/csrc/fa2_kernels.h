@@ -13,6 +13,9 @@ namespace gr {
 enum Repulsion : int { REP_TREE = 0, REP_DENSE = 1, REP_PM = 2, REP_NONE = 3 };
 enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_KAHAN = 2 };
 
+// Rows with more CSR entries than this are summed by k_long_rows (one workgroup per row).
+constexpr uint32_t LONG_ROW = 128;
+
 // Force parts of one evaluation (gr_fa2_session_forces): the reference's iterate() order.
 enum Parts : uint32_t { PART_ATTRACTION = 1, PART_REPULSION = 2, PART_GRAVITY = 4 };
 
@@ -34,6 +37,11 @@ struct FaDev {
     void *rep[3] = {};
     uint32_t *row_off = nullptr, *nbr = nullptr;
     void *w = nullptr;
+    // rows longer than LONG_ROW entries (ascending node ids): their attraction is summed by
+    // k_long_rows with parallel gathers into LDS, one workgroup per row
+    uint32_t nlong = 0;
+    uint32_t *long_rows = nullptr;
+    void *long_acc = nullptr;  // dim sums per long row
     GrState *st = nullptr;
 
     // batch layout: graph b owns nodes [gnode[b], gnode[b+1]) and node chunks

@@ -1070,12 +1070,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     for (int a = 0; a < D; a++) f[a] = fc[a] = (T)0;
     uint32_t next = (live && lo < hi) ? lo : 0xFFFFFFFFu;
     uint32_t k = next;
+    // the scan ends at the last cell of the wave's last graph: later trees are others'
+    uint32_t end = live ? hi : 0u;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const uint32_t o = __shfl_xor(k, m, GR_WAVE);
         k = o < k ? o : k;
+        const uint32_t e = __shfl_xor(end, m, GR_WAVE);
+        end = e > end ? e : end;
     }
-    while (k < ncells) {
+    end = gr_uniform(end < ncells ? end : ncells);
+    while (k < end) {
         k = gr_uniform(k);
         const WalkCell<T, D> cell = wc[k];
         const bool act = next == k;
@@ -1279,6 +1284,53 @@ __global__ __launch_bounds__(GR_BLOCK) void k_dense(const FaDev d, uint32_t with
     }
 }
 
+// state.rs:152-162 for a row longer than LONG_ROW: the workgroup gathers the row's
+// neighbour positions (and weights) into LDS 256 at a time, and one lane sums them in
+// ascending entry order, the same sequence k_finish uses for a short row, so the sum is
+// bit-identical; only the gathers run in parallel instead of one dependent load per entry.
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK) void k_long_rows(const FaDev d) {
+    if (halted(d)) return;
+    __shared__ T s_p[D][GR_BLOCK];
+    __shared__ T s_w[GR_BLOCK];
+    const uint32_t v = d.long_rows[blockIdx.x];
+    const uint32_t e0 = d.row_off[v], e1 = d.row_off[v + 1];
+    const T *pi = (const T *)d.pi;
+    const T *wgt = (const T *)d.w;
+    T pv[D], uv[D], cc[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        pv[a] = pi[(size_t)v * D + a];
+        uv[a] = cc[a] = (T)0;
+    }
+    for (uint32_t base = e0; base < e1; base += GR_BLOCK) {
+        const uint32_t e = base + threadIdx.x;
+        if (e < e1) {
+            const uint32_t j = d.nbr[e];
+#pragma unroll
+            for (int a = 0; a < D; a++) s_p[a][threadIdx.x] = pi[(size_t)j * D + a];
+            if (wgt) s_w[threadIdx.x] = wgt[e];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t cnt = e1 - base < GR_BLOCK ? e1 - base : GR_BLOCK;
+            for (uint32_t t = 0; t < cnt; t++) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    T dd = s_p[a][t] - pv[a];
+                    if (wgt) dd = dd * s_w[t];
+                    acc_add<T, KAHAN>(uv[a], cc[a], dd);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < D; a++) ((T *)d.long_acc)[(size_t)blockIdx.x * D + a] = uv[a];
+    }
+}
+
 constexpr uint32_t FIN_STAGE = 4096;  // CSR entries staged in LDS per round
 
 // Completes u (attraction + repulsion, then gravity, state.rs:130-148 order) and the
@@ -1341,6 +1393,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         const bool attr = mode == 0 && (parts & PART_ATTRACTION);
         uint32_t e = live ? d.row_off[v] : 0;
         const uint32_t e1 = live ? d.row_off[v + 1] : 0;
+        const bool longrow = e1 - e > LONG_ROW;  // summed by k_long_rows
+        if (longrow) e = e1;
         if (attr) {
             // rounds of FIN_STAGE entries covering [row_off[base], row_off[rend])
             const uint32_t g0 = d.row_off[base], g1 = d.row_off[rend];
@@ -1368,6 +1422,18 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
             }
         }
         if (!live) continue;
+        if (attr && longrow) {
+            uint32_t lo = 0, hi = d.nlong;  // its slot among the ascending long rows
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (d.long_rows[mid] < v)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+#pragma unroll
+            for (int a = 0; a < D; a++) uv[a] = ((const T *)d.long_acc)[(size_t)lo * D + a];
+        }
         if (mode == 0) {
             if (parts & PART_REPULSION) {
 #pragma unroll
@@ -1625,6 +1691,8 @@ static void forces_impl(FaDev &d, uint32_t parts) {
         head_impl<D>(d);
         parts &= ~(uint32_t)PART_REPULSION;
     }
+    if (mode == 0 && (parts & PART_ATTRACTION) && d.nlong > 0)
+        hipLaunchKernelGGL((k_long_rows<T, D, KAHAN>), dim3(d.nlong), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d, mode,
                        parts);
     hipLaunchKernelGGL(k_stats, dim3(d.nchunks), dim3(GR_BLOCK), 0, s, d);

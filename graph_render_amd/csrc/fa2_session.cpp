@@ -131,6 +131,40 @@ struct gr_fa2_session {
     std::vector<void *> allocs;
     int device = 0;
     bool tree_valid = false;
+    // One iteration captured as a HIP graph and replayed; rebuilt after anything that
+    // changes the buffers its launches were captured with (a capacity recovery).
+    bool use_graph = true;
+    hipGraph_t it_graph = nullptr;
+    hipGraphExec_t it_exec = nullptr;
+
+    void drop_graph() {
+        if (it_exec) (void)hipGraphExecDestroy(it_exec);
+        if (it_graph) (void)hipGraphDestroy(it_graph);
+        it_exec = nullptr;
+        it_graph = nullptr;
+    }
+
+    // Enqueues one iteration: the captured graph when there is one, else direct launches.
+    void iteration() {
+        if (use_graph && !it_exec) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                gr::launch_iteration(d);
+                if (hipStreamEndCapture(d.stream, &g) == hipSuccess && g) {
+                    if (hipGraphInstantiate(&it_exec, g, nullptr, nullptr, 0) == hipSuccess)
+                        it_graph = g;
+                    else
+                        (void)hipGraphDestroy(g);
+                }
+            }
+            (void)hipGetLastError();
+            if (!it_exec) use_graph = false;  // capture unavailable: launch directly
+        }
+        if (it_exec)
+            (void)hipGraphLaunch(it_exec, d.stream);
+        else
+            gr::launch_iteration(d);
+    }
 
     // The last gr_fa2_session_record(): state checkpoints at the start of each segment of
     // `every` iterations, and the state it ended in.
@@ -149,6 +183,7 @@ struct gr_fa2_session {
 
     ~gr_fa2_session() {
         drop_recording();
+        drop_graph();
         for (void *p : allocs) (void)hipFree(p);
         if (d.stream) (void)hipStreamDestroy(d.stream);
     }
@@ -273,6 +308,7 @@ struct gr_fa2_session {
             if (s) return s;
         }
         if (!grew) return fail(GR_ERR_DEVICE, "halt", "iteration halted without a capacity cause");
+        drop_graph();  // its launches hold the old buffers
         gr::launch_clear_halt(d);
         return GR_OK;
     }
@@ -468,6 +504,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     d.jitter_tolerance = p.jitter_tolerance;
     d.seed = p.seed;
     d.export_bounds = (x.flags & GR_EXT_EXPORT_TREE) ? 1 : 0;
+    s->use_graph = (x.flags & GR_EXT_DIRECT_LAUNCH) == 0;
     d.pm_side = x.pm_grid;
     (void)hipGetDevice(&s->device);
     if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) {
@@ -487,6 +524,10 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     for (uint32_t v = 0; v < n; v++) s->mass[v] = (double)(g.row_off[v + 1] - g.row_off[v]) + 1.0;
     d.m2 = (uint32_t)g.nbr.size();
     d.weighted = x.edge_weights ? 1 : 0;
+    std::vector<uint32_t> long_rows;
+    for (uint32_t v = 0; v < n; v++)
+        if (g.row_off[v + 1] - g.row_off[v] > gr::LONG_ROW) long_rows.push_back(v);
+    d.nlong = (uint32_t)long_rows.size();
 
     const size_t es = s->esize();
     const size_t nn = n ? n : 1;
@@ -508,6 +549,8 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     TRY_ALLOC(d.row_off, (nn + 1) * 4);
     TRY_ALLOC(d.nbr, (g.nbr.size() + 1) * 4);
     if (d.weighted) TRY_ALLOC(d.w, (g.nbr.size() + 1) * es);
+    TRY_ALLOC(d.long_rows, ((size_t)d.nlong + 1) * 4);
+    TRY_ALLOC(d.long_acc, ((size_t)d.nlong + 1) * d.dim * es);
     TRY_ALLOC(d.st, sizeof(GrState));
     // batch tables: node offsets, graph-aligned node chunks, node -> graph
     d.ngraphs = ngraphs;
@@ -609,7 +652,8 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
     bool good = upload(d.row_off, g.row_off.data(), ((size_t)n + 1) * 4) &&
-                upload(d.nbr, g.nbr.data(), g.nbr.size() * 4);
+                upload(d.nbr, g.nbr.data(), g.nbr.size() * 4) &&
+                upload(d.long_rows, long_rows.data(), long_rows.size() * 4);
     if (d.precision == gr::PREC_FP64) {
         good = good && upload(d.mass, s->mass.data(), (size_t)n * 8);
         if (d.weighted) good = good && upload(d.w, g.roww.data(), g.roww.size() * 8);
@@ -697,7 +741,7 @@ static gr_status run_iters(gr_fa2_session *s, uint32_t iters, uint32_t *applied)
     int recoveries = 0;
     while (remaining > 0 && !st.done) {
         uint32_t k = remaining < batch ? remaining : batch;
-        for (uint32_t q = 0; q < k; q++) gr::launch_iteration(s->d);
+        for (uint32_t q = 0; q < k; q++) s->iteration();
         HIP_TRY(hipGetLastError());
         if ((r = s->read_state(st))) return r;
         if (st.halt) {
@@ -863,7 +907,7 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
             for (uint32_t j = 0; j < L; j++) {
                 if ((r = s->save_state((char *)tape_pos + (size_t)j * pb, tg + j, ts + j)))
                     return r;
-                gr::launch_iteration(d);
+                s->iteration();
             }
             HIP_TRY(hipMemcpyAsync(tg + L, d.gr, sizeof(GrGraph), hipMemcpyDeviceToDevice,
                                    d.stream));
@@ -1071,7 +1115,7 @@ gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t rep
     for (uint32_t q = 0; q < reps; q++) {
         if (walk) gr::launch_walk(s->d);
         if (tree) gr::launch_tree(s->d);
-        if (it) gr::launch_iteration(s->d);
+        if (it) s->iteration();
         if (pm) gr::launch_forces(s->d, 2);
     }
     HIP_TRY(hipEventRecord(e1, s->d.stream));

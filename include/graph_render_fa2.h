@@ -66,7 +66,9 @@ typedef enum gr_repulsion {
     GR_REP_PM = 3        /* particle mesh: CIC deposit, FFT convolution, CIC gradient read */
 } gr_repulsion;
 
-#define GR_EXT_EXPORT_TREE 1u /* keep cell bounds for gr_fa2_session_tree() */
+#define GR_EXT_EXPORT_TREE 1u   /* keep cell bounds for gr_fa2_session_tree() */
+#define GR_EXT_DIRECT_LAUNCH 2u /* launch each iteration's kernels one by one instead of
+                                   replaying the iteration as a captured HIP graph */
 
 /* Extensions the reference does not have. A NULL gr_fa2_ext pointer means exactly the
  * reference's semantics: theta 0.8 (barnes_hut.rs:24), FP64, unit edge weights, Mulberry32

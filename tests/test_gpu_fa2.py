@@ -117,6 +117,35 @@ def test_weighted_attraction_is_bit_exact(oracle):
         assert np.array_equal(sess.forces(3), ref.forces(3))
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_kahan"])
+def test_hub_rows_attraction(oracle, weighted, precision):
+    # hubs of degree 2999, 700 and 129..131 take the long-row path (more than 128 entries),
+    # the rest the per-lane path; FP64 must stay bit-exact, both precisions one sequence
+    n = 3000
+    rng = np.random.default_rng(21)
+    hub = [(0, j) for j in range(1, n)]
+    mid = [(5, j) for j in rng.choice(np.arange(6, n), 700, replace=False)]
+    edge = [(7 + k, j) for k in range(3) for j in range(100, 100 + 129 + k)]
+    rest = [tuple(e) for e in rng.integers(0, n, (2 * n, 2))]
+    s, t = np.array(hub + mid + edge + rest, np.uint32).T
+    w = rng.lognormal(0.0, 0.5, s.size) if weighted else None
+    ref = oracle_state(oracle, n, s, t, weights=w)
+    ref.run(2)
+    print(f"hub rows weighted={weighted} precision={precision}", flush=True)
+    with session(n, s, t, start=ref.positions(), weights=w, precision=precision) as sess:
+        got = sess.forces(1)
+    want = ref.forces(1)
+    if precision == "fp64":
+        assert np.array_equal(got, want)
+    else:
+        p32 = ref.positions().astype(np.float32).astype(np.float64)
+        with session(n, s, t, start=p32, weights=w) as s64:
+            want = s64.forces(1)
+        _, field = errors(got, want)
+        assert field <= 1e-5
+
+
 def test_theta_zero_equals_dense(oracle):
     """barnes_hut/tests.rs:61-70 on the device."""
     n = 400
@@ -289,3 +318,26 @@ def test_layout_forceatlas2_matches_the_reference_snapshot_pins(oracle):
         print(f"snapshot pin seed={seed} n={n}", flush=True)
         geo = G.ForceAtlas2.run(G.Topology(n, s, t), G.Fa2Params())
         assert S.sha256(S.of_geometry(nodes, edges, s, t, geo)) == want, seed
+
+
+@pytest.mark.parametrize("repulsion", ["tree", "dense", "pm"])
+def test_graph_replay_equals_direct_launches(repulsion):
+    # iterations replayed from the captured HIP graph, with host reads, forces and timing
+    # in between, give the same bits as the kernels launched one by one
+    n = 3000 if repulsion != "dense" else 600
+    s, t = bh_graph(n)
+    p = G.Fa2Params(max_iter=200)
+    out = []
+    for direct in (False, True):
+        x = G.Fa2Ext(repulsion=repulsion, direct_launch=direct,
+                     precision="fp32" if repulsion == "pm" else "fp64",
+                     pm_grid=256 if repulsion == "pm" else 0)
+        with G.Session(G.Topology(n, s, t), p, x) as sess:
+            trace = []
+            for chunk in (1, 7, 33, 2):
+                sess.step(chunk)
+                trace.append(sess.positions())
+                trace.append(sess.forces(7))
+            out.append(trace)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

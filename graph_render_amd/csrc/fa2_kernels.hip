@@ -828,7 +828,6 @@ __device__ __forceinline__ void com_cell(const FaDev &d, uint32_t k) {
 
 constexpr uint32_t COM_TILE = 4096;
 constexpr uint32_t SPAN_SLOTS = 32;   // spanning cells are ancestors of the tile's end: <= 1 per level
-constexpr uint32_t SPAN_LDS = 12288;  // span cells k_com_span sorts in LDS (more: plain loop)
 
 // One workgroup per tile of COM_TILE consecutive cells, its sums held in LDS. Leaves (and
 // cells the deep builder already summed) are read in one parallel pass; then every cell
@@ -935,42 +934,25 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
 }
 
 // The cells whose subtree crosses a tile (the ancestors of tile ends), finished by one
-// workgroup level by level, bucketed by level in LDS.
+// workgroup level by level: bucketed by level into d.span_cells, each one's children read
+// once into d.span_kids (structure only, all cells in parallel), then per level one round
+// of independent loads of the children's sums.
 template <typename T, int D>
 __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntiles) {
     const GrState *st = d.st;
     if (halted(d)) return;
-    __shared__ uint32_t cells[SPAN_LDS];
     __shared__ uint32_t loff[34];
     __shared__ uint32_t lcnt[33];
-    __shared__ uint32_t total_s;
     const uint32_t C = st->n_cells;
     ntiles = (C + COM_TILE - 1) / COM_TILE < ntiles ? (C + COM_TILE - 1) / COM_TILE : ntiles;
     if (threadIdx.x < 33) lcnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) total_s = 0;
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-        uint32_t cnt = d.span_n[t];
-        atomicAdd(&total_s, cnt);
+        const uint32_t cnt = d.span_n[t];
         for (uint32_t q = 0; q < cnt; q++)
             atomicAdd(&lcnt[d.clevel[d.span[t * SPAN_SLOTS + q]]], 1u);
     }
     __syncthreads();
-    const uint32_t total = total_s;
-    if (total > SPAN_LDS) {
-        // very large trees: the same levels, straight from the tile slots
-        for (int l = 31; l >= 0; l--) {
-            if (lcnt[l] == 0) continue;
-            for (uint32_t t = threadIdx.x; t < ntiles * SPAN_SLOTS; t += blockDim.x) {
-                uint32_t tile = t / SPAN_SLOTS, q = t % SPAN_SLOTS;
-                if (q >= d.span_n[tile]) continue;
-                uint32_t k = d.span[t];
-                if (d.clevel[k] == (uint32_t)l) com_cell<T, D>(d, k);
-            }
-            __syncthreads();
-        }
-        return;
-    }
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (int l = 0; l < 33; l++) {
@@ -982,15 +964,16 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
     __syncthreads();
     if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
     __syncthreads();
+    uint32_t *cells = d.span_cells;
     for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-        uint32_t cnt = d.span_n[t];
+        const uint32_t cnt = d.span_n[t];
         for (uint32_t q = 0; q < cnt; q++) {
-            uint32_t k = d.span[t * SPAN_SLOTS + q];
+            const uint32_t k = d.span[t * SPAN_SLOTS + q];
             cells[atomicAdd(&lcnt[d.clevel[k]], 1u)] = k;
         }
     }
     __syncthreads();
-    // each span cell's children, read once (structure only), into a small global table
+    const uint32_t total = loff[33];
     constexpr int KIDS = 1 << D;
     {
         const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
@@ -1080,9 +1063,26 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         end = e > end ? e : end;
     }
     end = gr_uniform(end < ncells ? end : ncells);
+    k = gr_uniform(k);
+    // The cell record travels as W dwords, lane l holding dword l: vector loads complete in
+    // order, so both possible successors (k + 1 and skip) can be fetched while this cell is
+    // worked on, and the one taken is waited for alone.
+    constexpr uint32_t W = sizeof(WalkCell<T, D>) / 4;
+    const uint32_t lane = gr_lane();
+    const uint32_t *__restrict__ wcw = (const uint32_t *)d.wc;
+    uint32_t cur = (lane < W && k < end) ? wcw[(size_t)k * W + lane] : 0u;
     while (k < end) {
-        k = gr_uniform(k);
-        const WalkCell<T, D> cell = wc[k];
+        WalkCell<T, D> cell;
+        {
+            uint32_t w[W];
+#pragma unroll
+            for (uint32_t q = 0; q < W; q++) w[q] = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)q);
+            __builtin_memcpy(&cell, w, sizeof(cell));
+        }
+        const uint32_t k1 = k + 1;
+        const uint32_t ks = cell.skip > k ? cell.skip : k1;
+        const uint32_t nx1 = (lane < W && k1 < end) ? wcw[(size_t)k1 * W + lane] : 0u;
+        const uint32_t nxs = (lane < W && ks < end) ? wcw[(size_t)ks * W + lane] : 0u;
         const bool act = next == k;
         bool open = false;
         if (act) {
@@ -1149,7 +1149,13 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             }
         }
         // skip always points past k; the max keeps the scan finite on any arena
-        k = (__ballot(open) != 0 || cell.skip <= k) ? k + 1 : cell.skip;
+        if (__ballot(open) != 0 || cell.skip <= k) {
+            k = k1;
+            cur = nx1;
+        } else {
+            k = ks;
+            cur = nxs;
+        }
     }
     if (live) {
 #pragma unroll

@@ -48,6 +48,7 @@ struct FaDev {
     // [gchunk[b], gchunk[b+1]); chunk c holds nodes [chunk[c], chunk[c+1]) of graph chunk_g[c]
     uint32_t ngraphs = 1;
     uint32_t levels = 31;  // key levels below the root (the graph id sits above them)
+    uint32_t key_bits = 64;  // key bits the sort orders: levels, graph id, non-finite flag
     GrGraph *gr = nullptr;
     uint32_t *gnode = nullptr, *gidn = nullptr;  // gidn: node -> graph (null for one graph)
     uint32_t nchunks = 0;

@@ -94,7 +94,8 @@ __device__ __forceinline__ Box<D> root_box(const GrGraph &g) {
     return b;
 }
 
-// Key layout: [63: non-finite][graph id][L levels of D slot bits], level 1 highest.
+// Key layout: [non-finite][graph id][L levels of D slot bits], level 1 highest, in the low
+// key_bits bits.
 template <int D>
 __device__ __forceinline__ int key_slot(uint64_t key, int level, int L) {
     return (int)((key >> (D * (L - level))) & ((1u << D) - 1));
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
         p[a] = (double)((const T *)d.pos[a])[i];
         if (!isfinite(p[a])) fin = false;
     }
-    uint64_t key = 1ull << 63;
+    uint64_t key = 1ull << (d.key_bits - 1);  // non-finite: after every finite key
     uint32_t stall = 0xFF;
     if (fin) {
         const int L = (int)d.levels;
@@ -1649,8 +1650,16 @@ static void tree_impl(FaDev &d) {
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
     hipStream_t s = d.stream;
     head_impl<D>(d);
-    hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
-    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, n, 64, d.sort_tmp, s);
+    // keys are written where an odd number of sort passes starts, so the sorted pairs
+    // always end in (d.keys, d.vals)
+    FaDev dk = d;
+    if (radix_digits((int)d.key_bits) & 1) {
+        dk.keys = d.keys_alt;
+        dk.vals = d.vals_alt;
+    }
+    hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, dk);
+    radix_sort_pairs(dk.keys, dk.vals, dk.keys == d.keys ? d.keys_alt : d.keys,
+                     dk.vals == d.vals ? d.vals_alt : d.vals, n, (int)d.key_bits, d.sort_tmp, s);
     hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.flag, d.gid, n, nullptr, &d.st->n_groups, d.scan_tmp, s);
     hipLaunchKernelGGL((k_group_compact<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);

@@ -2,6 +2,7 @@
 // and its CSR, start positions, device residency of one Fa2State and the iteration loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -556,10 +557,24 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     // batch tables: node offsets, graph-aligned node chunks, node -> graph
     d.ngraphs = ngraphs;
     {
+        // The sort takes whole 8-bit digits. The key holds the fewest digits that still give
+        // WANT levels (the flag and graph id included); points sharing all of them are
+        // separated by the deep-group builder, which replays the reference insertion below
+        // the last key level, so the tree is the same for any level count.
         const uint32_t gbits = ngraphs > 1 ? (uint32_t)std::ceil(std::log2((double)ngraphs)) : 0;
         const uint32_t maxl = d.dim == 2 ? 31 : 21;
-        const uint32_t fit = (63 - gbits) / d.dim;
-        d.levels = fit < maxl ? fit : maxl;
+        uint32_t biggest = 1;
+        for (uint32_t b = 0; b < ngraphs; b++) biggest = std::max(biggest, node_counts[b]);
+        // levels to spare beyond the depth an even spread of the biggest graph fills
+        const uint32_t fill = (uint32_t)std::ceil(std::log2((double)biggest) / d.dim);
+        const uint32_t want = fill + (d.dim == 2 ? 8 : 5);
+        uint32_t digits = 4, levels = 0;
+        for (;; digits++) {
+            levels = (8 * digits - 1 - gbits) / d.dim;
+            if (levels >= want || levels >= maxl || digits == 8) break;
+        }
+        d.levels = levels < maxl ? levels : maxl;
+        d.key_bits = d.dim * d.levels + gbits + 1;
     }
     std::vector<uint32_t> gnode(ngraphs + 1, 0), gchunk(ngraphs + 1, 0), chunk(1, 0), chunk_g;
     const uint32_t CHUNK = 2048;

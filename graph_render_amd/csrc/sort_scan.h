@@ -12,8 +12,11 @@ size_t scan_tmp_bytes(uint32_t max_n);
 void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uint32_t *d_count,
                     uint32_t *d_total, void *tmp, hipStream_t stream);
 
-// Stable LSD sort of n (key, value) pairs by key bits [0, end_bit).
+// Stable LSD sort of n (key, value) pairs by key bits [0, end_bit), radix_digits(end_bit)
+// passes; the result is in (keys, vals) after an even number of passes and in
+// (keys_alt, vals_alt) after an odd number.
 size_t radix_tmp_bytes(uint32_t n);
+int radix_digits(int end_bit);
 void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                       uint32_t n, int end_bit, void *tmp, hipStream_t stream);
 

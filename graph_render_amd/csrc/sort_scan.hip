@@ -230,8 +230,9 @@ size_t radix_tmp_bytes(uint32_t n) {
     return ((size_t)RS_BUCKETS * nb + RS_BUCKETS) * sizeof(uint32_t) + 256;
 }
 
-// Sorts bits [0, end_bit) of the keys, rounded up to whole 8-bit digits. The result lands
-// in (keys, vals); (keys_alt, vals_alt) are scratch of the same size.
+// Sorts bits [0, end_bit) of the keys, rounded up to whole 8-bit digits, ping-ponging
+// between (keys, vals) and (keys_alt, vals_alt): after an even number of digits the result
+// is in (keys, vals), after an odd number in the alternates. Nothing is copied.
 void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                       uint32_t n, int end_bit, void *tmp, hipStream_t stream) {
     if (n == 0) return;
@@ -255,10 +256,8 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32
         vi = vo;
         vo = tv;
     }
-    if (ki != keys) {
-        (void)hipMemcpyAsync(keys, ki, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
-        (void)hipMemcpyAsync(vals, vi, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
-    }
 }
+
+int radix_digits(int end_bit) { return (end_bit + RS_BITS - 1) / RS_BITS; }
 
 }  // namespace gr

@@ -1414,6 +1414,27 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
                 }
                 __syncthreads();
                 const uint32_t lim = e1 < s1 ? e1 : s1;
+                // four neighbours' gathers in flight at once, then their terms in entry
+                // order: the same sum as one entry at a time
+                constexpr int U = 4;
+                for (; e + U <= lim; e += U) {
+                    T pj[U][D];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t j = s_nbr[e + u - s0];
+#pragma unroll
+                        for (int a = 0; a < D; a++) pj[u][a] = pi[(size_t)j * D + a];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+#pragma unroll
+                        for (int a = 0; a < D; a++) {
+                            T dd = pj[u][a] - pv[a];
+                            if (wgt) dd = dd * s_w[e + u - s0];
+                            acc_add<T, KAHAN>(uv[a], cc[a], dd);
+                        }
+                    }
+                }
                 for (; e < lim; e++) {
                     const uint32_t j = s_nbr[e - s0];
                     T pj[D];

@@ -1070,8 +1070,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     // worked on, and the one taken is waited for alone.
     constexpr uint32_t W = sizeof(WalkCell<T, D>) / 4;
     const uint32_t lane = gr_lane();
+    const uint32_t dw = lane < W ? lane : 0u;  // lanes past the record re-read dword 0
     const uint32_t *__restrict__ wcw = (const uint32_t *)d.wc;
-    uint32_t cur = (lane < W && k < end) ? wcw[(size_t)k * W + lane] : 0u;
+    uint32_t cur = k < end ? wcw[(size_t)k * W + dw] : 0u;
     while (k < end) {
         WalkCell<T, D> cell;
         {
@@ -1082,31 +1083,35 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         }
         const uint32_t k1 = k + 1;
         const uint32_t ks = cell.skip > k ? cell.skip : k1;
-        const uint32_t nx1 = (lane < W && k1 < end) ? wcw[(size_t)k1 * W + lane] : 0u;
-        const uint32_t nxs = (lane < W && ks < end) ? wcw[(size_t)ks * W + lane] : 0u;
+        // successors past the scan's end are never used: read the last cell instead
+        const uint32_t nx1 = wcw[(size_t)(k1 < end ? k1 : end - 1) * W + dw];
+        const uint32_t nxs = wcw[(size_t)(ks < end ? ks : end - 1) * W + dw];
+        // every lane evaluates the test (no divergent branch); only acting lanes keep it
         const bool act = next == k;
-        bool open = false;
-        if (act) {
-            const bool holds = cell.start <= p && p < cell.end;
-            T dx[D];
+        const bool holds = cell.start <= p && p < cell.end;
+        T dx[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            dx[a] = xi[a] - cell.c[a];
+            if constexpr (!std::is_same<T, double>::value) dx[a] = dx[a] - cell.clo[a];
+        }
+        T d2 = dx[0] * dx[0] + dx[1] * dx[1];
+        if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
+        const bool accept = act && !holds && d2 > (T)0 && cell.w2 < theta2 * d2;
+        if (__ballot(accept) != 0) {
+            const T ff = pair_force<T>(mi * cell.mass, d2, kr);
 #pragma unroll
             for (int a = 0; a < D; a++) {
-                dx[a] = xi[a] - cell.c[a];
-                if constexpr (!std::is_same<T, double>::value) dx[a] = dx[a] - cell.clo[a];
+                T s = f[a], c = fc[a];
+                acc_add<T, KAHAN>(s, c, dx[a] * ff);
+                f[a] = accept ? s : f[a];
+                fc[a] = accept ? c : fc[a];
             }
-            T d2 = dx[0] * dx[0] + dx[1] * dx[1];
-            if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
-            if (!holds && d2 > (T)0 && cell.w2 < theta2 * d2) {
-                const T ff = pair_force<T>(mi * cell.mass, d2, kr);
-#pragma unroll
-                for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], dx[a] * ff);
-                next = cell.skip;
-            } else {
-                next = k + 1;
-                open = true;
-            }
-            if (next >= hi) next = 0xFFFFFFFFu;  // past its graph's tree: done
         }
+        const bool open = act && !accept;
+        uint32_t nxt = accept ? cell.skip : k1;
+        nxt = nxt >= hi ? 0xFFFFFFFFu : nxt;  // past its graph's tree: done
+        next = act ? nxt : next;
         const bool leaf = cell.skip == k + 1;
         const bool pairs = open && leaf;
         if (__ballot(pairs) != 0) {

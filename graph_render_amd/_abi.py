@@ -56,6 +56,7 @@ SIGNATURES = {
                                          C.c_void_p, C.POINTER(GrFa2Params), C.c_void_p]),
     "gr_fa2_session_release": (None, [C.c_void_p]),
     "gr_fa2_session_step": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "gr_fa2_session_set_params": (C.c_int, [C.c_void_p, C.POINTER(GrFa2Params), C.c_double]),
     "gr_fa2_session_positions": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_fa2_session_set_positions": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_fa2_session_forces": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),

@@ -784,6 +784,37 @@ gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_
     return ok();
 }
 
+gr_status gr_fa2_session_set_params(gr_fa2_session *s, const gr_fa2_params *params,
+                                    double theta) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (!params) return fail(GR_ERR_PARAM, "params", "a parameter block");
+    gr_fa2_params p = *params;
+    if (p.dim != s->d.dim)
+        return fail(GR_ERR_UNSUPPORTED, "dim", "the dimension the session was created with");
+    // the published ranges (registry/tunable.rs:229-240), refused rather than clamped
+    struct Check {
+        const char *name;
+        double v, lo, hi;
+    } checks[3] = {{"jitter_tolerance", p.jitter_tolerance, 0.0, 1e6},
+                   {"scaling_ratio", p.scaling_ratio, 0.0, 1e6},
+                   {"gravity", p.gravity, 0.0, 1e6}};
+    for (const Check &c : checks)
+        if (!std::isfinite(c.v) || c.v < c.lo || c.v > c.hi)
+            return fail(GR_ERR_PARAM_OUT_OF_RANGE, c.name,
+                        "inside the range the schema publishes, never clamped");
+    if (!std::isfinite(theta)) return fail(GR_ERR_PARAM, "theta", "a finite opening angle");
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
+    s->params = p;
+    s->d.scaling_ratio = p.scaling_ratio;
+    s->d.gravity = p.gravity;
+    s->d.jitter_tolerance = p.jitter_tolerance;
+    s->d.seed = p.seed;
+    if (theta >= 0.0) s->d.theta2 = theta * theta;
+    s->drop_graph();      // its launches carry the old values
+    s->drop_recording();  // a reverse sweep would replay them with the new ones
+    return ok();
+}
+
 gr_status gr_fa2_session_record(gr_fa2_session *s, uint32_t iters, uint32_t checkpoint_every,
                                 uint32_t *done_out) {
     if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");

@@ -367,6 +367,14 @@ class Session:
         assert p.size == self.n * self.dim
         _check(A.lib().gr_fa2_session_set_positions(self._h, p.ctypes.data))
 
+    def set_params(self, params: Fa2Params, theta: Optional[float] = None):
+        """New jitter tolerance, scaling ratio, gravity and seed (and opening angle) for
+        the following iterations; the adaptive-speed state carries on."""
+        cp = params._c()
+        _check(A.lib().gr_fa2_session_set_params(self._h, C.byref(cp),
+                                                 -1.0 if theta is None else float(theta)))
+        self.params = params
+
     def record(self, iters: int, checkpoint_every: int = 20) -> int:
         """Runs up to `iters` iterations like step(), keeping a checkpoint every
         `checkpoint_every` iterations for adjoint(). Returns the iterations applied."""

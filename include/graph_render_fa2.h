@@ -130,6 +130,15 @@ void gr_fa2_session_release(gr_fa2_session *s);
  * receives the iterations actually applied by this call. */
 gr_status gr_fa2_session_step(gr_fa2_session *s, uint32_t iters, uint32_t *done_out);
 
+/* Changes a live session's parameters between steps (the studio's tick-by-tick drive of a
+ * layout, layout/force/session.rs; the reference's FA2 takes them once, state.rs:20-54):
+ * jitter_tolerance, scaling_ratio, gravity and seed take effect on the next iteration;
+ * theta >= 0 replaces the opening angle (< 0 keeps it). Values outside the published
+ * ranges (registry/tunable.rs:229-240) are refused, never clamped; dim must not change.
+ * The swing/traction/speed state carries on. Drops any recording. */
+gr_status gr_fa2_session_set_params(gr_fa2_session *s, const gr_fa2_params *params,
+                                    double theta);
+
 /* Current positions, n*dim row-major f64 (widened from FP32 in the FP32 precisions). */
 gr_status gr_fa2_session_positions(gr_fa2_session *s, double *out);
 gr_status gr_fa2_session_set_positions(gr_fa2_session *s, const double *in);

@@ -590,6 +590,15 @@ void gro_fa2_free(gro_fa2 *s) {
     free(s);
 }
 
+/* A live session's parameter change between iterations: the next iterate() uses them. */
+void gro_fa2_set_params(gro_fa2 *s, double jitter_tolerance, double scaling_ratio,
+                        double gravity, uint32_t seed) {
+    s->jitter_tolerance = jitter_tolerance;
+    s->scaling_ratio = scaling_ratio;
+    s->gravity = gravity;
+    s->seed = seed;
+}
+
 void gro_fa2_set_theta2(gro_fa2 *s, double theta2, int repulsion) {
     s->theta2 = theta2;
     s->repulsion = repulsion;

@@ -63,6 +63,8 @@ void gro_fa2_forces(gro_fa2 *s, uint32_t parts, double *out);
 void gro_fa2_scalars(const gro_fa2 *s, double *out5);
 uint32_t gro_fa2_simple_edges(const gro_fa2 *s, uint32_t *lo, uint32_t *hi);
 void gro_fa2_set_theta2(gro_fa2 *s, double theta2, int repulsion);
+void gro_fa2_set_params(gro_fa2 *s, double jitter_tolerance, double scaling_ratio,
+                        double gravity, uint32_t seed);
 
 /* ---- quadtree / octree (layout/force/quadtree*.rs, octree*.rs) ---- */
 typedef struct gro_tree gro_tree;

@@ -62,6 +62,8 @@ def lib():
         L.gro_fa2_simple_edges.restype = C.c_uint32
         L.gro_fa2_simple_edges.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.gro_fa2_set_theta2.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.gro_fa2_set_params.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                         C.c_uint32]
         L.gro_tree_build.restype = C.c_void_p
         L.gro_tree_build.argtypes = [C.c_uint32, C.c_uint32, f64p, f64p, C.c_void_p]
         L.gro_tree_free.argtypes = [C.c_void_p]
@@ -178,6 +180,9 @@ class Fa2:
 
     def set_repulsion(self, tree: bool, theta2: float = THETA2_REFERENCE):
         lib().gro_fa2_set_theta2(self._h, theta2, 1 if tree else 0)
+
+    def set_params(self, jitter_tolerance=1.0, scaling_ratio=2.0, gravity=1.0, seed=0):
+        lib().gro_fa2_set_params(self._h, jitter_tolerance, scaling_ratio, gravity, seed)
 
     def simple_edges(self):
         m = lib().gro_fa2_simple_edges(self._h, None, None)

@@ -341,3 +341,38 @@ def test_graph_replay_equals_direct_launches(repulsion):
             out.append(trace)
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+def test_live_parameter_change_tracks_the_oracle(oracle):
+    # a session drives FA2 tick by tick and changes its parameters between ticks; the
+    # device (through a replayed graph) follows the oracle doing the same
+    n = 1500
+    s, t = oracle.gate_edges(n)
+    ref = oracle_state(oracle, n, s, t)
+    with session(n, s, t, max_iter=1000) as sess:
+        sess.step(3)
+        for _ in range(3):
+            ref.iterate()
+        p2 = G.Fa2Params(scaling_ratio=5.0, gravity=0.25, jitter_tolerance=0.5, seed=7)
+        sess.set_params(p2, theta=1.1)
+        ref.set_params(jitter_tolerance=0.5, scaling_ratio=5.0, gravity=0.25, seed=7)
+        ref.set_repulsion(True, 1.1 * 1.1)
+        sess.step(3)
+        for _ in range(3):
+            ref.iterate()
+        g, r = sess.positions(), ref.positions()
+    ext = (r.max(0) - r.min(0)).max()
+    assert np.abs(g - r).max() <= 1e-10 * ext
+
+
+def test_set_params_refuses_out_of_range_values():
+    s, t = bh_graph(100)
+    with session(100, s, t) as sess:
+        for bad in (dict(gravity=-1.0), dict(scaling_ratio=float("nan")),
+                    dict(jitter_tolerance=2e6)):
+            with pytest.raises(G.StageError) as e:
+                sess.set_params(G.Fa2Params(**bad))
+            assert e.value.name == next(iter(bad))
+        with pytest.raises(G.StageError):
+            sess.set_params(G.Fa2Params(dim=3))
+        sess.step(2)  # still usable

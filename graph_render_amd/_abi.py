@@ -73,6 +73,9 @@ SIGNATURES = {
                                          C.c_void_p]),
     "gr_fa2_session_time": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32,
                                       C.POINTER(C.c_double)]),
+    "gr_pm_side_for": (C.c_uint32, [C.c_uint32]),
+    "gr_pm_charge": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "gr_snapshot_encode": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,

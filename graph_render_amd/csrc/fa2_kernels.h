@@ -103,6 +103,11 @@ struct FaDev {
     unsigned long long *pm_acc = nullptr;  // fixed-point deposit
     void *pm_fr = nullptr;    // frame and the rung the spectrum was built for
     double pm_fix = 0.0;      // fixed-point scale of the deposit (a power of two)
+    // law: 0 FA2 (mass deposit, G = r/|r|^2, infinite reach, read times k*m_i); 1 d3's
+    // many-body (unit charge, G = -r/l with l raised below dmin^2 and cut at dmax^2, reach
+    // ceil(dmax/h), read times charge*alpha; particle_mesh/kernel.rs, charge.rs)
+    int pm_law = 0;
+    double pm_dmin2 = 0.0, pm_dmax2 = 0.0, pm_strength = 0.0;
 
     int ignore_done = 0;  // set on force evaluations, which run after an early exit too
 
@@ -111,6 +116,8 @@ struct FaDev {
 
 size_t walkcell_bytes(int precision, uint32_t dim);
 size_t pm_frame_bytes();
+// step, h, origin x, origin y, cells, reach, valid from a frame record (pm.hip)
+void pm_frame_read(const void *frame, double *out7);
 void pm_twiddles(uint32_t side, float2 *host_out);
 
 // Positions changed from the host: recompute the bounds/mean partials.

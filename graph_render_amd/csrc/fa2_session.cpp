@@ -1179,6 +1179,59 @@ gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t rep
     return ok();
 }
 
+// mesh.rs:31-34: ceil(sqrt n) rounded up to a power of two, held to 128..1024
+uint32_t gr_pm_side_for(uint32_t n) {
+    const double root = std::ceil(std::sqrt((double)n));
+    uint32_t side = 1;
+    while ((double)side < root) side <<= 1;
+    return side < 128 ? 128 : (side > 1024 ? 1024 : side);
+}
+
+gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charge, double alpha,
+                       double distance_min, double distance_max, double *out,
+                       double *frame_out) {
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (frame_out)
+        for (int k = 0; k < 7; k++) frame_out[k] = 0.0;
+    if (!std::isfinite(charge) || !std::isfinite(alpha) || !(distance_min >= 0.0) ||
+        !(distance_max >= 0.0) || std::isnan(distance_max))
+        return fail(GR_ERR_PARAM, "charge", "finite charge and alpha, distances >= 0");
+    if (side == 0) side = gr_pm_side_for(n);
+    if (n == 0) return ok();
+    gr_fa2_params p;
+    gr_fa2_params_default(&p);
+    gr_fa2_ext x;
+    gr_fa2_ext_default(&x);
+    x.repulsion = GR_REP_PM;
+    x.precision = GR_PREC_FP64;
+    x.pm_grid = side;
+    x.start = xy;
+    x.flags = GR_EXT_DIRECT_LAUNCH;
+    gr_fa2_session *s = gr_fa2_session_create(n, nullptr, nullptr, 0, &p, &x);
+    if (!s) return g_err.code;
+    gr::FaDev &d = s->d;
+    d.pm_law = 1;
+    d.pm_dmin2 = distance_min * distance_min;
+    d.pm_dmax2 = distance_max * distance_max;
+    d.pm_strength = charge * alpha;
+    d.pm_fix = std::ldexp(1.0, 60 - (int)std::ceil(std::log2((double)n + 1.0)));  // unit charges
+    gr::launch_forces(d, gr::PART_REPULSION);
+    gr_status r = GR_OK;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(d.stream) != hipSuccess)
+        r = fail(GR_ERR_DEVICE, "pm", "the particle-mesh launches failed");
+    if (!r) r = read_columns(s, d.rep, out);
+    if (!r && frame_out) {
+        std::vector<uint8_t> fr(gr::pm_frame_bytes());
+        if (hipMemcpy(fr.data(), d.pm_fr, fr.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+            r = fail(GR_ERR_DEVICE, "pm", "frame readback");
+        } else {
+            gr::pm_frame_read(fr.data(), frame_out);
+        }
+    }
+    gr_fa2_session_release(s);
+    return r ? r : ok();
+}
+
 gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *source,
                         const uint32_t *target, size_t m, const uint8_t *params,
                         size_t params_len, const gr_fa2_ext *ext, float *out_x, float *out_y,

@@ -45,13 +45,19 @@ __global__ void k_pm_frame(const FaDev d) {
     const int side = (int)d.pm_side;
     const double span = fmax(g->box_hi[0] - g->box_lo[0], g->box_hi[1] - g->box_lo[1]);
     if (d.n < 2 || g->n_finite == 0 || !isfinite(span)) return;
+    if (d.pm_law == 1 && !(d.pm_dmax2 > 0.0)) return;  // mesh.rs:166: no law, no field
     double s0 = floor(4.0 * log2(span / (double)side));
     int step = s0 < -32.0 ? -32 : (int)s0;
     for (int guard = 0; guard < 4096; guard++, step++) {
         const double h = pm_h(step);
         const double cells = floor(span / h) + 3.0;
         if (cells > (double)side) continue;
-        const int c = (int)cells, reach = c - 1;
+        const int c = (int)cells;
+        int reach = c - 1;
+        if (d.pm_law == 1) {  // frame.rs:77-78: the law's own reach
+            const double r = ceil(sqrt(d.pm_dmax2) / h);
+            reach = r < (double)(c - 1) ? (int)r : c - 1;
+        }
         if (c + reach > side) continue;
         f->step = step;
         f->h = h;
@@ -95,8 +101,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_sample(const FaDev d) {
         float2 g = make_float2(0.f, 0.f);
         if (dy >= -reach && dy <= reach && dx >= -reach && dx <= reach) {
             const double rx = dx * f->h, ry = dy * f->h;
-            const double l = rx * rx + ry * ry;
-            if (l != 0.0) g = make_float2((float)(rx / l * scale), (float)(ry / l * scale));
+            double l = rx * rx + ry * ry;
+            if (d.pm_law == 1) {  // kernel.rs:80-92
+                if (l != 0.0 && l < d.pm_dmax2) {
+                    if (l < d.pm_dmin2) l = sqrt(d.pm_dmin2 * l);
+                    g = make_float2((float)(-rx / l * scale), (float)(-ry / l * scale));
+                }
+            } else if (l != 0.0) {
+                g = make_float2((float)(rx / l * scale), (float)(ry / l * scale));
+            }
         }
         d.pm_b[q] = g;
     }
@@ -215,7 +228,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_deposit(const FaDev d) {
     int cx, cy;
     double fx, fy;
     pm_stencil(f, px, py, cx, cy, fx, fy);
-    const double m = (double)((const T *)d.mass)[i] * d.pm_fix;
+    const double m = (d.pm_law == 1 ? 1.0 : (double)((const T *)d.mass)[i]) * d.pm_fix;
     const uint32_t P = d.pm_side;
     const size_t at = (size_t)cy * P + cx;
     atomicAdd(&d.pm_acc[at], (unsigned long long)llrint(m * ((1.0 - fx) * (1.0 - fy))));
@@ -269,7 +282,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_read(const FaDev d) {
         ex += (double)e.x * w[q];
         ey += (double)e.y * w[q];
     }
-    const double km = d.scaling_ratio * (double)((const T *)d.mass)[i];
+    const double km =
+        d.pm_law == 1 ? d.pm_strength : d.scaling_ratio * (double)((const T *)d.mass)[i];
     rx[i] = (T)(km * ex);
     ry[i] = (T)(km * ey);
 }
@@ -317,6 +331,17 @@ void pm_repulsion(FaDev &d) {
 }
 
 size_t pm_frame_bytes() { return sizeof(PmFrame); }
+
+void pm_frame_read(const void *frame, double *out7) {
+    const PmFrame *f = (const PmFrame *)frame;
+    out7[0] = f->step;
+    out7[1] = f->h;
+    out7[2] = f->ox;
+    out7[3] = f->oy;
+    out7[4] = f->cells;
+    out7[5] = f->reach;
+    out7[6] = f->valid;
+}
 
 void pm_twiddles(uint32_t P, float2 *host_out) {
     for (uint32_t t = 0; t < P / 2; t++) {

@@ -202,6 +202,22 @@ gr_status gr_snapshot_encode(uint32_t n, const char *const *node_ids, uint32_t m
                              const uint32_t *note_index, uint8_t *out, size_t cap,
                              size_t *len_out);
 
+/* ---- the particle mesh under d3's many-body law (layout.force's PM arm) ---- */
+
+/* mesh.rs:31-34: the mesh side for n nodes, ceil(sqrt n) to a power of two in 128..1024. */
+uint32_t gr_pm_side_for(uint32_t n);
+
+/* One charge pass of the reference's particle mesh (particle_mesh/charge.rs:29-47) on the
+ * device: frame (frame.rs:50-89, reach ceil(dmax/h)), unit-charge CIC deposit, the law
+ * G(r) = -r/l (l = |r|^2, raised to sqrt(dmin^2 l) below dmin^2, zero at l >= dmax^2,
+ * kernel.rs:80-92) convolved on a side x side mesh (0 = gr_pm_side_for(n)), CIC read, times
+ * charge * alpha: the velocity each node gains (out, n*2 f64). xy: n*2 row-major f64.
+ * frame_out (may be NULL): step, h, origin x, origin y, cells, reach, 1 if a frame was
+ * placed. The mesh transforms in FP32; positions, frame and deposit weights are FP64. */
+gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charge, double alpha,
+                       double distance_min, double distance_max, double *out,
+                       double *frame_out);
+
 /* Last failure of the calling thread: code, parameter/column name, rule text. */
 gr_status gr_last_error(const char **name, const char **rule);
 

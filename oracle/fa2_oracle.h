@@ -85,6 +85,8 @@ double gro_pm_h(int step);
 void gro_fft2(double *re, double *im, int side, int sign);
 int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, int side,
                   double *out, double *frame_out);
+int gro_pm_charge(uint32_t n, const double *xy, int side, double charge, double alpha,
+                  double dmin, double dmax, double *out, double *frame_out);
 
 #ifdef __cplusplus
 }

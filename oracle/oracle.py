@@ -77,6 +77,9 @@ def lib():
         L.gro_fft2.argtypes = [f64p, f64p, C.c_int, C.c_int]
         L.gro_pm_forces.restype = C.c_int
         L.gro_pm_forces.argtypes = [C.c_uint32, f64p, f64p, C.c_double, C.c_int, f64p, f64p]
+        L.gro_pm_charge.restype = C.c_int
+        L.gro_pm_charge.argtypes = [C.c_uint32, f64p, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, f64p, f64p]
         _lib = L
     return _lib
 
@@ -247,6 +250,22 @@ def pm_forces(xy: np.ndarray, mass: np.ndarray, k: float, side: int):
     ok = lib().gro_pm_forces(n, xy.reshape(-1) if n else np.zeros(2),
                              np.ascontiguousarray(mass, np.float64) if n else np.zeros(1),
                              k, side, out, fr)
+    frame = None
+    if ok:
+        frame = dict(h=fr[0], ox=fr[1], oy=fr[2], step=int(fr[3]), cells=int(fr[4]),
+                     reach=int(fr[5]))
+    return out[: 2 * n].reshape(n, 2), frame
+
+
+def pm_charge(xy: np.ndarray, side: int, charge: float = -90.0, alpha: float = 1.0,
+              distance_min: float = 1.0, distance_max: float = 520.0):
+    """d3-law particle-mesh charge pass (the velocities it adds): (deltas (n,2), frame)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    fr = np.zeros(6)
+    ok = lib().gro_pm_charge(n, xy.reshape(-1) if n else np.zeros(2), side, charge, alpha,
+                             distance_min, distance_max, out, fr)
     frame = None
     if ok:
         frame = dict(h=fr[0], ox=fr[1], oy=fr[2], step=int(fr[3]), cells=int(fr[4]),

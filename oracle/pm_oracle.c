@@ -1,6 +1,15 @@
 /*
  * CPU oracle for the particle-mesh repulsion: TEST INFRASTRUCTURE ONLY.
  *
+ * Two laws share one mesh solve (frame, CIC deposit, FFT convolution, CIC read):
+ *   gro_pm_forces  FA2's repulsion, described below;
+ *   gro_pm_charge  the reference's own d3 many-body law (particle_mesh/kernel.rs:1-12,
+ *                  80-92; charge.rs:29-47): unit charge per node, G(r) = -r/l with l raised
+ *                  to sqrt(dmin^2 l) below dmin^2 and zero at l >= dmax^2, reach
+ *                  ceil(dmax/h) (frame.rs:70-80), read times charge * alpha. Pinned against
+ *                  the reference's fixtures/gpu/mesh-1k-{start,settled}.gmfx (frame and
+ *                  charge deltas, tests/test_pm_oracle.py).
+ *
  * FA2's repulsion k*m_i*m_j*(p_i - p_j)/|p_i - p_j|^2 evaluated on a mesh, in FP64:
  *   frame   the rung ladder of crates/graph-core/src/layout/force/particle_mesh/frame.rs:50-89
  *           (h = 2^(step/4), cells = floor(span/h) + 3, origin snapped to a multiple of h)
@@ -37,8 +46,10 @@ typedef struct {
     double h, ox, oy;
 } pm_frame;
 
-/* frame.rs:50-89 with reach = cells - 1. Returns 0 when no frame exists. */
-static int place(double lox, double loy, double hix, double hiy, int side, pm_frame *f) {
+/* frame.rs:50-89: the smallest rung whose cells and reach (ceil(dmax/h), at most cells - 1;
+ * FA2's infinite reach is cells - 1) fit the side. Returns 0 when no frame exists. */
+static int place(double lox, double loy, double hix, double hiy, int side, double dmax,
+                 pm_frame *f) {
     double span = fmax(hix - lox, hiy - loy);
     if (!isfinite(span)) return 0;
     double st = floor(4.0 * log2(span / (double)side));
@@ -48,7 +59,8 @@ static int place(double lox, double loy, double hix, double hiy, int side, pm_fr
         double cells = floor(span / h) + 3.0;
         if (cells > (double)side) continue;
         int c = (int)cells;
-        int reach = c - 1;
+        double r = ceil(dmax / h);
+        int reach = r < (double)(c - 1) ? (int)r : c - 1;
         if (c + reach > side) continue;
         f->step = step;
         f->h = h;
@@ -124,11 +136,12 @@ void gro_fft2(double *re, double *im, int side, int sign) {
     free(ti);
 }
 
-/* forces: out[i] = k*m_i*E(p_i); frame_out (may be NULL): h, ox, oy, step, cells, reach.
- * Returns 1 if a frame was placed, 0 otherwise (out is zeroed). */
-int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, int side,
-                  double *out, double *frame_out) {
-    memset(out, 0, sizeof(double) * 2 * n);
+/* The mesh field E at every node: deposit `charge` (NULL: unit charges) with CIC weights,
+ * convolve with the law (0: FA2's r/|r|^2; 1: d3's -r/l with its cutoffs), read with CIC.
+ * Returns 1 if a frame was placed (e is 2n), else 0 (e zeroed). */
+static int pm_field(uint32_t n, const double *xy, const double *charge, int side, int law,
+                    double dmin2, double dmax2, double *e, double *frame_out) {
+    memset(e, 0, sizeof(double) * 2 * n);
     double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY;
     uint32_t nf = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -141,7 +154,10 @@ int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, in
         nf++;
     }
     pm_frame f;
-    if (n < 2 || nf == 0 || !place(lox, loy, hix, hiy, side, &f)) return 0;
+    const double dmax = law == 1 ? sqrt(dmax2) : INFINITY;
+    if (n < 2 || nf == 0 || (law == 1 && !(dmax2 > 0.0)) ||
+        !place(lox, loy, hix, hiy, side, dmax, &f))
+        return 0;
     if (frame_out) {
         frame_out[0] = f.h;
         frame_out[1] = f.ox;
@@ -159,7 +175,7 @@ int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, in
         int cx, cy;
         double fx, fy;
         stencil(&f, x, y, &cx, &cy, &fx, &fy);
-        double m = mass[i];
+        double m = charge ? charge[i] : 1.0;
         size_t at = (size_t)cy * side + cx;
         dre[at] += m * ((1.0 - fx) * (1.0 - fy));
         dre[at + 1] += m * (fx * (1.0 - fy));
@@ -171,10 +187,17 @@ int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, in
         for (int dx = -f.reach; dx <= f.reach; dx++) {
             double rx = dx * f.h, ry = dy * f.h;
             double l = rx * rx + ry * ry;
-            if (l == 0.0) continue;
             size_t at = (size_t)((dy + side) % side) * side + (size_t)((dx + side) % side);
-            gre[at] = rx / l * scale;
-            gim[at] = ry / l * scale;
+            if (law == 1) {
+                if (l == 0.0 || l >= dmax2) continue;
+                if (l < dmin2) l = sqrt(dmin2 * l);
+                gre[at] = -rx / l * scale;
+                gim[at] = -ry / l * scale;
+            } else {
+                if (l == 0.0) continue;
+                gre[at] = rx / l * scale;
+                gim[at] = ry / l * scale;
+            }
         }
     }
     gro_fft2(dre, dim_, side, -1);
@@ -188,7 +211,7 @@ int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, in
     for (uint32_t i = 0; i < n; i++) {
         double x = xy[2 * i], y = xy[2 * i + 1];
         if (!isfinite(x) || !isfinite(y)) {
-            out[2 * i] = out[2 * i + 1] = NAN;
+            e[2 * i] = e[2 * i + 1] = NAN;
             continue;
         }
         int cx, cy;
@@ -202,12 +225,35 @@ int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, in
             ex += dre[cell[q]] * w[q];
             ey += dim_[cell[q]] * w[q];
         }
-        out[2 * i] = k * mass[i] * ex;
-        out[2 * i + 1] = k * mass[i] * ey;
+        e[2 * i] = ex;
+        e[2 * i + 1] = ey;
     }
     free(dre);
     free(dim_);
     free(gre);
     free(gim);
+    return 1;
+}
+
+/* FA2: out[i] = k*m_i*E(p_i), mass deposited; frame_out (may be NULL): h, ox, oy, step,
+ * cells, reach. Returns 1 if a frame was placed, 0 otherwise (out is zeroed). */
+int gro_pm_forces(uint32_t n, const double *xy, const double *mass, double k, int side,
+                  double *out, double *frame_out) {
+    int placed = pm_field(n, xy, mass, side, 0, 0.0, 0.0, out, frame_out);
+    if (!placed) return 0;
+    for (uint32_t i = 0; i < n; i++) {
+        out[2 * i] *= k * mass[i];
+        out[2 * i + 1] *= k * mass[i];
+    }
+    return 1;
+}
+
+/* d3: out[i] = charge*alpha*E(p_i), unit charges deposited (charge.rs:29-47). */
+int gro_pm_charge(uint32_t n, const double *xy, int side, double charge, double alpha,
+                  double dmin, double dmax, double *out, double *frame_out) {
+    int placed = pm_field(n, xy, NULL, side, 1, dmin * dmin, dmax * dmax, out, frame_out);
+    if (!placed) return 0;
+    const double strength = charge * alpha;
+    for (uint32_t i = 0; i < 2 * n; i++) out[i] *= strength;
     return 1;
 }

@@ -57,3 +57,22 @@ def test_pm_layout_runs_and_stays_finite():
         assert sess.step(30) == 30
         p = sess.positions()
     assert np.isfinite(p).all()
+
+
+@pytest.mark.parametrize("state", ["start", "settled"])
+def test_d3_charge_pass_matches_the_reference_fixture(state):
+    """The device mesh under the reference's d3 law against its own fixture (f64): frame
+    bit for bit, charge deltas within the FP32 transform's error."""
+    import os
+    from graph_render_amd import pm
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "pm_d3_mesh1k.npz"))
+    xy, want, fr = f[f"{state}_xy"], f[f"{state}_delta_charge"], f[f"{state}_frame"]
+    assert pm.side_for(xy.shape[0]) == int(fr[6]) == 128
+    got, frame = pm.charge_pass(xy)
+    print(f"d3 charge pass {state}: frame {frame}", flush=True)
+    assert frame["step"] == int(fr[0]) and frame["h"] == fr[1]
+    assert (frame["ox"], frame["oy"]) == (fr[2], fr[3])
+    assert frame["cells"] == int(fr[4]) and frame["reach"] == int(fr[5])
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"  max error / max |delta| = {err:.2e}", flush=True)
+    assert err <= 1e-5

@@ -70,3 +70,22 @@ def test_mesh_is_linear_in_mass(oracle):
     f1, _ = oracle.pm_forces(xy, m, 1.0, 128)
     f2, _ = oracle.pm_forces(xy, m, 2.0, 128)
     assert np.allclose(f2, 2 * f1, rtol=1e-12, atol=1e-12)
+
+
+def _d3_fixture():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "pm_d3_mesh1k.npz")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("state", ["start", "settled"])
+def test_d3_law_charge_pass_reproduces_the_reference_fixture(oracle, state):
+    """fixtures/gpu/mesh-1k-*.gmfx: the reference's own frame and charge-pass deltas (f64)
+    for n = 1000 on a 128 mesh, at layout.force's defaults and alpha 1."""
+    f = _d3_fixture()
+    xy, want, fr = f[f"{state}_xy"], f[f"{state}_delta_charge"], f[f"{state}_frame"]
+    got, frame = oracle.pm_charge(xy, int(fr[6]), -90.0, 1.0, 1.0, 520.0)
+    assert frame["step"] == int(fr[0]) and frame["h"] == fr[1]
+    assert (frame["ox"], frame["oy"]) == (fr[2], fr[3])
+    assert frame["cells"] == int(fr[4]) and frame["reach"] == int(fr[5])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()

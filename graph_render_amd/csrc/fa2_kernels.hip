@@ -1086,9 +1086,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         // successors past the scan's end are never used: read the last cell instead
         const uint32_t nx1 = wcw[(size_t)(k1 < end ? k1 : end - 1) * W + dw];
         const uint32_t nxs = wcw[(size_t)(ks < end ? ks : end - 1) * W + dw];
-        // every lane evaluates the test (no divergent branch); only acting lanes keep it
-        const bool act = next == k;
-        const bool holds = cell.start <= p && p < cell.end;
+        // every lane evaluates the test (no divergent branch); only acting lanes keep it.
+        // The tests are kept as wave masks (compare results), combined on the scalar unit.
+        const uint64_t acting = wave_cmp_u32<GR_CMP_EQ>(next, k);
+        const uint64_t holding = wave_cmp_u32<GR_CMP_ULE>(cell.start, p) &
+                                 wave_cmp_u32<GR_CMP_ULT>(p, cell.end);
         T dx[D];
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -1097,8 +1099,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         }
         T d2 = dx[0] * dx[0] + dx[1] * dx[1];
         if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
-        const bool accept = act && !holds && d2 > (T)0 && cell.w2 < theta2 * d2;
-        if (__ballot(accept) != 0) {
+        // barnes_hut.rs's d2 > 0 is implied: w2 >= 0, so w2 < theta2 * d2 needs d2 > 0
+        const uint64_t accepted = acting & ~holding & wave_lt(cell.w2, theta2 * d2);
+        const bool accept = __builtin_amdgcn_inverse_ballot_w64(accepted);
+        const bool act = __builtin_amdgcn_inverse_ballot_w64(acting);
+        if (accepted != 0) {
             const T ff = pair_force<T>(mi * cell.mass, d2, kr);
 #pragma unroll
             for (int a = 0; a < D; a++) {
@@ -1108,13 +1113,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
                 fc[a] = accept ? c : fc[a];
             }
         }
-        const bool open = act && !accept;
-        uint32_t nxt = accept ? cell.skip : k1;
+        const bool open = act & !accept;
+        const uint64_t opened = acting & ~accepted;
+        // skip after an accept, k + 1 after an open (as an offset: one uniform operand)
+        uint32_t nxt = k1 + (accept ? cell.skip - k1 : 0u);
         nxt = nxt >= hi ? 0xFFFFFFFFu : nxt;  // past its graph's tree: done
         next = act ? nxt : next;
-        const bool leaf = cell.skip == k + 1;
-        const bool pairs = open && leaf;
-        if (__ballot(pairs) != 0) {
+        const bool leaf = cell.skip == k1;
+        if (leaf && opened != 0) {
+            const bool pairs = open;
             T lf[D], lc[D];
 #pragma unroll
             for (int a = 0; a < D; a++) lf[a] = lc[a] = (T)0;
@@ -1155,13 +1162,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             }
         }
         // skip always points past k; the max keeps the scan finite on any arena
-        if (__ballot(open) != 0 || cell.skip <= k) {
-            k = k1;
-            cur = nx1;
-        } else {
-            k = ks;
-            cur = nxs;
-        }
+        const bool down = opened != 0 || cell.skip <= k;
+        k = down ? k1 : ks;
+        cur = down ? nx1 : nxs;
     }
     if (live) {
 #pragma unroll

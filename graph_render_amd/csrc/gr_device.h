@@ -86,6 +86,21 @@ __device__ __forceinline__ uint32_t gr_uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// A comparison as a wave mask (bit l: lane l's result; inactive lanes read 0), so tests can
+// be combined with scalar mask operations. The codes are the compiler's integer predicates.
+enum { GR_CMP_EQ = 32, GR_CMP_ULT = 36, GR_CMP_ULE = 37 };
+template <int PRED>
+__device__ __forceinline__ uint64_t wave_cmp_u32(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_uicmp(a, b, PRED);
+}
+// a < b (ordered: false when either is NaN) as a wave mask
+__device__ __forceinline__ uint64_t wave_lt(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, 4);
+}
+__device__ __forceinline__ uint64_t wave_lt(double a, double b) {
+    return __builtin_amdgcn_fcmp(a, b, 4);
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8). This
 // maps them so that XCD x owns logical blocks [x*q, (x+1)*q): consecutive bodies or rows
 // share one XCD's L2. Launch gr_xcd_grid(nb) workgroups; a logical index >= nb is idle.

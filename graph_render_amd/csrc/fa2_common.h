@@ -3,6 +3,7 @@
 // force and the early-exit/halt test.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <type_traits>
@@ -27,7 +28,23 @@ struct WalkCell : CentreLo<T, D> {
     T mass;
     T w2;
     uint32_t skip, start, end;
+    // dword index of `mass` in the record (the lo centre, if any, then the centre)
+    static constexpr uint32_t MASS_DWORD =
+        (std::is_same<T, double>::value ? 0u : (uint32_t)D) + (uint32_t)(D * sizeof(T) / 4);
+    // dword index of `start` (`end` follows it): after mass, w2 and skip
+    static constexpr uint32_t START_DWORD = MASS_DWORD + (uint32_t)(2 * sizeof(T) / 4) + 1;
 };
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winvalid-offsetof"
+template <typename C>
+constexpr bool mass_dword_ok() {
+    return offsetof(C, mass) == 4 * C::MASS_DWORD && offsetof(C, start) == 4 * C::START_DWORD &&
+           offsetof(C, end) == 4 * (C::START_DWORD + 1);
+}
+static_assert(mass_dword_ok<WalkCell<float, 2>>() && mass_dword_ok<WalkCell<float, 3>>() &&
+                  mass_dword_ok<WalkCell<double, 2>>() && mass_dword_ok<WalkCell<double, 3>>(),
+              "MASS_DWORD and START_DWORD name the record's fields");
+#pragma clang diagnostic pop
 
 __device__ __forceinline__ uint32_t graph_of(const FaDev &d, uint32_t node) {
     return d.gidn ? d.gidn[node] : 0u;

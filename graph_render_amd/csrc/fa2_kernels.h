@@ -60,6 +60,7 @@ struct FaDev {
     uint32_t *vals = nullptr, *vals_alt = nullptr;
     uint8_t *stall = nullptr;
     uint32_t *order = nullptr;
+    uint32_t *leaf = nullptr;  // per tree position: the leaf cell holding it
     void *sp[3] = {};
     void *sm = nullptr;
     uint32_t *flag = nullptr, *gid = nullptr, *dflag = nullptr;

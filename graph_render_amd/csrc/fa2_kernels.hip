@@ -714,6 +714,10 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
     wc[k].start = start;
     wc[k].end = end;
     d.clevel[k] = (uint8_t)level;
+    if (skip == k + 1) {  // a leaf: its positions point back to it (the walk's own-cell test)
+        const uint32_t e = end < d.n ? end : d.n;
+        for (uint32_t q = start; q < e; q++) d.leaf[q] = k;
+    }
     if (d.export_bounds) {
         double *o = d.cbounds + (size_t)k * 2 * D;
 #pragma unroll
@@ -1023,7 +1027,9 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
 // the smallest of them, so each cell is loaded once per wave (a scalar load) and each lane
 // still sees its own cells in preorder, accepting or opening them exactly as the reference
 // does. A lane parked on an accepted cell waits until the wave reaches that cell's skip.
-template <typename T, int D, bool KAHAN>
+// BUF: the cell arena spans under 4 GiB, so records are read as buffer loads: a scalar byte
+// offset per record and the lane's dword offset, fixed for the whole scan.
+template <typename T, int D, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
@@ -1049,6 +1055,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const uint32_t seed = d.seed, tick = gg.iter_no;
     const uint32_t lo = gg.cell_lo, hi = gg.cell_hi;
     const uint32_t ncells = st->n_cells;
+    const uint32_t own = (live && p < st->n_finite) ? d.leaf[p] : 0xFFFFFFFFu;
     T f[D], fc[D];
 #pragma unroll
     for (int a = 0; a < D; a++) f[a] = fc[a] = (T)0;
@@ -1072,7 +1079,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const uint32_t lane = gr_lane();
     const uint32_t dw = lane < W ? lane : 0u;  // lanes past the record re-read dword 0
     const uint32_t *__restrict__ wcw = (const uint32_t *)d.wc;
-    uint32_t cur = k < end ? wcw[(size_t)k * W + dw] : 0u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)wcw, (short)0, BUF ? (int)(d.cap * W * 4) : 0, 0x00020000);
+    auto record = [&](uint32_t c) -> uint32_t {
+        if constexpr (BUF) {
+            return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(dw * 4),
+                                                                 (int)(c * (W * 4)), 0);
+        } else {
+            return wcw[(size_t)c * W + dw];
+        }
+    };
+    uint32_t cur = k < end ? record(k) : 0u;
     while (k < end) {
         WalkCell<T, D> cell;
         {
@@ -1084,13 +1101,16 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         const uint32_t k1 = k + 1;
         const uint32_t ks = cell.skip > k ? cell.skip : k1;
         // successors past the scan's end are never used: read the last cell instead
-        const uint32_t nx1 = wcw[(size_t)(k1 < end ? k1 : end - 1) * W + dw];
-        const uint32_t nxs = wcw[(size_t)(ks < end ? ks : end - 1) * W + dw];
+        const uint32_t nx1 = record(k1 < end ? k1 : end - 1);
+        const uint32_t nxs = record(ks < end ? ks : end - 1);
         // every lane evaluates the test (no divergent branch); only acting lanes keep it.
         // The tests are kept as wave masks (compare results), combined on the scalar unit.
-        const uint64_t acting = wave_cmp_u32<GR_CMP_EQ>(next, k);
-        const uint64_t holding = wave_cmp_u32<GR_CMP_ULE>(cell.start, p) &
-                                 wave_cmp_u32<GR_CMP_ULT>(p, cell.end);
+        // a lane acts on k when k is its next cell and still inside its graph's tree
+        const uint64_t acting = wave_cmp_u32<GR_CMP_EQ>(next, k) & wave_cmp_u32<GR_CMP_ULT>(k, hi);
+        // the cell holds the lane's own point when its subtree [k, skip) holds that leaf
+        // (the same as start <= p < end, as a subtree's positions are its leaves')
+        const uint64_t holding = wave_cmp_u32<GR_CMP_ULE>(k, own) &
+                                 wave_cmp_u32<GR_CMP_ULT>(own, cell.skip);
         T dx[D];
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -1104,7 +1124,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         const bool accept = __builtin_amdgcn_inverse_ballot_w64(accepted);
         const bool act = __builtin_amdgcn_inverse_ballot_w64(acting);
         if (accepted != 0) {
-            const T ff = pair_force<T>(mi * cell.mass, d2, kr);
+            // the mass is read from the record only here, where a lane takes the force
+            constexpr uint32_t MD = WalkCell<T, D>::MASS_DWORD;
+            T cmass;
+            if constexpr (std::is_same<T, double>::value) {
+                const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)MD);
+                const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)MD + 1);
+                cmass = __builtin_bit_cast(double, lo | (hi << 32));
+            } else {
+                cmass = __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)cur, (int)MD));
+            }
+            const T ff = pair_force<T>(mi * cmass, d2, kr);
 #pragma unroll
             for (int a = 0; a < D; a++) {
                 T s = f[a], c = fc[a];
@@ -1116,17 +1146,21 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         const bool open = act & !accept;
         const uint64_t opened = acting & ~accepted;
         // skip after an accept, k + 1 after an open (as an offset: one uniform operand)
-        uint32_t nxt = k1 + (accept ? cell.skip - k1 : 0u);
-        nxt = nxt >= hi ? 0xFFFFFFFFu : nxt;  // past its graph's tree: done
-        next = act ? nxt : next;
+        // skip after an accept, k + 1 after an open (an acting lane's next is k); a next at
+        // or past hi never acts again, as the scan only moves forward
+        next = accept ? cell.skip : next + (act ? 1u : 0u);
         const bool leaf = cell.skip == k1;
         if (leaf && opened != 0) {
             const bool pairs = open;
             T lf[D], lc[D];
 #pragma unroll
             for (int a = 0; a < D; a++) lf[a] = lc[a] = (T)0;
-            const uint32_t qend = cell.end < n ? cell.end : n;
-            for (uint32_t q = cell.start; q < qend; q++) {
+            // the leaf's range is read from the record only here
+            constexpr uint32_t SD = WalkCell<T, D>::START_DWORD;
+            const uint32_t cstart = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)SD);
+            const uint32_t cend = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)SD + 1);
+            const uint32_t qend = cend < n ? cend : n;
+            for (uint32_t q = cstart; q < qend; q++) {
                 const uint32_t j = order[q];
                 T xj[D];
 #pragma unroll
@@ -1703,8 +1737,11 @@ static void tree_impl(FaDev &d) {
 
 template <typename T, int D, bool KAHAN>
 static void walk_impl(FaDev &d) {
-    hipLaunchKernelGGL((k_walk<T, D, KAHAN>), dim3(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK))),
-                       dim3(GR_BLOCK), 0, d.stream, d);
+    const dim3 grid(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK)));
+    if ((uint64_t)d.cap * sizeof(WalkCell<T, D>) < (1ull << 31))
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+    else
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
 }
 
 void pm_repulsion(FaDev &d);  // pm.hip

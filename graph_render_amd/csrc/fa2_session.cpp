@@ -605,6 +605,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     TRY_ALLOC(d.vals_alt, nn * 4);
     TRY_ALLOC(d.stall, nn);
     TRY_ALLOC(d.order, nn * 4);
+    TRY_ALLOC(d.leaf, nn * 4);
     TRY_ALLOC(d.flag, nn * 4);
     TRY_ALLOC(d.gid, nn * 4);
     TRY_ALLOC(d.dflag, nn * 4);

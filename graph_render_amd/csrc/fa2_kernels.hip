@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const uint32_t dw = lane < W ? lane : 0u;  // lanes past the record re-read dword 0
     const uint32_t *__restrict__ wcw = (const uint32_t *)d.wc;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)wcw, (short)0, BUF ? (int)(d.cap * W * 4) : 0, 0x00020000);
+        (void *)wcw, (short)0, BUF ? (int)((d.cap + 1) * W * 4) : 0, 0x00020000);
     auto record = [&](uint32_t c) -> uint32_t {
         if constexpr (BUF) {
             return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(dw * 4),
@@ -1099,10 +1099,12 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             __builtin_memcpy(&cell, w, sizeof(cell));
         }
         const uint32_t k1 = k + 1;
-        const uint32_t ks = cell.skip > k ? cell.skip : k1;
-        // successors past the scan's end are never used: read the last cell instead
-        const uint32_t nx1 = record(k1 < end ? k1 : end - 1);
-        const uint32_t nxs = record(ks < end ? ks : end - 1);
+        // skip always points past k; the max keeps the scan finite on any arena, the min
+        // keeps the read inside it (the arena holds one spare record past its capacity, so
+        // cell `end` may be read; a successor at or past end is never used)
+        const uint32_t ks = min(max(cell.skip, k1), end);
+        const uint32_t nx1 = record(k1);
+        const uint32_t nxs = record(ks);
         // every lane evaluates the test (no divergent branch); only acting lanes keep it.
         // The tests are kept as wave masks (compare results), combined on the scalar unit.
         // a lane acts on k when k is its next cell and still inside its graph's tree
@@ -1144,13 +1146,12 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             }
         }
         const bool open = act & !accept;
-        const uint64_t opened = acting & ~accepted;
-        // skip after an accept, k + 1 after an open (as an offset: one uniform operand)
+        // some lane opened k (accepted lanes are a subset of the acting ones)
+        const bool down = acting != accepted;
         // skip after an accept, k + 1 after an open (an acting lane's next is k); a next at
         // or past hi never acts again, as the scan only moves forward
         next = accept ? cell.skip : next + (act ? 1u : 0u);
-        const bool leaf = cell.skip == k1;
-        if (leaf && opened != 0) {
+        if (down && cell.skip == k1) {
             const bool pairs = open;
             T lf[D], lc[D];
 #pragma unroll
@@ -1195,8 +1196,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
                 for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], lf[a]);
             }
         }
-        // skip always points past k; the max keeps the scan finite on any arena
-        const bool down = opened != 0 || cell.skip <= k;
+        // down to k + 1 when a lane opened k, else on to skip
         k = down ? k1 : ks;
         cur = down ? nx1 : nxs;
     }
@@ -1738,7 +1738,7 @@ static void tree_impl(FaDev &d) {
 template <typename T, int D, bool KAHAN>
 static void walk_impl(FaDev &d) {
     const dim3 grid(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK)));
-    if ((uint64_t)d.cap * sizeof(WalkCell<T, D>) < (1ull << 31))
+    if ((uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31))
         hipLaunchKernelGGL((k_walk<T, D, KAHAN, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
     else
         hipLaunchKernelGGL((k_walk<T, D, KAHAN, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);

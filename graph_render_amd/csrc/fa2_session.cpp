@@ -243,7 +243,9 @@ struct gr_fa2_session {
     gr_status alloc_cells(uint32_t cap) {
         d.cap = cap;
         gr_status s;
-        if ((s = alloc(&d.wc, (size_t)cap * gr::walkcell_bytes(d.precision, d.dim)))) return s;
+        // one spare record: the walk may read the record just past the scanned cells
+        if ((s = alloc(&d.wc, ((size_t)cap + 1) * gr::walkcell_bytes(d.precision, d.dim))))
+            return s;
         if ((s = alloc(&d.clevel, (size_t)cap))) return s;
         const size_t tiles = (cap + 4095) / 4096;
         if ((s = alloc(&d.span, tiles * 32 * 4))) return s;

@@ -1027,9 +1027,10 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
 // the smallest of them, so each cell is loaded once per wave (a scalar load) and each lane
 // still sees its own cells in preorder, accepting or opening them exactly as the reference
 // does. A lane parked on an accepted cell waits until the wave reaches that cell's skip.
-// BUF: the cell arena spans under 4 GiB, so records are read as buffer loads: a scalar byte
+// BUF: the cell arena spans under 2 GiB, so records are read as buffer loads: a scalar byte
 // offset per record and the lane's dword offset, fixed for the whole scan.
-template <typename T, int D, bool KAHAN, bool BUF>
+// ONE: a single-graph session, where every lane's tree ends at the scan's end.
+template <typename T, int D, bool KAHAN, bool BUF, bool ONE>
 __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
@@ -1059,8 +1060,12 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     T f[D], fc[D];
 #pragma unroll
     for (int a = 0; a < D; a++) f[a] = fc[a] = (T)0;
-    uint32_t next = (live && lo < hi) ? lo : 0xFFFFFFFFu;
-    uint32_t k = next;
+    // A lane acts on the scan's cell k when it opened the cell before (the scan then moves
+    // to k + 1, its next cell), or when k is the cell it waits for: its tree's root, or the
+    // skip of the last cell it accepted. Waits only move forward with the scan.
+    uint32_t wait = (live && lo < hi) ? lo : 0xFFFFFFFFu;
+    uint64_t carried = 0;
+    uint32_t k = wait;
     // the scan ends at the last cell of the wave's last graph: later trees are others'
     uint32_t end = live ? hi : 0u;
 #pragma unroll
@@ -1107,8 +1112,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         const uint32_t nxs = record(ks);
         // every lane evaluates the test (no divergent branch); only acting lanes keep it.
         // The tests are kept as wave masks (compare results), combined on the scalar unit.
-        // a lane acts on k when k is its next cell and still inside its graph's tree
-        const uint64_t acting = wave_cmp_u32<GR_CMP_EQ>(next, k) & wave_cmp_u32<GR_CMP_ULT>(k, hi);
+        // the acting lanes, inside their graph's tree (a lane's tree ends at its hi)
+        const uint64_t acting =
+            ONE ? carried | wave_cmp_u32<GR_CMP_EQ>(wait, k)
+                : (carried | wave_cmp_u32<GR_CMP_EQ>(wait, k)) & wave_cmp_u32<GR_CMP_ULT>(k, hi);
         // the cell holds the lane's own point when its subtree [k, skip) holds that leaf
         // (the same as start <= p < end, as a subtree's positions are its leaves')
         const uint64_t holding = wave_cmp_u32<GR_CMP_ULE>(k, own) &
@@ -1146,11 +1153,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             }
         }
         const bool open = act & !accept;
-        // some lane opened k (accepted lanes are a subset of the acting ones)
-        const bool down = acting != accepted;
-        // skip after an accept, k + 1 after an open (an acting lane's next is k); a next at
-        // or past hi never acts again, as the scan only moves forward
-        next = accept ? cell.skip : next + (act ? 1u : 0u);
+        // the lanes that opened k act on k + 1 next; an accepting lane waits for skip
+        carried = acting & ~accepted;
+        const bool down = carried != 0;
+        wait = accept ? cell.skip : wait;
         if (down && cell.skip == k1) {
             const bool pairs = open;
             T lf[D], lc[D];
@@ -1738,10 +1744,14 @@ static void tree_impl(FaDev &d) {
 template <typename T, int D, bool KAHAN>
 static void walk_impl(FaDev &d) {
     const dim3 grid(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK)));
-    if ((uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31))
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+    const bool buf = (uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31);
+    const bool one = d.gidn == nullptr;
+    if (buf && one)
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+    else if (buf)
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
     else
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
 }
 
 void pm_repulsion(FaDev &d);  // pm.hip

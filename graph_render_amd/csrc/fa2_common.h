@@ -80,6 +80,21 @@ __device__ __forceinline__ bool halted(const FaDev &d) {
 
 // m_a*m_b/d2*k. FP64 divides exactly as the reference does; the FP32 precisions use the
 // hardware reciprocal (about 1 ulp), well inside their 1e-5 force tolerance.
+// a * b + c: FP32 fuses it into one rounding (FP32 results are held to a tolerance, not
+// to bits); FP64 rounds the product and the sum apart, as the reference's f64 code does.
+template <typename T>
+__device__ __forceinline__ T mul_add(T a, T b, T c) {
+    if constexpr (std::is_same<T, float>::value) {
+        return __builtin_fmaf(a, b, c);
+    } else {
+        return a * b + c;
+    }
+}
+
+// The hardware reciprocal (FP32 only).
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
 template <typename T>
 __device__ __forceinline__ T pair_force(T mm, T d2, T kr) {
     if constexpr (std::is_same<T, double>::value) {

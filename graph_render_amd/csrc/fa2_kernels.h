@@ -72,8 +72,10 @@ struct FaDev {
     uint8_t *clevel = nullptr;  // cell depth; 255 for cells summed by the deep builder
     uint32_t *span = nullptr;   // cells whose subtree crosses a centre-of-mass tile
     uint32_t *span_n = nullptr; // per tile: how many of its span slots are used
+    uint8_t *span_lev = nullptr; // the level of each span slot's cell
     uint32_t *span_cells = nullptr; // the span cells bucketed by level (32 slots per tile)
     uint32_t *span_kids = nullptr;  // children of the span cells (2^dim per cell)
+    uint32_t *span_slot = nullptr;  // per cell: its slot in span_cells (span cells only)
     double *raw = nullptr;      // unnormalised (mass, m*x...) per cell
     double *cbounds = nullptr;  // 2*dim per cell when the session exports the tree
     int export_bounds = 0;

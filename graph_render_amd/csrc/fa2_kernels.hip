@@ -888,7 +888,10 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
             for (int a = 0; a <= D; a++) sums[i][a] = r[a];
         } else if (skip > t1) {
             uint32_t at = atomicAdd(&nspan, 1u);  // LDS
-            if (at < SPAN_SLOTS) slots[at] = kk;
+            if (at < SPAN_SLOTS) {
+                slots[at] = kk;
+                d.span_lev[(size_t)blockIdx.x * SPAN_SLOTS + at] = (uint8_t)l;
+            }
             l = 255;
         } else if (skip == kk + 1) {
             double s[1 + D];
@@ -938,48 +941,160 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
     }
 }
 
+// The span cells the LDS path of k_com_span holds (2D), and its child references: a child
+// that is itself a span cell is named by its slot, flagged; any other child by its cell.
+constexpr uint32_t SPAN_LDS = 3072;
+constexpr int SPAN_PER = (int)(SPAN_LDS / 1024);
+constexpr uint32_t SPAN_REF = 0x80000000u, NO_KID = 0xFFFFFFFFu;
+
 // The cells whose subtree crosses a tile (the ancestors of tile ends), finished by one
-// workgroup level by level: bucketed by level into d.span_cells, each one's children read
-// once into d.span_kids (structure only, all cells in parallel), then per level one round
-// of independent loads of the children's sums.
+// workgroup level by level, deepest first, after bucketing them by level into
+// d.span_cells. In 2D, when they fit, every child that is not a span cell is final already:
+// its sums are loaded once, all in parallel, into registers, the span cells' sums stay in
+// LDS, and a level costs only LDS traffic and a barrier. Otherwise each one's children are
+// read once into d.span_kids and each level does one round of loads of their sums.
 template <typename T, int D>
 __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntiles) {
     const GrState *st = d.st;
     if (halted(d)) return;
     __shared__ uint32_t loff[34];
     __shared__ uint32_t lcnt[33];
+    constexpr int KIDS = 1 << D;
+    __shared__ double ssum[D == 2 ? SPAN_LDS : 1][1 + D];
+    __shared__ uint32_t kref[D == 2 ? SPAN_LDS : 1][KIDS];
+    __shared__ uint32_t scells[D == 2 ? SPAN_LDS : 1];  // the first span cells, as in d.span_cells
     const uint32_t C = st->n_cells;
     ntiles = (C + COM_TILE - 1) / COM_TILE < ntiles ? (C + COM_TILE - 1) / COM_TILE : ntiles;
     if (threadIdx.x < 33) lcnt[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-        const uint32_t cnt = d.span_n[t];
-        for (uint32_t q = 0; q < cnt; q++)
-            atomicAdd(&lcnt[d.clevel[d.span[t * SPAN_SLOTS + q]]], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int l = 0; l < 33; l++) {
-            loff[l] = acc;
-            acc += lcnt[l];
-        }
-        loff[33] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
-    __syncthreads();
+    // Every (tile, slot) pair on its own thread, SPAN_ROUNDS pairs per thread per chunk: a
+    // chunk's loads are all issued before any is used, and kept for the bucketing pass.
+    const uint32_t pairs = ntiles * SPAN_SLOTS;
+    constexpr int SPAN_ROUNDS = 16;
     uint32_t *cells = d.span_cells;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-        const uint32_t cnt = d.span_n[t];
-        for (uint32_t q = 0; q < cnt; q++) {
-            const uint32_t k = d.span[t * SPAN_SLOTS + q];
-            cells[atomicAdd(&lcnt[d.clevel[k]], 1u)] = k;
+    for (uint32_t base = 0; base < pairs; base += SPAN_ROUNDS * 1024u) {
+        uint32_t kv[SPAN_ROUNDS];
+        uint8_t lv[SPAN_ROUNDS];
+#pragma unroll
+        for (int r = 0; r < SPAN_ROUNDS; r++) {
+            const uint32_t i = base + (uint32_t)r * 1024u + threadIdx.x;
+            const bool used = i < pairs && (i % SPAN_SLOTS) < d.span_n[i / SPAN_SLOTS];
+            kv[r] = used ? d.span[i] : NO_KID;
+            lv[r] = used ? d.span_lev[i] : (uint8_t)0;
+        }
+#pragma unroll
+        for (int r = 0; r < SPAN_ROUNDS; r++)
+            if (kv[r] != NO_KID) atomicAdd(&lcnt[lv[r]], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int l = 0; l < 33; l++) {
+                loff[l] = acc;
+                acc += lcnt[l];
+            }
+            loff[33] = acc;
+        }
+        __syncthreads();
+        if (base == 0 && pairs <= SPAN_ROUNDS * 1024u) {
+            // the only chunk: bucket straight from the registers
+            if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SPAN_ROUNDS; r++) {
+                if (kv[r] == NO_KID) continue;
+                const uint32_t at = atomicAdd(&lcnt[lv[r]], 1u);
+                cells[at] = kv[r];
+                d.span_slot[kv[r]] = at;
+                if (D == 2 && at < SPAN_LDS) scells[at] = kv[r];
+            }
+            __syncthreads();
         }
     }
-    __syncthreads();
+    if (pairs > SPAN_ROUNDS * 1024u) {
+        // many tiles: a second pass over the pairs, bucketing by the counts just taken
+        if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < pairs; i += blockDim.x) {
+            if ((i % SPAN_SLOTS) < d.span_n[i / SPAN_SLOTS]) {
+                const uint32_t k = d.span[i];
+                const uint32_t at = atomicAdd(&lcnt[d.span_lev[i]], 1u);
+                cells[at] = k;
+                d.span_slot[k] = at;
+                if (D == 2 && at < SPAN_LDS) scells[at] = k;
+            }
+        }
+        __syncthreads();
+    }
     const uint32_t total = loff[33];
-    constexpr int KIDS = 1 << D;
+    const WalkCell<T, D> *wcs = (const WalkCell<T, D> *)d.wc;
+    if (D == 2 && total <= SPAN_LDS && C < SPAN_REF) {
+        double pre[SPAN_PER][KIDS][1 + D];
+#pragma unroll
+        for (int j = 0; j < SPAN_PER; j++) {
+            const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
+#pragma unroll
+            for (int x = 0; x < KIDS; x++)
+#pragma unroll
+                for (int y = 0; y <= D; y++) pre[j][x][y] = 0.0;
+            if (q >= total) continue;
+            const uint32_t k = scells[q], skip = wcs[k].skip;
+            uint32_t c = k + 1;
+#pragma unroll
+            for (int x = 0; x < KIDS; x++) {
+                uint32_t ref = NO_KID;
+                if (c < skip) {
+                    const uint32_t cs = wcs[c].skip;
+                    const uint32_t t0c = (c / COM_TILE) * COM_TILE;
+                    const uint32_t tend = t0c + COM_TILE < C ? t0c + COM_TILE : C;
+                    // the test k_com_tiles sorts a cell out by
+                    if (d.clevel[c] != 255 && cs > tend) {
+                        const uint32_t slot = d.span_slot[c];
+                        if (slot < total && scells[slot] == c) {
+                            ref = SPAN_REF | slot;
+                        } else {  // inconsistent span lists: never expected
+                            d.st->halt = 1;
+                            ref = NO_KID;
+                        }
+                    } else {
+                        ref = c;
+                        const double *r = d.raw + (size_t)c * (1 + D);
+#pragma unroll
+                        for (int y = 0; y <= D; y++) pre[j][x][y] = r[y];
+                    }
+                    c = cs > c ? cs : skip;
+                }
+                kref[q][x] = ref;
+            }
+        }
+        __syncthreads();
+        for (int l = 31; l >= 0; l--) {
+            const uint32_t a = loff[l], b = loff[l + 1];
+            if (a == b) continue;  // block-uniform
+#pragma unroll
+            for (int j = 0; j < SPAN_PER; j++) {
+                const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
+                if (q < a || q >= b) continue;
+                // children in slot order, each added exactly as k_com_tiles adds them
+                double s[1 + D];
+#pragma unroll
+                for (int y = 0; y <= D; y++) s[y] = 0.0;
+#pragma unroll
+                for (int x = 0; x < KIDS; x++) {
+                    const uint32_t ref = kref[q][x];
+                    if (ref == NO_KID) continue;
+                    const bool span = (ref & SPAN_REF) != 0;
+                    const uint32_t sl = span ? (ref & ~SPAN_REF) : 0u;
+#pragma unroll
+                    for (int y = 0; y <= D; y++) s[y] += span ? ssum[sl][y] : pre[j][x][y];
+                }
+#pragma unroll
+                for (int y = 0; y <= D; y++) ssum[q][y] = s[y];
+                body_final<T, D>(d, scells[q], s);
+            }
+            __syncthreads();
+        }
+        return;
+    }
     {
         const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
         for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
@@ -1023,10 +1138,10 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
 // ------------------------------------------------------------------ walk
 
 // barnes_hut.rs:112-184 as one wave-uniform preorder scan per 64 consecutive tree
-// positions. Every lane keeps its own next cell (the reference's `k`); the wave visits
-// the smallest of them, so each cell is loaded once per wave (a scalar load) and each lane
-// still sees its own cells in preorder, accepting or opening them exactly as the reference
-// does. A lane parked on an accepted cell waits until the wave reaches that cell's skip.
+// positions. The wave visits the smallest of its lanes' next cells (each lane's `k` of the
+// reference), so each cell is loaded once per wave and each lane still sees its own cells
+// in preorder, accepting or opening them exactly as the reference does. A lane that opened
+// the cell acts on k + 1; a lane that accepted it waits until the scan reaches its skip.
 // BUF: the cell arena spans under 2 GiB, so records are read as buffer loads: a scalar byte
 // offset per record and the lane's dword offset, fixed for the whole scan.
 // ONE: a single-graph session, where every lane's tree ends at the scan's end.
@@ -1051,6 +1166,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const T mi = sm[pc];
     const uint32_t i = order[pc];
     const T theta2 = (T)d.theta2, kr = (T)d.scaling_ratio;
+    const T mikr = mi * kr;
     // each lane walks its own graph's tree, cells [lo, hi) of the arena
     const GrGraph &gg = d.gr[graph_of(d, i)];
     const uint32_t seed = d.seed, tick = gg.iter_no;
@@ -1126,8 +1242,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             dx[a] = xi[a] - cell.c[a];
             if constexpr (!std::is_same<T, double>::value) dx[a] = dx[a] - cell.clo[a];
         }
-        T d2 = dx[0] * dx[0] + dx[1] * dx[1];
-        if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
+        T d2 = mul_add(dx[0], dx[0], dx[1] * dx[1]);
+        if (D == 3) d2 = mul_add(dx[D - 1], dx[D - 1], d2);
         // barnes_hut.rs's d2 > 0 is implied: w2 >= 0, so w2 < theta2 * d2 needs d2 > 0
         const uint64_t accepted = acting & ~holding & wave_lt(cell.w2, theta2 * d2);
         const bool accept = __builtin_amdgcn_inverse_ballot_w64(accepted);
@@ -1143,13 +1259,19 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             } else {
                 cmass = __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)cur, (int)MD));
             }
-            const T ff = pair_force<T>(mi * cmass, d2, kr);
+            // FP64 forms the reference's mi * mc / d2 * kr; FP32 takes (mi * kr) * mc / d2
+            const T ff = std::is_same<T, double>::value ? pair_force<T>(mi * cmass, d2, kr)
+                                                        : mikr * cmass * fast_rcp(d2);
 #pragma unroll
             for (int a = 0; a < D; a++) {
-                T s = f[a], c = fc[a];
-                acc_add<T, KAHAN>(s, c, dx[a] * ff);
-                f[a] = accept ? s : f[a];
-                fc[a] = accept ? c : fc[a];
+                if constexpr (KAHAN) {
+                    T s = f[a], c = fc[a];
+                    acc_add<T, KAHAN>(s, c, dx[a] * ff);
+                    f[a] = accept ? s : f[a];
+                    fc[a] = accept ? c : fc[a];
+                } else {
+                    f[a] = accept ? mul_add(dx[a], ff, f[a]) : f[a];
+                }
             }
         }
         const bool open = act & !accept;

@@ -250,8 +250,10 @@ struct gr_fa2_session {
         const size_t tiles = (cap + 4095) / 4096;
         if ((s = alloc(&d.span, tiles * 32 * 4))) return s;
         if ((s = alloc(&d.span_n, tiles * 4))) return s;
+        if ((s = alloc(&d.span_lev, tiles * 32))) return s;
         if ((s = alloc(&d.span_cells, tiles * 32 * 4))) return s;
         if ((s = alloc(&d.span_kids, tiles * 32 * ((size_t)1 << d.dim) * 4))) return s;
+        if ((s = alloc(&d.span_slot, (size_t)cap * 4))) return s;
         if ((s = alloc(&d.raw, (size_t)cap * (1 + d.dim) * 8))) return s;
         if (d.export_bounds && (s = alloc(&d.cbounds, (size_t)cap * 2 * d.dim * 8))) return s;
         return GR_OK;

@@ -665,6 +665,8 @@ __global__ void k_cells_ready(const FaDev d) {
         return;
     }
     st->n_cells = st->need_cells;
+    // the centre-of-mass pass counts its span cells per level from zero
+    for (int l = 0; l < 66; l++) d.span_lcnt[l] = 0;
 }
 
 
@@ -891,6 +893,7 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
             if (at < SPAN_SLOTS) {
                 slots[at] = kk;
                 d.span_lev[(size_t)blockIdx.x * SPAN_SLOTS + at] = (uint8_t)l;
+                atomicAdd(&d.span_lcnt[l], 1u);
             }
             l = 255;
         } else if (skip == kk + 1) {
@@ -941,129 +944,127 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
     }
 }
 
-// The span cells the LDS path of k_com_span holds (2D), and its child references: a child
-// that is itself a span cell is named by its slot, flagged; any other child by its cell.
+// The span cells the LDS path of k_com_span holds (2D), and child references: a child that
+// is itself a span cell is flagged (by its cell in d.span_kids, by its slot in LDS).
 constexpr uint32_t SPAN_LDS = 3072;
 constexpr int SPAN_PER = (int)(SPAN_LDS / 1024);
 constexpr uint32_t SPAN_REF = 0x80000000u, NO_KID = 0xFFFFFFFFu;
 
-// The cells whose subtree crosses a tile (the ancestors of tile ends), finished by one
-// workgroup level by level, deepest first, after bucketing them by level into
-// d.span_cells. In 2D, when they fit, every child that is not a span cell is final already:
-// its sums are loaded once, all in parallel, into registers, the span cells' sums stay in
-// LDS, and a level costs only LDS traffic and a barrier. Otherwise each one's children are
-// read once into d.span_kids and each level does one round of loads of their sums.
+// Level offsets of the span buckets from the per-level counts k_com_tiles took.
+__device__ __forceinline__ void span_offsets(const FaDev &d, uint32_t *loff) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int l = 0; l < 33; l++) {
+            loff[l] = acc;
+            acc += d.span_lcnt[l];
+        }
+        loff[33] = acc;
+    }
+    __syncthreads();
+}
+
+// One thread per (tile, span slot): puts the span cell in its level's bucket and gathers
+// its children in slot order (k + 1, then each child's skip): a child that also crosses a
+// tile is referenced, any other child is final and its sums are copied. Spread over many
+// workgroups, so the chains of dependent loads of all span cells run side by side.
 template <typename T, int D>
-__global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntiles) {
+__global__ __launch_bounds__(GR_BLOCK) void k_span_prep(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
     __shared__ uint32_t loff[34];
-    __shared__ uint32_t lcnt[33];
+    span_offsets(d, loff);
+    const uint32_t C = st->n_cells;
+    const uint32_t ntiles = (C + COM_TILE - 1) / COM_TILE;
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= ntiles * SPAN_SLOTS || (i % SPAN_SLOTS) >= d.span_n[i / SPAN_SLOTS]) return;
+    constexpr int KIDS = 1 << D;
+    const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
+    const uint32_t k = d.span[i], l = d.span_lev[i];
+    const uint32_t at = loff[l] + atomicAdd(&d.span_lcnt[33 + l], 1u);
+    d.span_cells[at] = k;
+    d.span_slot[k] = at;
+    const uint32_t skip = wc[k].skip;
+    uint32_t c = k + 1;
+#pragma unroll
+    for (int x = 0; x < KIDS; x++) {
+        uint32_t ref = NO_KID;
+        double v[1 + D];
+#pragma unroll
+        for (int y = 0; y <= D; y++) v[y] = 0.0;
+        if (c < skip) {
+            const uint32_t cs = wc[c].skip;
+            const uint32_t t0c = (c / COM_TILE) * COM_TILE;
+            const uint32_t tend = t0c + COM_TILE < C ? t0c + COM_TILE : C;
+            // the test k_com_tiles sorts a cell out by
+            if (d.clevel[c] != 255 && cs > tend) {
+                ref = SPAN_REF | c;
+            } else {
+                ref = c;
+                const double *r = d.raw + (size_t)c * (1 + D);
+#pragma unroll
+                for (int y = 0; y <= D; y++) v[y] = r[y];
+            }
+            c = cs > c ? cs : skip;
+        }
+        d.span_kids[(size_t)at * KIDS + x] = ref;
+#pragma unroll
+        for (int y = 0; y <= D; y++) d.span_kval[((size_t)at * KIDS + x) * (1 + D) + y] = v[y];
+    }
+}
+
+// The span cells (the ancestors of tile ends), finished by one workgroup level by level,
+// deepest first, from the buckets and children k_span_prep laid out. Every child sum is
+// added in slot order, exactly as k_com_tiles adds them. In 2D, when they fit, the span
+// cells' sums stay in LDS and a level costs only LDS traffic and a barrier; otherwise a
+// span child's sum is read back from d.raw, where the level before wrote it.
+template <typename T, int D>
+__global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ uint32_t loff[34];
     constexpr int KIDS = 1 << D;
     __shared__ double ssum[D == 2 ? SPAN_LDS : 1][1 + D];
     __shared__ uint32_t kref[D == 2 ? SPAN_LDS : 1][KIDS];
-    __shared__ uint32_t scells[D == 2 ? SPAN_LDS : 1];  // the first span cells, as in d.span_cells
-    const uint32_t C = st->n_cells;
-    ntiles = (C + COM_TILE - 1) / COM_TILE < ntiles ? (C + COM_TILE - 1) / COM_TILE : ntiles;
-    if (threadIdx.x < 33) lcnt[threadIdx.x] = 0;
-    __syncthreads();
-    // Every (tile, slot) pair on its own thread, SPAN_ROUNDS pairs per thread per chunk: a
-    // chunk's loads are all issued before any is used, and kept for the bucketing pass.
-    const uint32_t pairs = ntiles * SPAN_SLOTS;
-    constexpr int SPAN_ROUNDS = 16;
-    uint32_t *cells = d.span_cells;
-    for (uint32_t base = 0; base < pairs; base += SPAN_ROUNDS * 1024u) {
-        uint32_t kv[SPAN_ROUNDS];
-        uint8_t lv[SPAN_ROUNDS];
-#pragma unroll
-        for (int r = 0; r < SPAN_ROUNDS; r++) {
-            const uint32_t i = base + (uint32_t)r * 1024u + threadIdx.x;
-            const bool used = i < pairs && (i % SPAN_SLOTS) < d.span_n[i / SPAN_SLOTS];
-            kv[r] = used ? d.span[i] : NO_KID;
-            lv[r] = used ? d.span_lev[i] : (uint8_t)0;
-        }
-#pragma unroll
-        for (int r = 0; r < SPAN_ROUNDS; r++)
-            if (kv[r] != NO_KID) atomicAdd(&lcnt[lv[r]], 1u);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int l = 0; l < 33; l++) {
-                loff[l] = acc;
-                acc += lcnt[l];
-            }
-            loff[33] = acc;
-        }
-        __syncthreads();
-        if (base == 0 && pairs <= SPAN_ROUNDS * 1024u) {
-            // the only chunk: bucket straight from the registers
-            if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < SPAN_ROUNDS; r++) {
-                if (kv[r] == NO_KID) continue;
-                const uint32_t at = atomicAdd(&lcnt[lv[r]], 1u);
-                cells[at] = kv[r];
-                d.span_slot[kv[r]] = at;
-                if (D == 2 && at < SPAN_LDS) scells[at] = kv[r];
-            }
-            __syncthreads();
-        }
-    }
-    if (pairs > SPAN_ROUNDS * 1024u) {
-        // many tiles: a second pass over the pairs, bucketing by the counts just taken
-        if (threadIdx.x < 33) lcnt[threadIdx.x] = loff[threadIdx.x];
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < pairs; i += blockDim.x) {
-            if ((i % SPAN_SLOTS) < d.span_n[i / SPAN_SLOTS]) {
-                const uint32_t k = d.span[i];
-                const uint32_t at = atomicAdd(&lcnt[d.span_lev[i]], 1u);
-                cells[at] = k;
-                d.span_slot[k] = at;
-                if (D == 2 && at < SPAN_LDS) scells[at] = k;
-            }
-        }
-        __syncthreads();
-    }
+    __shared__ uint32_t scells[D == 2 ? SPAN_LDS : 1];
+    span_offsets(d, loff);
     const uint32_t total = loff[33];
-    const WalkCell<T, D> *wcs = (const WalkCell<T, D> *)d.wc;
-    if (D == 2 && total <= SPAN_LDS && C < SPAN_REF) {
+    const uint32_t *cells = d.span_cells;
+    if (D == 2 && total <= SPAN_LDS) {
         double pre[SPAN_PER][KIDS][1 + D];
+        uint32_t ref[SPAN_PER][KIDS];
 #pragma unroll
         for (int j = 0; j < SPAN_PER; j++) {
             const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
-#pragma unroll
-            for (int x = 0; x < KIDS; x++)
-#pragma unroll
-                for (int y = 0; y <= D; y++) pre[j][x][y] = 0.0;
-            if (q >= total) continue;
-            const uint32_t k = scells[q], skip = wcs[k].skip;
-            uint32_t c = k + 1;
+            const bool mine = q < total;
+            if (mine) scells[q] = cells[q];
 #pragma unroll
             for (int x = 0; x < KIDS; x++) {
-                uint32_t ref = NO_KID;
-                if (c < skip) {
-                    const uint32_t cs = wcs[c].skip;
-                    const uint32_t t0c = (c / COM_TILE) * COM_TILE;
-                    const uint32_t tend = t0c + COM_TILE < C ? t0c + COM_TILE : C;
-                    // the test k_com_tiles sorts a cell out by
-                    if (d.clevel[c] != 255 && cs > tend) {
-                        const uint32_t slot = d.span_slot[c];
-                        if (slot < total && scells[slot] == c) {
-                            ref = SPAN_REF | slot;
-                        } else {  // inconsistent span lists: never expected
-                            d.st->halt = 1;
-                            ref = NO_KID;
-                        }
-                    } else {
-                        ref = c;
-                        const double *r = d.raw + (size_t)c * (1 + D);
+                ref[j][x] = mine ? d.span_kids[(size_t)q * KIDS + x] : NO_KID;
 #pragma unroll
-                        for (int y = 0; y <= D; y++) pre[j][x][y] = r[y];
+                for (int y = 0; y <= D; y++)
+                    pre[j][x][y] = mine ? d.span_kval[((size_t)q * KIDS + x) * (1 + D) + y] : 0.0;
+            }
+        }
+        __syncthreads();
+        // a span child's cell becomes its slot
+#pragma unroll
+        for (int j = 0; j < SPAN_PER; j++) {
+            const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
+            if (q >= total) continue;
+#pragma unroll
+            for (int x = 0; x < KIDS; x++) {
+                uint32_t r = ref[j][x];
+                if (r != NO_KID && (r & SPAN_REF)) {
+                    const uint32_t c = r & ~SPAN_REF;
+                    const uint32_t slot = d.span_slot[c];
+                    if (slot < total && scells[slot] == c) {
+                        r = SPAN_REF | slot;
+                    } else {  // inconsistent span lists: never expected
+                        d.st->halt = 1;
+                        r = NO_KID;
                     }
-                    c = cs > c ? cs : skip;
                 }
-                kref[q][x] = ref;
+                kref[q][x] = r;
             }
         }
         __syncthreads();
@@ -1074,60 +1075,51 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t ntile
             for (int j = 0; j < SPAN_PER; j++) {
                 const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
                 if (q < a || q >= b) continue;
-                // children in slot order, each added exactly as k_com_tiles adds them
                 double s[1 + D];
 #pragma unroll
                 for (int y = 0; y <= D; y++) s[y] = 0.0;
 #pragma unroll
                 for (int x = 0; x < KIDS; x++) {
-                    const uint32_t ref = kref[q][x];
-                    if (ref == NO_KID) continue;
-                    const bool span = (ref & SPAN_REF) != 0;
-                    const uint32_t sl = span ? (ref & ~SPAN_REF) : 0u;
+                    const uint32_t r = kref[q][x];
+                    if (r == NO_KID) continue;
+                    const bool span = (r & SPAN_REF) != 0;
+                    const uint32_t sl = span ? (r & ~SPAN_REF) : 0u;
 #pragma unroll
                     for (int y = 0; y <= D; y++) s[y] += span ? ssum[sl][y] : pre[j][x][y];
                 }
 #pragma unroll
                 for (int y = 0; y <= D; y++) ssum[q][y] = s[y];
-                body_final<T, D>(d, scells[q], s);
             }
             __syncthreads();
         }
+        // the records of every span cell, all in parallel once the sums are done
+#pragma unroll
+        for (int j = 0; j < SPAN_PER; j++) {
+            const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
+            if (q >= total) continue;
+            double s[1 + D];
+#pragma unroll
+            for (int y = 0; y <= D; y++) s[y] = ssum[q][y];
+            body_final<T, D>(d, scells[q], s);
+        }
         return;
     }
-    {
-        const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
-        for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
-            const uint32_t k = cells[q], skip = wc[k].skip;
-            uint32_t c = k + 1;
-            for (int x = 0; x < KIDS; x++) {
-                uint32_t kid = 0xFFFFFFFFu;
-                if (c < skip) {
-                    kid = c;
-                    const uint32_t nc = wc[c].skip;
-                    c = nc > c ? nc : skip;
-                }
-                d.span_kids[(size_t)q * KIDS + x] = kid;
-            }
-        }
-    }
-    __syncthreads();
     for (int l = 31; l >= 0; l--) {
         const uint32_t a = loff[l], b = loff[l + 1];
         if (a == b) continue;  // block-uniform
         for (uint32_t q = a + threadIdx.x; q < b; q += blockDim.x) {
-            uint32_t kid[KIDS];
-#pragma unroll
-            for (int x = 0; x < KIDS; x++) kid[x] = d.span_kids[(size_t)q * KIDS + x];
             double s[1 + D];
 #pragma unroll
             for (int y = 0; y <= D; y++) s[y] = 0.0;
 #pragma unroll
             for (int x = 0; x < KIDS; x++) {
-                if (kid[x] == 0xFFFFFFFFu) break;
-                const double *r = d.raw + (size_t)kid[x] * (1 + D);
+                const uint32_t r = d.span_kids[(size_t)q * KIDS + x];
+                if (r == NO_KID) continue;
+                const double *v = (r & SPAN_REF)
+                                      ? d.raw + (size_t)(r & ~SPAN_REF) * (1 + D)
+                                      : d.span_kval + ((size_t)q * KIDS + x) * (1 + D);
 #pragma unroll
-                for (int y = 0; y <= D; y++) s[y] += r[y];
+                for (int y = 0; y <= D; y++) s[y] += v[y];
             }
             body_final<T, D>(d, cells[q], s);
         }
@@ -1860,7 +1852,9 @@ static void tree_impl(FaDev &d) {
     hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
     hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
-    hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d, ntiles);
+    hipLaunchKernelGGL((k_span_prep<T, D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
+                       dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d);
 }
 
 template <typename T, int D, bool KAHAN>

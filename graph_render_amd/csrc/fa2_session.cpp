@@ -254,6 +254,9 @@ struct gr_fa2_session {
         if ((s = alloc(&d.span_cells, tiles * 32 * 4))) return s;
         if ((s = alloc(&d.span_kids, tiles * 32 * ((size_t)1 << d.dim) * 4))) return s;
         if ((s = alloc(&d.span_slot, (size_t)cap * 4))) return s;
+        if ((s = alloc(&d.span_kval, tiles * 32 * ((size_t)1 << d.dim) * (1 + d.dim) * 8)))
+            return s;
+        if ((s = alloc(&d.span_lcnt, 128 * 4))) return s;
         if ((s = alloc(&d.raw, (size_t)cap * (1 + d.dim) * 8))) return s;
         if (d.export_bounds && (s = alloc(&d.cbounds, (size_t)cap * 2 * d.dim * 8))) return s;
         return GR_OK;

@@ -74,8 +74,9 @@ struct FaDev {
     uint32_t *span_n = nullptr; // per tile: how many of its span slots are used
     uint8_t *span_lev = nullptr; // the level of each span slot's cell
     uint32_t *span_cells = nullptr; // the span cells bucketed by level (32 slots per tile)
-    uint32_t *span_kids = nullptr;  // per bucketed span cell: its 2^dim child references
-    double *span_kval = nullptr;    // per bucketed span cell: its final children's sums
+    uint32_t span_cap = 0;          // span slots (32 per tile)
+    uint32_t *span_kids = nullptr;  // child references per span slot: by cell, then by slot
+    double *span_kval = nullptr;    // per span slot: its final children's sums (lane-major)
     uint32_t *span_slot = nullptr;  // per cell: its slot in span_cells (span cells only)
     uint32_t *span_lcnt = nullptr;  // span cells per level [0, 33), then bucket fills [33, 66)
     double *raw = nullptr;      // unnormalised (mass, m*x...) per cell

@@ -945,7 +945,7 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
 }
 
 // The span cells the LDS path of k_com_span holds (2D), and child references: a child that
-// is itself a span cell is flagged (by its cell in d.span_kids, by its slot in LDS).
+// is itself a span cell is flagged (named by its cell, then by its slot).
 constexpr uint32_t SPAN_LDS = 3072;
 constexpr int SPAN_PER = (int)(SPAN_LDS / 1024);
 constexpr uint32_t SPAN_REF = 0x80000000u, NO_KID = 0xFFFFFFFFu;
@@ -1007,8 +1007,41 @@ __global__ __launch_bounds__(GR_BLOCK) void k_span_prep(const FaDev d) {
             c = cs > c ? cs : skip;
         }
         d.span_kids[(size_t)at * KIDS + x] = ref;
+        // values lane-major ([child][component][span cell]), read back coalesced
 #pragma unroll
-        for (int y = 0; y <= D; y++) d.span_kval[((size_t)at * KIDS + x) * (1 + D) + y] = v[y];
+        for (int y = 0; y <= D; y++)
+            d.span_kval[(size_t)(x * (1 + D) + y) * d.span_cap + at] = v[y];
+    }
+}
+
+// One thread per bucketed span cell: a child that is itself a span cell is named by its
+// slot from here on. The references are written lane-major ([child][span cell]) into the
+// second half of d.span_kids, where the single-workgroup pass reads them coalesced.
+template <int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_span_link(const FaDev d) {
+    const GrState *st = d.st;
+    if (halted(d)) return;
+    __shared__ uint32_t loff[34];
+    span_offsets(d, loff);
+    const uint32_t total = loff[33];
+    const uint32_t q = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (q >= total) return;
+    constexpr int KIDS = 1 << D;
+    uint32_t *link = d.span_kids + (size_t)d.span_cap * KIDS;
+#pragma unroll
+    for (int x = 0; x < KIDS; x++) {
+        uint32_t r = d.span_kids[(size_t)q * KIDS + x];
+        if (r != NO_KID && (r & SPAN_REF)) {
+            const uint32_t c = r & ~SPAN_REF;
+            const uint32_t slot = d.span_slot[c];
+            if (slot < total && d.span_cells[slot] == c) {
+                r = SPAN_REF | slot;
+            } else {  // inconsistent span lists: never expected
+                d.st->halt = 1;
+                r = NO_KID;
+            }
+        }
+        link[(size_t)x * d.span_cap + q] = r;
     }
 }
 
@@ -1024,11 +1057,11 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
     __shared__ uint32_t loff[34];
     constexpr int KIDS = 1 << D;
     __shared__ double ssum[D == 2 ? SPAN_LDS : 1][1 + D];
-    __shared__ uint32_t kref[D == 2 ? SPAN_LDS : 1][KIDS];
     __shared__ uint32_t scells[D == 2 ? SPAN_LDS : 1];
     span_offsets(d, loff);
     const uint32_t total = loff[33];
     const uint32_t *cells = d.span_cells;
+    const uint32_t *link = d.span_kids + (size_t)d.span_cap * KIDS;
     if (D == 2 && total <= SPAN_LDS) {
         double pre[SPAN_PER][KIDS][1 + D];
         uint32_t ref[SPAN_PER][KIDS];
@@ -1039,32 +1072,11 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
             if (mine) scells[q] = cells[q];
 #pragma unroll
             for (int x = 0; x < KIDS; x++) {
-                ref[j][x] = mine ? d.span_kids[(size_t)q * KIDS + x] : NO_KID;
+                ref[j][x] = mine ? link[(size_t)x * d.span_cap + q] : NO_KID;
 #pragma unroll
                 for (int y = 0; y <= D; y++)
-                    pre[j][x][y] = mine ? d.span_kval[((size_t)q * KIDS + x) * (1 + D) + y] : 0.0;
-            }
-        }
-        __syncthreads();
-        // a span child's cell becomes its slot
-#pragma unroll
-        for (int j = 0; j < SPAN_PER; j++) {
-            const uint32_t q = threadIdx.x + (uint32_t)j * 1024u;
-            if (q >= total) continue;
-#pragma unroll
-            for (int x = 0; x < KIDS; x++) {
-                uint32_t r = ref[j][x];
-                if (r != NO_KID && (r & SPAN_REF)) {
-                    const uint32_t c = r & ~SPAN_REF;
-                    const uint32_t slot = d.span_slot[c];
-                    if (slot < total && scells[slot] == c) {
-                        r = SPAN_REF | slot;
-                    } else {  // inconsistent span lists: never expected
-                        d.st->halt = 1;
-                        r = NO_KID;
-                    }
-                }
-                kref[q][x] = r;
+                    pre[j][x][y] = mine ? d.span_kval[(size_t)(x * (1 + D) + y) * d.span_cap + q]
+                                        : 0.0;
             }
         }
         __syncthreads();
@@ -1080,7 +1092,7 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
                 for (int y = 0; y <= D; y++) s[y] = 0.0;
 #pragma unroll
                 for (int x = 0; x < KIDS; x++) {
-                    const uint32_t r = kref[q][x];
+                    const uint32_t r = ref[j][x];
                     if (r == NO_KID) continue;
                     const bool span = (r & SPAN_REF) != 0;
                     const uint32_t sl = span ? (r & ~SPAN_REF) : 0u;
@@ -1113,13 +1125,17 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
             for (int y = 0; y <= D; y++) s[y] = 0.0;
 #pragma unroll
             for (int x = 0; x < KIDS; x++) {
-                const uint32_t r = d.span_kids[(size_t)q * KIDS + x];
+                const uint32_t r = link[(size_t)x * d.span_cap + q];
                 if (r == NO_KID) continue;
-                const double *v = (r & SPAN_REF)
-                                      ? d.raw + (size_t)(r & ~SPAN_REF) * (1 + D)
-                                      : d.span_kval + ((size_t)q * KIDS + x) * (1 + D);
+                if (r & SPAN_REF) {
+                    const double *v = d.raw + (size_t)cells[r & ~SPAN_REF] * (1 + D);
 #pragma unroll
-                for (int y = 0; y <= D; y++) s[y] += v[y];
+                    for (int y = 0; y <= D; y++) s[y] += v[y];
+                } else {
+#pragma unroll
+                    for (int y = 0; y <= D; y++)
+                        s[y] += d.span_kval[(size_t)(x * (1 + D) + y) * d.span_cap + q];
+                }
             }
             body_final<T, D>(d, cells[q], s);
         }
@@ -1853,6 +1869,8 @@ static void tree_impl(FaDev &d) {
     const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
     hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
     hipLaunchKernelGGL((k_span_prep<T, D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
+                       dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_span_link<D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
                        dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d);
 }

@@ -252,7 +252,8 @@ struct gr_fa2_session {
         if ((s = alloc(&d.span_n, tiles * 4))) return s;
         if ((s = alloc(&d.span_lev, tiles * 32))) return s;
         if ((s = alloc(&d.span_cells, tiles * 32 * 4))) return s;
-        if ((s = alloc(&d.span_kids, tiles * 32 * ((size_t)1 << d.dim) * 4))) return s;
+        d.span_cap = (uint32_t)(tiles * 32);
+        if ((s = alloc(&d.span_kids, 2 * tiles * 32 * ((size_t)1 << d.dim) * 4))) return s;
         if ((s = alloc(&d.span_slot, (size_t)cap * 4))) return s;
         if ((s = alloc(&d.span_kval, tiles * 32 * ((size_t)1 << d.dim) * (1 + d.dim) * 8)))
             return s;

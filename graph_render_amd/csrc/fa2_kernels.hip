@@ -850,24 +850,24 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
     if (halted(d)) return;
     __shared__ double sums[COM_TILE][1 + D];
     __shared__ uint32_t skips[COM_TILE];
-    __shared__ uint8_t lv[COM_TILE];
-    __shared__ uint32_t present, nspan;
+    __shared__ uint16_t bylevel[COM_TILE];  // the tile's internal cells, bucketed by level
+    __shared__ uint32_t lcnt[32], loff[33];
+    __shared__ uint32_t nspan;
     const uint32_t C = st->n_cells;
     const uint32_t t0 = blockIdx.x * COM_TILE;
     if (t0 >= C) return;
     const uint32_t t1 = t0 + COM_TILE < C ? t0 + COM_TILE : C;
     const WalkCell<T, D> *wc = (const WalkCell<T, D> *)d.wc;
-    if (threadIdx.x == 0) {
-        present = 0;
-        nspan = 0;
-    }
+    if (threadIdx.x < 32) lcnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) nspan = 0;
     __syncthreads();
-    uint32_t mask = 0;
     uint32_t *slots = d.span + (size_t)blockIdx.x * SPAN_SLOTS;
     constexpr int PER = COM_TILE / COM_THREADS;
+    enum : uint8_t { K_NONE, K_DONE, K_LEAF, K_INNER };
     // independent loads first (level, skip, leaf range of every cell this thread owns), so
     // their latencies overlap
     uint32_t lvl[PER], skp[PER], qs[PER], qe[PER];
+    uint8_t kind[PER];
 #pragma unroll
     for (int j = 0; j < PER; j++) {
         const uint32_t kk = t0 + threadIdx.x + j * COM_THREADS;
@@ -879,23 +879,25 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
 #pragma unroll
     for (int j = 0; j < PER; j++) {
         const uint32_t kk = t0 + threadIdx.x + j * COM_THREADS;
+        kind[j] = K_NONE;
         if (kk >= t1) continue;
         const uint32_t i = kk - t0;
-        uint32_t l = lvl[j];
+        const uint32_t l = lvl[j];
         const uint32_t skip = skp[j];
         skips[i] = skip;
         if (l == 255) {  // summed by the deep builder
             const double *r = d.raw + (size_t)kk * (1 + D);
 #pragma unroll
             for (int a = 0; a <= D; a++) sums[i][a] = r[a];
-        } else if (skip > t1) {
+            kind[j] = K_DONE;
+        } else if (skip > t1) {  // crosses the tile: a span cell
             uint32_t at = atomicAdd(&nspan, 1u);  // LDS
             if (at < SPAN_SLOTS) {
                 slots[at] = kk;
                 d.span_lev[(size_t)blockIdx.x * SPAN_SLOTS + at] = (uint8_t)l;
                 atomicAdd(&d.span_lcnt[l], 1u);
             }
-            l = 255;
+            kind[j] = K_DONE;
         } else if (skip == kk + 1) {
             double s[1 + D];
 #pragma unroll
@@ -909,38 +911,62 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
             }
 #pragma unroll
             for (int a = 0; a <= D; a++) sums[i][a] = s[a];
-            body_final<T, D>(d, kk, s);
-            l = 255;
+            kind[j] = K_LEAF;
+        } else if (l < 32) {
+            atomicAdd(&lcnt[l], 1u);
+            kind[j] = K_INNER;
         }
-        lv[i] = (uint8_t)l;
-        if (l < 32) mask |= 1u << l;
     }
-    atomicOr(&present, mask);
     __syncthreads();
-    if (threadIdx.x == 0) d.span_n[blockIdx.x] = nspan < SPAN_SLOTS ? nspan : SPAN_SLOTS;
-    const uint32_t levels = present;
+    if (threadIdx.x == 0) {
+        d.span_n[blockIdx.x] = nspan < SPAN_SLOTS ? nspan : SPAN_SLOTS;
+        uint32_t acc = 0;
+        for (int l = 0; l < 32; l++) {
+            loff[l] = acc;
+            acc += lcnt[l];
+        }
+        loff[32] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) lcnt[threadIdx.x] = loff[threadIdx.x];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+        if (kind[j] == K_INNER)
+            bylevel[atomicAdd(&lcnt[lvl[j]], 1u)] = (uint16_t)(threadIdx.x + j * COM_THREADS);
+    __syncthreads();
+    // deepest level first: a cell sums its children in slot order (k+1, then each skip)
     for (int l = 31; l >= 0; l--) {
-        if (!(levels & (1u << l))) continue;  // block-uniform
-        for (uint32_t k = t0 + threadIdx.x; k < t1; k += COM_THREADS) {
-            const uint32_t i = k - t0;
-            if (lv[i] != (uint32_t)l) continue;
+        const uint32_t a = loff[l], b = loff[l + 1];
+        if (a == b) continue;  // block-uniform
+        for (uint32_t x = a + threadIdx.x; x < b; x += COM_THREADS) {
+            const uint32_t i = bylevel[x], k = t0 + i;
             double s[1 + D];
 #pragma unroll
-            for (int a = 0; a <= D; a++) s[a] = 0.0;
+            for (int y = 0; y <= D; y++) s[y] = 0.0;
             const uint32_t skip = skips[i];
             uint32_t c = k + 1;
             for (int q = 0; q < (1 << D) && c < skip; q++) {
                 const uint32_t ci = c - t0;
 #pragma unroll
-                for (int a = 0; a <= D; a++) s[a] += sums[ci][a];
+                for (int y = 0; y <= D; y++) s[y] += sums[ci][y];
                 const uint32_t nc = skips[ci];
                 c = nc > c ? nc : skip;
             }
 #pragma unroll
-            for (int a = 0; a <= D; a++) sums[i][a] = s[a];
-            body_final<T, D>(d, k, s);
+            for (int y = 0; y <= D; y++) sums[i][y] = s[y];
         }
         __syncthreads();
+    }
+    // the records of the tile's leaves and internal cells, all at once
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        if (kind[j] != K_LEAF && kind[j] != K_INNER) continue;
+        const uint32_t i = threadIdx.x + j * COM_THREADS;
+        double s[1 + D];
+#pragma unroll
+        for (int y = 0; y <= D; y++) s[y] = sums[i][y];
+        body_final<T, D>(d, t0 + i, s);
     }
 }
 

@@ -405,7 +405,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_walk(const FaDev d, const AdjD
             }
             T d2 = dx[0] * dx[0] + dx[1] * dx[1];
             if (D == 3) d2 = d2 + dx[D - 1] * dx[D - 1];
-            if (!holds && d2 > (T)0 && cell.w2 < theta2 * d2) {
+            if (!holds && d2 > (T)0 && cell.w2 < open_rhs(d2, theta2)) {
                 const T inv = recip(d2);
                 T jv[D], ru = (T)0;
                 jmul<T, D>(dx, ui, inv, jv);

@@ -26,7 +26,7 @@ template <typename T, int D>
 struct WalkCell : CentreLo<T, D> {
     T c[D];
     T mass;
-    T w2;
+    T w2;  // opening threshold: w² (FP64) or w²/θ² (FP32), see open_threshold
     uint32_t skip, start, end;
     // dword index of `mass` in the record (the lo centre, if any, then the centre)
     static constexpr uint32_t MASS_DWORD =
@@ -94,6 +94,26 @@ __device__ __forceinline__ T mul_add(T a, T b, T c) {
 // The hardware reciprocal (FP32 only).
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
+// The walk record's opening threshold (its `w2` field). FP64 keeps the reference's w² and
+// tests w² < θ²·d² (barnes_hut.rs); FP32 keeps w²/θ², formed once per cell, and tests it
+// against d² (θ = 0 gives an infinite threshold: nothing is accepted, as in FP64).
+template <typename T>
+__device__ __forceinline__ T open_threshold(double w, double theta2) {
+    if constexpr (std::is_same<T, double>::value) {
+        return w * w;
+    } else {
+        return (T)(w * w / theta2);
+    }
+}
+template <typename T>
+__device__ __forceinline__ T open_rhs(T d2, T theta2) {
+    if constexpr (std::is_same<T, double>::value) {
+        return theta2 * d2;
+    } else {
+        return d2;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T pair_force(T mm, T d2, T kr) {

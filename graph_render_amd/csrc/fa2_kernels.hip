@@ -711,7 +711,7 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
                                           uint32_t level) {
     WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
     double w = b.hi[0] - b.lo[0];
-    wc[k].w2 = (T)(w * w);
+    wc[k].w2 = open_threshold<T>(w, d.theta2);
     wc[k].skip = skip;
     wc[k].start = start;
     wc[k].end = end;
@@ -1279,7 +1279,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         T d2 = mul_add(dx[0], dx[0], dx[1] * dx[1]);
         if (D == 3) d2 = mul_add(dx[D - 1], dx[D - 1], d2);
         // barnes_hut.rs's d2 > 0 is implied: w2 >= 0, so w2 < theta2 * d2 needs d2 > 0
-        const uint64_t accepted = acting & ~holding & wave_lt(cell.w2, theta2 * d2);
+        const uint64_t accepted = acting & ~holding & wave_lt(cell.w2, open_rhs(d2, theta2));
         const bool accept = __builtin_amdgcn_inverse_ballot_w64(accepted);
         const bool act = __builtin_amdgcn_inverse_ballot_w64(acting);
         if (accepted != 0) {

@@ -696,25 +696,51 @@ __device__ __forceinline__ void body_final(const FaDev &d, uint32_t k, const dou
     double *r = d.raw + (size_t)k * (1 + D);
 #pragma unroll
     for (int a = 0; a <= D; a++) r[a] = s[a];
-    wc[k].mass = (T)s[0];
+    // the record's head (the lo centre in FP32, the centre, the mass) as one store
+    constexpr bool LO = !std::is_same<T, double>::value;
+    struct __attribute__((packed, aligned(4))) Head {
+        T lo[LO ? D : 1];
+        T c[D];
+        T mass;
+    };
+    Head h;
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const double c = s[1 + a] / s[0];
-        wc[k].c[a] = (T)c;
-        if constexpr (!std::is_same<T, double>::value) wc[k].clo[a] = (T)(c - (double)(T)c);
+        h.c[a] = (T)c;
+        if constexpr (LO) h.lo[a] = (T)(c - (double)(T)c);
+    }
+    h.mass = (T)s[0];
+    if constexpr (LO) {
+        static_assert(sizeof(Head) == sizeof(T) * (2 * D + 1), "head layout");
+        *(Head *)&wc[k] = h;
+    } else {
+        struct __attribute__((packed, aligned(4))) Head64 {
+            T c[D];
+            T mass;
+        };
+        Head64 h64;
+#pragma unroll
+        for (int a = 0; a < D; a++) h64.c[a] = h.c[a];
+        h64.mass = h.mass;
+        *(Head64 *)&wc[k] = h64;
     }
 }
 
 template <typename T, int D>
 __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<D> &b,
                                           uint32_t skip, uint32_t start, uint32_t end,
-                                          uint32_t level) {
+                                          uint32_t level, double w) {
     WalkCell<T, D> *wc = (WalkCell<T, D> *)d.wc;
-    double w = b.hi[0] - b.lo[0];
-    wc[k].w2 = open_threshold<T>(w, d.theta2);
-    wc[k].skip = skip;
-    wc[k].start = start;
-    wc[k].end = end;
+    // the record's structural tail (w2, skip, start, end) as one store: four separate
+    // stores into 36-byte records cost a write transaction each
+    struct __attribute__((packed, aligned(4))) Tail {
+        T w2;
+        uint32_t skip, start, end;
+    };
+    static_assert(sizeof(Tail) == sizeof(T) + 12, "tail layout");
+    const Tail tail = {open_threshold<T>(w, d.theta2), skip, start, end};
+    *(Tail *)&wc[k].w2 = tail;
     d.clevel[k] = (uint8_t)level;
     if (skip == k + 1) {  // a leaf: its positions point back to it (the walk's own-cell test)
         const uint32_t e = end < d.n ? end : d.n;
@@ -728,6 +754,25 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
             o[D + a] = b.hi[a];
         }
     }
+}
+
+// True when every midpoint of a descent from this root to `deepest` levels is exact: a
+// square root of power-of-two side w0 whose corners stay below w0 * 2^(50 - deepest) in
+// magnitude, so each level's corners are multiples of its width inside 52 bits. Then the
+// cell at level l is exactly w0 / 2^l wide, the width the reference's narrowing gives.
+template <int D>
+__device__ __forceinline__ bool dyadic_root(const Box<D> &b, int deepest) {
+    const double w0 = b.hi[0] - b.lo[0];
+    int ex = 0;
+    if (!(w0 > 0.0) || !isfinite(w0) || frexp(w0, &ex) != 0.5) return false;
+    const double lim = ldexp(w0, 50 - deepest);
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        if (b.hi[a] - b.lo[a] != w0) ok = false;
+        if (!(fabs(b.lo[a]) < lim && fabs(b.hi[a]) < lim)) ok = false;
+    }
+    return ok;
 }
 
 // Writes group g's cells in preorder (preorder.rs:14-99 layout: skip, start, end per cell).
@@ -754,9 +799,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     const uint32_t gb = graph_of(d, d.order[start]);
     Box<D> b = root_box<D>(d.gr[gb]);
     int top = type == G_DEEP ? L - 1 : level;
+    // only the widths are needed unless the bounds are exported; on a dyadic root they
+    // follow from the level alone and the descent is skipped
+    const bool descend = d.export_bounds || !dyadic_root<D>(b, top);
+    const double w0 = b.hi[0] - b.lo[0];
     for (int l = 0; l <= top; l++) {
-        if (l > 0) box_quad<D>(b, key_slot<D>(key, l, L));
+        if (l > 0 && descend) box_quad<D>(b, key_slot<D>(key, l, L));
         if (l <= left) continue;
+        const double w = descend ? b.hi[0] - b.lo[0] : ldexp(w0, -l);
         uint32_t skip;
         if (l <= last_internal) {
             uint32_t end = gend;
@@ -766,10 +816,10 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
                 skip = h + 1 < G ? d.gbase[h + 1] : C;
                 end = h + 1 < G ? d.gstart[h + 1] : nf;
             }
-            emit_cell<T, D>(d, idx, b, skip, start, end, (uint32_t)l);
+            emit_cell<T, D>(d, idx, b, skip, start, end, (uint32_t)l, w);
         } else {
             skip = idx + 1;
-            emit_cell<T, D>(d, idx, b, skip, start, gend, (uint32_t)l);
+            emit_cell<T, D>(d, idx, b, skip, start, gend, (uint32_t)l, w);
         }
         if (l == 0) {  // the graph's root: its tree is cells [idx, skip)
             d.gr[gb].cell_lo = idx;
@@ -791,7 +841,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
                 lb.hi[a] = bk[D + a];
             }
             emit_cell<T, D>(d, base + c, lb, base + (uint32_t)ik[0], (uint32_t)ik[1],
-                            (uint32_t)ik[2], 255u);
+                            (uint32_t)ik[2], 255u, lb.hi[0] - lb.lo[0]);
             body_final<T, D>(d, base + c, d.lc_sum + (size_t)(lc + c) * (1 + D));
         }
     }

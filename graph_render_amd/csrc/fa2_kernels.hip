@@ -670,22 +670,50 @@ __global__ void k_cells_ready(const FaDev d) {
 }
 
 
-// The last group h >= g whose key shares `level` levels with group g's.
+// The last group h in [lo, hi) whose key shares `level` levels with `key`, given that group
+// lo does and group hi (when hi < G) does not. Four probes per round, issued together, so
+// a round trip narrows the interval fivefold; with no bound yet (hi == G) the bracket is
+// found first by probing at lo + 4, 16, 64, 256. The chain of dependent loads matters more
+// than their count: the first group of a long run searches it for every level it opens.
 template <int D>
-__device__ uint32_t gallop_fwd(const uint64_t *gkey, uint32_t G, uint32_t g, int level, int L) {
-    uint64_t key = gkey[g];
-    uint32_t lo = g, step = 1;
-    while (lo + step < G && key_same<D>(key, gkey[lo + step], level, L)) {
-        lo += step;
-        step <<= 1;
+__device__ uint32_t run_end(const uint64_t *gkey, uint32_t G, uint64_t key, uint32_t lo,
+                            uint32_t hi, int level, int L) {
+    constexpr int P = 4;
+    if (hi >= G) {
+        hi = G;
+        for (uint32_t shift = 2; hi == G && lo + 1 < G; shift += 8) {
+            uint32_t pr[P];
+#pragma unroll
+            for (int u = 0; u < P; u++) {
+                const uint64_t at = (uint64_t)lo + (1ull << (shift + 2 * u < 62 ? shift + 2 * u : 62));
+                pr[u] = at < G ? (uint32_t)at : G - 1;
+            }
+            bool m[P];
+#pragma unroll
+            for (int u = 0; u < P; u++) m[u] = key_same<D>(key, gkey[pr[u]], level, L);
+#pragma unroll
+            for (int u = 0; u < P; u++) {
+                if (m[u]) lo = pr[u] > lo ? pr[u] : lo;
+                else hi = pr[u] < hi ? pr[u] : hi;
+            }
+        }
     }
-    uint32_t hi = lo + step < G ? lo + step : G;  // first index known (or assumed) not to match
     while (hi - lo > 1) {
-        uint32_t mid = lo + (hi - lo) / 2;
-        if (key_same<D>(key, gkey[mid], level, L))
-            lo = mid;
-        else
-            hi = mid;
+        const uint64_t span = hi - lo;
+        uint32_t pr[P];
+#pragma unroll
+        for (int u = 0; u < P; u++) pr[u] = lo + (uint32_t)(span * (uint64_t)(u + 1) / (P + 1));
+        bool m[P];
+#pragma unroll
+        for (int u = 0; u < P; u++) m[u] = key_same<D>(key, gkey[pr[u]], level, L);
+        uint32_t nlo = lo, nhi = hi;
+#pragma unroll
+        for (int u = 0; u < P; u++) {
+            if (m[u]) nlo = pr[u] > nlo ? pr[u] : nlo;
+            else nhi = pr[u] < nhi ? pr[u] : nhi;
+        }
+        lo = nlo;
+        hi = nhi;
     }
     return lo;
 }
@@ -803,6 +831,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     // follow from the level alone and the descent is skipped
     const bool descend = d.export_bounds || !dyadic_root<D>(b, top);
     const double w0 = b.hi[0] - b.lo[0];
+    uint32_t run_hi = G;  // groups from here on share no more levels than the last one searched
     for (int l = 0; l <= top; l++) {
         if (l > 0 && descend) box_quad<D>(b, key_slot<D>(key, l, L));
         if (l <= left) continue;
@@ -812,7 +841,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
             uint32_t end = gend;
             skip = gskip;
             if (l <= right) {
-                uint32_t h = gallop_fwd<D>(gkey, G, g, l, L);
+                // runs nest: this level's run ends no later than the last level's did
+                const uint32_t h = run_end<D>(gkey, G, key, g, run_hi, l, L);
+                run_hi = h + 1;
                 skip = h + 1 < G ? d.gbase[h + 1] : C;
                 end = h + 1 < G ? d.gstart[h + 1] : nf;
             }

@@ -671,17 +671,17 @@ __global__ void k_cells_ready(const FaDev d) {
 
 
 // The last group h in [lo, hi) whose key shares `level` levels with `key`, given that group
-// lo does and group hi (when hi < G) does not. Four probes per round, issued together, so
-// a round trip narrows the interval fivefold; with no bound yet (hi == G) the bracket is
-// found first by probing at lo + 4, 16, 64, 256. The chain of dependent loads matters more
-// than their count: the first group of a long run searches it for every level it opens.
+// lo does and group hi (when hi < G) does not. Two probes per round, issued together, so a
+// round trip narrows the interval threefold; with no bound yet (hi == G) the bracket is
+// found first by probing at lo + 4, 16, then 64, 256, ... (two probes per round measured
+// faster than one, four or eight at 1M groups).
 template <int D>
 __device__ uint32_t run_end(const uint64_t *gkey, uint32_t G, uint64_t key, uint32_t lo,
                             uint32_t hi, int level, int L) {
-    constexpr int P = 4;
+    constexpr int P = 2;
     if (hi >= G) {
         hi = G;
-        for (uint32_t shift = 2; hi == G && lo + 1 < G; shift += 8) {
+        for (uint32_t shift = 2; hi == G && lo + 1 < G; shift += 4) {
             uint32_t pr[P];
 #pragma unroll
             for (int u = 0; u < P; u++) {

@@ -271,6 +271,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head_graph(const FaDev d) {
         }
         for (int a = 0; a < D; a++) g->mean[a] = vsum[a] / nb;
         if (nfin > 0) cover_root<D>(g, vmin, vmax);
+        if (d.ngraphs == 1) {  // the session totals are this graph's (k_head_global's job)
+            GrState *st = d.st;
+            st->pool_used = 0;
+            st->lcell_used = 0;
+            st->span_count = 0;
+            st->any_deep = 0;
+            st->n_finite = nfin;
+            st->live = g->done ? 0u : 1u;
+        }
     }
 }
 
@@ -308,6 +317,45 @@ __global__ __launch_bounds__(GR_BLOCK) void k_head_global(const FaDev d) {
 // levels, so its key bits are exactly the slots d3's insertion would take. A level whose
 // narrowing no longer shrinks the x-span is where the reference stops splitting and chains
 // instead (build.rs:76-85); the key stops there and the level is kept as the point's stall.
+// True when every midpoint of a descent from this root to `deepest` levels is exact: a
+// square root of power-of-two side w0 whose corners stay below w0 * 2^(50 - deepest) in
+// magnitude, so each level's corners are multiples of its width inside 52 bits. Then the
+// cell at level l is exactly w0 / 2^l wide, the width the reference's narrowing gives.
+template <int D>
+__device__ __forceinline__ bool dyadic_root(const Box<D> &b, int deepest) {
+    const double w0 = b.hi[0] - b.lo[0];
+    int ex = 0;
+    if (!(w0 > 0.0) || !isfinite(w0) || frexp(w0, &ex) != 0.5) return false;
+    const double lim = ldexp(w0, 50 - deepest);
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        if (b.hi[a] - b.lo[a] != w0) ok = false;
+        if (!(fabs(b.lo[a]) < lim && fabs(b.hi[a]) < lim)) ok = false;
+    }
+    return ok;
+}
+
+// Bit i of x to bit 2i (x < 2^32), and to bit 3i (x < 2^21): the Morton interleave.
+__device__ __forceinline__ uint64_t spread2(uint64_t x) {
+    x &= 0xFFFFFFFFull;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+    x &= 0x1FFFFFull;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
     if (halted(d)) return;
@@ -327,15 +375,36 @@ __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
         const uint32_t gb = graph_of(d, i);
         key = (uint64_t)gb << (D * L);
         Box<D> b = root_box<D>(d.gr[gb]);
-        for (int l = 1; l <= L; l++) {
-            double before = b.hi[0] - b.lo[0];
-            int slot = box_narrow<D>(b, p);
-            double after = b.hi[0] - b.lo[0];
-            if (!(after < before)) {
-                stall = (uint32_t)(l - 1);
-                break;
+        if (dyadic_root<D>(b, L)) {
+            // Every midpoint of the descent is exact, so no level stalls and the slot bits
+            // of level l are bit L - l of each axis' cell index q = max{q : lo + q w_L <= p}:
+            // one product gives q to within one, exact grid comparisons settle it.
+            const double w0 = b.hi[0] - b.lo[0];
+            const double wl = ldexp(w0, -L), inv = ldexp(1.0, L) / w0;
+            const int64_t top = ((int64_t)1 << L) - 1;
+            uint64_t mort = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                int64_t q = (int64_t)floor((p[a] - b.lo[a]) * inv);
+                q = q < 0 ? 0 : (q > top ? top : q);
+                if (b.lo[a] + (double)q * wl > p[a])
+                    q--;
+                else if (q < top && b.lo[a] + (double)(q + 1) * wl <= p[a])
+                    q++;
+                mort |= (D == 2 ? spread2((uint64_t)q) : spread3((uint64_t)q)) << a;
             }
-            key |= (uint64_t)slot << (D * (L - l));
+            key |= mort;
+        } else {
+            for (int l = 1; l <= L; l++) {
+                double before = b.hi[0] - b.lo[0];
+                int slot = box_narrow<D>(b, p);
+                double after = b.hi[0] - b.lo[0];
+                if (!(after < before)) {
+                    stall = (uint32_t)(l - 1);
+                    break;
+                }
+                key |= (uint64_t)slot << (D * (L - l));
+            }
         }
     }
     // Reverse input order: the stable sort then leaves equal keys in descending point
@@ -782,25 +851,6 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
             o[D + a] = b.hi[a];
         }
     }
-}
-
-// True when every midpoint of a descent from this root to `deepest` levels is exact: a
-// square root of power-of-two side w0 whose corners stay below w0 * 2^(50 - deepest) in
-// magnitude, so each level's corners are multiples of its width inside 52 bits. Then the
-// cell at level l is exactly w0 / 2^l wide, the width the reference's narrowing gives.
-template <int D>
-__device__ __forceinline__ bool dyadic_root(const Box<D> &b, int deepest) {
-    const double w0 = b.hi[0] - b.lo[0];
-    int ex = 0;
-    if (!(w0 > 0.0) || !isfinite(w0) || frexp(w0, &ex) != 0.5) return false;
-    const double lim = ldexp(w0, 50 - deepest);
-    bool ok = true;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        if (b.hi[a] - b.lo[a] != w0) ok = false;
-        if (!(fabs(b.lo[a]) < lim && fabs(b.hi[a]) < lim)) ok = false;
-    }
-    return ok;
 }
 
 // Writes group g's cells in preorder (preorder.rs:14-99 layout: skip, start, end per cell).
@@ -1947,7 +1997,8 @@ __global__ void k_clear_halt(const FaDev d) {
 template <int D>
 static void head_impl(FaDev &d) {
     hipLaunchKernelGGL((k_head_graph<D>), dim3(d.ngraphs), dim3(GR_BLOCK), 0, d.stream, d);
-    hipLaunchKernelGGL(k_head_global, dim3(1), dim3(GR_BLOCK), 0, d.stream, d);
+    if (d.ngraphs > 1)
+        hipLaunchKernelGGL(k_head_global, dim3(1), dim3(GR_BLOCK), 0, d.stream, d);
 }
 
 template <typename T, int D>

@@ -63,6 +63,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_load(const FaDev d, const T *p
         const T x = planar[(size_t)a * d.n + v];
         ((T *)d.pos[a])[v] = x;
         ((T *)d.pi)[(size_t)v * D + a] = x;
+        ((T *)d.pk)[(size_t)v * 4 + a] = x;
     }
 }
 

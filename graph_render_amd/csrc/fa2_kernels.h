@@ -32,6 +32,7 @@ struct FaDev {
     // state, element type T (double or float) chosen by precision
     void *pos[3] = {};
     void *pi = nullptr;  // positions again, interleaved (x, y[, z]) per node, for gathers
+    void *pk = nullptr;  // (x, y, [z,] mass) per node in 4 slots: one line for the tree gather
     void *mass = nullptr;
     void *u[3] = {};
     void *rep[3] = {};

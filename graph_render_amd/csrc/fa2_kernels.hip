@@ -195,7 +195,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_bounds(const FaDev d) {
         for (int a = 0; a < D; a++) {
             p[a] = (double)P[a][i];
             ((T *)d.pi)[(size_t)i * D + a] = P[a][i];
+            ((T *)d.pk)[(size_t)i * 4 + a] = P[a][i];
         }
+        ((T *)d.pk)[(size_t)i * 4 + D] = ((const T *)d.mass)[i];
         bounds_take<D>(p, lo, hi, sum, nfin);
     }
     bounds_write<D>(lo, hi, sum, nfin, d.bpart);
@@ -430,10 +432,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_gather(const FaDev d) {
     uint32_t i = d.vals[src];
     uint64_t key = d.keys[src];
     d.order[p] = i;
-    const T *pi = (const T *)d.pi;
+    // position and mass of point i from one 4-slot record (one cache line)
+    const T *pk = (const T *)d.pk + (size_t)i * 4;
+    T rec[4];
 #pragma unroll
-    for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = pi[(size_t)i * D + a];
-    ((T *)d.sm)[p] = ((const T *)d.mass)[i];
+    for (int a = 0; a < 4; a++) rec[a] = pk[a];
+#pragma unroll
+    for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = rec[a];
+    ((T *)d.sm)[p] = rec[D];
     d.flag[p] = (p < nf && (p == 0 || d.keys[p - 1] != key)) ? 1u : 0u;
 }
 
@@ -1937,6 +1943,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
             T np = (T)((double)P[v] + dd);
             P[v] = np;
             ((T *)d.pi)[(size_t)v * D + a] = np;
+            ((T *)d.pk)[(size_t)v * 4 + a] = np;
             p[a] = (double)np;
             step += fabs(dd);
         }

@@ -554,6 +554,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
         TRY_ALLOC(d.sp[a], nn * es);
     }
     TRY_ALLOC(d.pi, nn * d.dim * es);
+    TRY_ALLOC(d.pk, nn * 4 * es);
     TRY_ALLOC(d.mass, nn * es);
     TRY_ALLOC(d.sm, nn * es);
     TRY_ALLOC(d.row_off, (nn + 1) * 4);

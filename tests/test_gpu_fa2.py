@@ -76,6 +76,27 @@ def test_tree_on_a_layout_is_bit_exact(oracle, n):
     assert np.array_equal(tr["bodies"], rt.bodies(ref.mass()))
 
 
+@pytest.mark.parametrize("offset", [0.0, 3.0e9, 1.0e13])
+def test_cell_widths_and_keys_on_and_off_dyadic_roots(oracle, offset):
+    # Without exported bounds a session takes cell widths from the level and keys from one
+    # product per axis when the root square is dyadic (offsets 0 and 3e9); at 1e13 the root
+    # is too far out for that and both fall back to the reference's descent. Either way the
+    # repulsion must be the oracle's, bit for bit, with and without exported bounds.
+    n = 5000
+    s, t = bh_graph(n)
+    rng = np.random.default_rng(7)
+    pts = rng.normal(0.0, 30.0, (n, 2)) + offset
+    print(f"dyadic case offset={offset}", flush=True)
+    with session(n, s, t, start=pts, export=False) as a:
+        fa = a.forces(2)
+    with session(n, s, t, start=pts, export=True) as b:
+        fb = b.forces(2)
+    ref = oracle_state(oracle, n, s, t, start=pts)
+    fr = ref.forces(2)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(fa, fr)
+
+
 # ------------------------------------------------------------------ forces
 
 

@@ -1430,18 +1430,18 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
             } else {
                 cmass = __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)cur, (int)MD));
             }
-            // FP64 forms the reference's mi * mc / d2 * kr; FP32 takes (mi * kr) * mc / d2
-            const T ff = std::is_same<T, double>::value ? pair_force<T>(mi * cmass, d2, kr)
-                                                        : mikr * cmass * fast_rcp(d2);
+            // only the accepting lanes run the force (an exec mask, not per-lane selects)
+            if (accept) {
+                // FP64 forms the reference's mi * mc / d2 * kr; FP32 takes (mi * kr) * mc / d2
+                const T ff = std::is_same<T, double>::value ? pair_force<T>(mi * cmass, d2, kr)
+                                                            : mikr * cmass * fast_rcp(d2);
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-                if constexpr (KAHAN) {
-                    T s = f[a], c = fc[a];
-                    acc_add<T, KAHAN>(s, c, dx[a] * ff);
-                    f[a] = accept ? s : f[a];
-                    fc[a] = accept ? c : fc[a];
-                } else {
-                    f[a] = accept ? mul_add(dx[a], ff, f[a]) : f[a];
+                for (int a = 0; a < D; a++) {
+                    if constexpr (KAHAN) {
+                        acc_add<T, KAHAN>(f[a], fc[a], dx[a] * ff);
+                    } else {
+                        f[a] = mul_add(dx[a], ff, f[a]);
+                    }
                 }
             }
         }

@@ -16,6 +16,14 @@ enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_KAHAN = 2 };
 // Rows with more CSR entries than this are summed by k_long_rows (one workgroup per row).
 constexpr uint32_t LONG_ROW = 128;
 
+// CSR entries whose neighbour positions (and weights) one k_finish round stages in LDS:
+// 24 KiB of dynamic LDS per workgroup. A workgroup of k_finish is also given about this
+// many rows + entries, so that its rows take one round.
+constexpr uint32_t FIN_LDS = 24576;
+constexpr uint32_t fin_stage(uint32_t elem_bytes, uint32_t dim, bool weighted) {
+    return FIN_LDS / (elem_bytes * (dim + (weighted ? 1u : 0u)));
+}
+
 // Force parts of one evaluation (gr_fa2_session_forces): the reference's iterate() order.
 enum Parts : uint32_t { PART_ATTRACTION = 1, PART_REPULSION = 2, PART_GRAVITY = 4 };
 
@@ -93,7 +101,8 @@ struct FaDev {
     int32_t *chain_next = nullptr;
 
     // reductions
-    uint32_t nparts = 0;
+    uint32_t nparts = 0;          // workgroups of k_finish, a multiple of the XCD count
+    uint32_t *part_rows = nullptr;  // k_finish workgroup b: rows [part_rows[b], part_rows[b+1])
     double *bpart = nullptr;  // bounds/mean partials
     double *spart = nullptr;  // swing/traction partials
     double *mpart = nullptr;  // moved partials

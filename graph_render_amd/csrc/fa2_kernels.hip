@@ -1679,115 +1679,127 @@ __global__ __launch_bounds__(GR_BLOCK) void k_long_rows(const FaDev d) {
     }
 }
 
-constexpr uint32_t FIN_STAGE = 4096;  // CSR entries staged in LDS per round
-
 // Completes u (attraction + repulsion, then gravity, state.rs:130-148 order) and the
 // swing/traction partials (schedule.rs:16-30). mode 0: repulsion from the walk (u = attr +
 // rep); mode 1: u already holds attraction (+ dense pairs).
 //
 // Work split: block b owns the rows whose (first entry + row index) falls in its share of
-// E + n, so rows and entries are balanced together; blocks are dealt so that one XCD owns one contiguous
-// range of rows (graph-local neighbours then share its L2), and each round stages up to
-// FIN_STAGE consecutive CSR entries in LDS with coalesced loads. A row is still summed by
-// one lane in ascending entry order, the reference's order.
+// E + n, so rows and entries are balanced together; blocks are dealt so that one XCD owns
+// one contiguous range of rows (graph-local neighbours then share its L2). Each round
+// gathers the neighbour positions of up to fin_stage() consecutive CSR entries into LDS,
+// one entry per lane with many gathers in flight (coalesced index loads, no divergence);
+// a row is then summed by one lane from LDS in ascending entry order, the reference's
+// order. Entries of rows longer than LONG_ROW are not staged (k_long_rows sums them).
+// The row split (part_rows) is fixed by the CSR and computed once per session; each
+// workgroup gets about one round's worth of rows + entries. Dynamic LDS: FIN_LDS bytes.
 template <typename T, int D, bool KAHAN>
 __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mode,
                                                       uint32_t parts) {
     const GrState *st = d.st;
     if (halted(d)) return;
-    __shared__ uint32_t s_nbr[FIN_STAGE];
-    __shared__ T s_w[FIN_STAGE];
-    __shared__ uint32_t s_rows[2];
-    const uint32_t nb = gridDim.x;
-    const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
-    const uint64_t E = d.m2;
-    const uint32_t n = d.n;
-    if (threadIdx.x < 2) {
-        // balance rows + entries: the first row r with row_off[r] + r at or past the cut
-        const uint64_t cut = ((E + n) * (uint64_t)(lb + threadIdx.x)) / nb;
-        uint32_t lo = 0, hi = n;
-        if (lb + threadIdx.x >= nb) {
-            lo = n;
-        } else {
-            while (lo < hi) {
-                uint32_t mid = lo + (hi - lo) / 2;
-                if ((uint64_t)d.row_off[mid] + mid < cut)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-        }
-        s_rows[threadIdx.x] = lo;
-    }
-    __syncthreads();
-    // rows [r0, r1): r0 = first row starting at or after this block's cut (row 0 for the
-    // first block), r1 the same for the next block
-    const uint32_t r0 = lb == 0 ? 0 : s_rows[0];
-    const uint32_t r1 = lb + 1 >= nb ? n : s_rows[1];
-    double sw = 0.0, tr = 0.0;
-    const double grav = d.gravity;
     const T *pi = (const T *)d.pi;
     const T *wgt = (const T *)d.w;
+    const uint32_t STAGE = fin_stage(sizeof(T), D, wgt != nullptr);
+    extern __shared__ __align__(16) unsigned char fin_lds[];
+    T *s_pj = (T *)fin_lds;     // [STAGE][D]
+    T *s_w = s_pj + STAGE * D;  // [STAGE], weighted graphs only
+    __shared__ uint32_t s_lr_lo[GR_BLOCK], s_lr_hi[GR_BLOCK], s_wl[GR_BLOCK / GR_WAVE];
+    const uint32_t lb = gr_xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t r0 = d.part_rows[lb], r1 = d.part_rows[lb + 1];
+    const double grav = d.gravity;
     for (uint32_t base = r0; base < r1; base += GR_BLOCK) {
         const uint32_t rend = base + GR_BLOCK < r1 ? base + GR_BLOCK : r1;
         const uint32_t v = base + threadIdx.x;
         const bool live = v < rend;
-        T pv[D], uv[D], cc[D];
+        // the row's inputs besides its neighbours, all loaded before the attraction rounds:
+        // mode 0 adds the walk's repulsion, mode 1 takes u as it stands
+        const bool addrep = mode == 0 && (parts & PART_REPULSION);
+        T pv[D], uv[D], cc[D], rv[D];
 #pragma unroll
         for (int a = 0; a < D; a++) {
             pv[a] = live ? pi[(size_t)v * D + a] : (T)0;
+            const T *other = (const T *)(mode == 0 ? d.rep[a] : d.u[a]);
+            rv[a] = live && (addrep || mode != 0) ? other[v] : (T)0;
             uv[a] = cc[a] = (T)0;
         }
+        const T mt = live ? ((const T *)d.mass)[v] : (T)0;
         const bool attr = mode == 0 && (parts & PART_ATTRACTION);
         uint32_t e = live ? d.row_off[v] : 0;
         const uint32_t e1 = live ? d.row_off[v + 1] : 0;
         const bool longrow = e1 - e > LONG_ROW;  // summed by k_long_rows
         if (longrow) e = e1;
         if (attr) {
-            // rounds of FIN_STAGE entries covering [row_off[base], row_off[rend])
+            // the chunk's long rows, ascending: their entries are left out of the staging
+            const uint64_t lm = gr_ballot(longrow);
+            const uint32_t wid = threadIdx.x / GR_WAVE;
+            __syncthreads();  // the previous chunk is done with s_lr / s_wl
+            if (gr_lane() == 0) s_wl[wid] = (uint32_t)__popcll(lm);
+            __syncthreads();
+            uint32_t nl = 0, off = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < GR_BLOCK / GR_WAVE; w++) {
+                if (w < wid) off += s_wl[w];
+                nl += s_wl[w];
+            }
+            if (longrow) {
+                const uint32_t slot = off + (uint32_t)__popcll(lm & gr_lanemask_lt());
+                s_lr_lo[slot] = d.row_off[v];
+                s_lr_hi[slot] = e1;
+            }
+            __syncthreads();
+            // segments between long rows, each in rounds of STAGE entries, covering
+            // [row_off[base], row_off[rend]); every short row lies inside one segment
             const uint32_t g0 = d.row_off[base], g1 = d.row_off[rend];
-            for (uint32_t s0 = g0; s0 < g1; s0 += FIN_STAGE) {
-                const uint32_t s1 = s0 + FIN_STAGE < g1 ? s0 + FIN_STAGE : g1;
-                __syncthreads();
-                for (uint32_t q = s0 + threadIdx.x; q < s1; q += GR_BLOCK) {
-                    s_nbr[q - s0] = d.nbr[q];
-                    if (wgt) s_w[q - s0] = wgt[q];
-                }
-                __syncthreads();
-                const uint32_t lim = e1 < s1 ? e1 : s1;
-                // four neighbours' gathers in flight at once, then their terms in entry
-                // order: the same sum as one entry at a time
-                constexpr int U = 4;
-                for (; e + U <= lim; e += U) {
-                    T pj[U][D];
+            uint32_t seg_lo = g0;
+            for (uint32_t li = 0; li <= nl; li++) {
+                const uint32_t seg_hi = li < nl ? s_lr_lo[li] : g1;
+                for (uint32_t s0 = seg_lo; s0 < seg_hi; s0 += STAGE) {
+                    const uint32_t s1 = s0 + STAGE < seg_hi ? s0 + STAGE : seg_hi;
+                    const uint32_t cnt = s1 - s0;
+                    __syncthreads();
+                    // G entries per lane per pass: their index loads, then their gathers,
+                    // all in flight before the first LDS store
+                    constexpr int G = 8;
+                    for (uint32_t q0 = threadIdx.x; q0 < cnt; q0 += G * GR_BLOCK) {
+                        uint32_t jj[G];
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const uint32_t j = s_nbr[e + u - s0];
+                        for (int k = 0; k < G; k++) {
+                            const uint32_t q = q0 + k * GR_BLOCK;
+                            jj[k] = q < cnt ? d.nbr[s0 + q] : 0u;
+                        }
+                        T pj[G][D], ww[G];
 #pragma unroll
-                        for (int a = 0; a < D; a++) pj[u][a] = pi[(size_t)j * D + a];
+                        for (int k = 0; k < G; k++) {
+                            const uint32_t q = q0 + k * GR_BLOCK;
+                            if (q < cnt) {
+#pragma unroll
+                                for (int a = 0; a < D; a++) pj[k][a] = pi[(size_t)jj[k] * D + a];
+                                if (wgt) ww[k] = wgt[s0 + q];
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < G; k++) {
+                            const uint32_t q = q0 + k * GR_BLOCK;
+                            if (q < cnt) {
+#pragma unroll
+                                for (int a = 0; a < D; a++) s_pj[(size_t)q * D + a] = pj[k][a];
+                                if (wgt) s_w[q] = ww[k];
+                            }
+                        }
                     }
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
+                    __syncthreads();
+                    const uint32_t lim = e1 < s1 ? e1 : s1;
+                    for (; e < lim; e++) {
+                        const uint32_t q = e - s0;
 #pragma unroll
                         for (int a = 0; a < D; a++) {
-                            T dd = pj[u][a] - pv[a];
-                            if (wgt) dd = dd * s_w[e + u - s0];
+                            T dd = s_pj[(size_t)q * D + a] - pv[a];
+                            if (wgt) dd = dd * s_w[q];
                             acc_add<T, KAHAN>(uv[a], cc[a], dd);
                         }
                     }
                 }
-                for (; e < lim; e++) {
-                    const uint32_t j = s_nbr[e - s0];
-                    T pj[D];
-#pragma unroll
-                    for (int a = 0; a < D; a++) pj[a] = pi[(size_t)j * D + a];
-#pragma unroll
-                    for (int a = 0; a < D; a++) {
-                        T dd = pj[a] - pv[a];
-                        if (wgt) dd = dd * s_w[e - s0];
-                        acc_add<T, KAHAN>(uv[a], cc[a], dd);
-                    }
-                }
+                if (li < nl) seg_lo = s_lr_hi[li];
             }
         }
         if (!live) continue;
@@ -1804,15 +1816,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
             for (int a = 0; a < D; a++) uv[a] = ((const T *)d.long_acc)[(size_t)lo * D + a];
         }
         if (mode == 0) {
-            if (parts & PART_REPULSION) {
+            if (addrep) {
 #pragma unroll
-                for (int a = 0; a < D; a++) uv[a] = uv[a] + ((const T *)d.rep[a])[v];
+                for (int a = 0; a < D; a++) uv[a] = uv[a] + rv[a];
             }
         } else {
 #pragma unroll
-            for (int a = 0; a < D; a++) uv[a] = ((const T *)d.u[a])[v];
+            for (int a = 0; a < D; a++) uv[a] = rv[a];
         }
-        const double mv = (double)((const T *)d.mass)[v];
+        const double mv = (double)mt;
         if (parts & PART_GRAVITY) {
             const GrGraph &gg = d.gr[graph_of(d, v)];
             double cen[D], nn = 0.0;
@@ -1841,8 +1853,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         d.nodeval[2 * (size_t)v] = mv * sqrt(nbk);
         d.nodeval[2 * (size_t)v + 1] = 0.5 * mv * sqrt(nfw);
     }
-    (void)sw;
-    (void)tr;
 }
 
 // swing and traction partial sums per node chunk (chunks never cross graphs), fixed order
@@ -2083,8 +2093,8 @@ static void forces_impl(FaDev &d, uint32_t parts) {
     }
     if (mode == 0 && (parts & PART_ATTRACTION) && d.nlong > 0)
         hipLaunchKernelGGL((k_long_rows<T, D, KAHAN>), dim3(d.nlong), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK), 0, s, d, mode,
-                       parts);
+    hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK),
+                       FIN_LDS, s, d, mode, parts);
     hipLaunchKernelGGL(k_stats, dim3(d.nchunks), dim3(GR_BLOCK), 0, s, d);
 }
 

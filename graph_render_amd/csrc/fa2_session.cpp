@@ -631,8 +631,25 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     TRY_ALLOC(d.lc_int, (size_t)d.lcell_cap * 16);
     TRY_ALLOC(d.lc_dbl, (size_t)d.lcell_cap * 2 * d.dim * 8);
     TRY_ALLOC(d.lc_sum, (size_t)d.lcell_cap * (1 + d.dim) * 8);
-    d.nparts = gr_xcd_grid(gr_blocks(nn, GR_BLOCK));  // a multiple of the XCD count
-    if (d.nparts > 2048) d.nparts = 2048;
+    // k_finish's row split: workgroup b takes the rows from the first one whose
+    // (first entry + row index) reaches b/nparts of (entries + rows), so that rows and
+    // entries are balanced together and each workgroup stages its entries in one round
+    std::vector<uint32_t> part_rows;
+    {
+        const uint64_t units = (uint64_t)g.nbr.size() + n;
+        const uint32_t per = gr::fin_stage((uint32_t)es, d.dim, d.weighted != 0);
+        d.nparts = gr_xcd_grid(gr_blocks(units, per));
+        part_rows.assign((size_t)d.nparts + 1, n);
+        uint32_t r = 0;
+        for (uint32_t b = 0; b < d.nparts; b++) {
+            const uint64_t cut = units * b / d.nparts;
+            while (r < n && (uint64_t)g.row_off[r] + r < cut) r++;
+            part_rows[b] = r;
+        }
+        part_rows[0] = 0;
+        part_rows[d.nparts] = n;
+    }
+    TRY_ALLOC(d.part_rows, ((size_t)d.nparts + 1) * 4);
     TRY_ALLOC(d.bpart, (size_t)d.nchunks * 10 * 8);
     TRY_ALLOC(d.spart, (size_t)d.nchunks * 2 * 8);
     TRY_ALLOC(d.mpart, (size_t)d.nchunks * 8);
@@ -679,7 +696,8 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     };
     bool good = upload(d.row_off, g.row_off.data(), ((size_t)n + 1) * 4) &&
                 upload(d.nbr, g.nbr.data(), g.nbr.size() * 4) &&
-                upload(d.long_rows, long_rows.data(), long_rows.size() * 4);
+                upload(d.long_rows, long_rows.data(), long_rows.size() * 4) &&
+                upload(d.part_rows, part_rows.data(), part_rows.size() * 4);
     if (d.precision == gr::PREC_FP64) {
         good = good && upload(d.mass, s->mass.data(), (size_t)n * 8);
         if (d.weighted) good = good && upload(d.w, g.roww.data(), g.roww.size() * 8);

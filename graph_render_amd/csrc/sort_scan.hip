@@ -121,11 +121,16 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    uint32_t dg[RS_ITEMS];  // every key's digit first (all loads in flight), then the counts
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_BUCKETS - 1)], 1u);
+        const uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
+        const uint64_t key = keys[i < n ? i : n - 1];  // n >= 1: an unconditional load
+        dg[r] = i < n ? (uint32_t)(key >> shift) & (RS_BUCKETS - 1) : RS_BUCKETS;
     }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++)
+        if (dg[r] < RS_BUCKETS) atomicAdd(&h[dg[r]], 1u);
     __syncthreads();
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
@@ -171,27 +176,40 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, co
     __shared__ uint32_t lds[8];
     const uint32_t tid = threadIdx.x, wave = tid >> 6;
     const uint32_t b = blockIdx.x;
+    const uint64_t tile = (uint64_t)b * RS_TILE;
+    const uint32_t tn = tile + RS_TILE <= n ? RS_TILE : (uint32_t)(n - tile);
+    // the tile's keys and values, all loads in flight at once; the ranking rounds below
+    // then run from registers and LDS only
+    uint64_t kk[RS_ITEMS];
+    uint32_t vv[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint32_t li = (uint32_t)r * GR_BLOCK + tid;
+        const uint64_t at = tile + (li < tn ? li : tn - 1);  // tn >= 1: unconditional loads
+        kk[r] = kin[at];
+        vv[r] = vin[at];
+    }
     {
         const uint32_t t = totals[tid];
-        const uint32_t inc = block_inclusive_scan(t, lds);
         const uint32_t h = hist[(size_t)tid * nb + b];
+        const uint32_t hn = hist[(size_t)tid * nb + (b + 1 < nb ? b + 1 : b)];
+        const uint32_t inc = block_inclusive_scan(t, lds);
         gbase[tid] = inc - t + h;
-        const uint32_t next = b + 1 < nb ? hist[(size_t)tid * nb + b + 1] : t;
+        const uint32_t next = b + 1 < nb ? hn : t;
         const uint32_t cnt = next - h;
         const uint32_t linc = block_inclusive_scan(cnt, lds);
         lbase[tid] = linc - cnt;
         run[tid] = linc - cnt;
     }
     __syncthreads();
-    const uint64_t tile = (uint64_t)b * RS_TILE;
-    const uint32_t tn = tile + RS_TILE <= n ? RS_TILE : (uint32_t)(n - tile);
     const uint64_t below = gr_lanemask_lt();
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
         if ((uint32_t)r * GR_BLOCK >= tn) break;  // block-uniform
         const uint32_t li = (uint32_t)r * GR_BLOCK + tid;
         const bool valid = li < tn;
-        const uint64_t key = valid ? kin[tile + li] : 0;
-        const uint32_t val = valid ? vin[tile + li] : 0;
+        const uint64_t key = kk[r];
+        const uint32_t val = vv[r];
         const uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
 #pragma unroll
         for (int q = 0; q < GR_BLOCK / GR_WAVE; q++) wcnt[q][tid] = 0;

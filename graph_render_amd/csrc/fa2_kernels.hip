@@ -732,19 +732,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_group_info(const FaDev d) {
     d.ginfo[g] = type | (level << 8);
 }
 
-__global__ void k_cells_ready(const FaDev d) {
-    GrState *st = d.st;
-    if (halted(d)) return;
-    if (st->need_cells > d.cap) {
-        st->halt = 1;
-        return;
-    }
-    st->n_cells = st->need_cells;
-    // the centre-of-mass pass counts its span cells per level from zero
-    for (int l = 0; l < 66; l++) d.span_lcnt[l] = 0;
-}
-
-
 // The last group h in [lo, hi) whose key shares `level` levels with `key`, given that group
 // lo does and group hi (when hi < G) does not. Two probes per round, issued together, so a
 // round trip narrows the interval threefold; with no bound yet (hi == G) the bracket is
@@ -862,13 +849,25 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
 // Writes group g's cells in preorder (preorder.rs:14-99 layout: skip, start, end per cell).
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
-    const GrState *st = d.st;
+    GrState *st = d.st;
     if (halted(d)) return;
+    // capacity check of the cells the scan asked for: every workgroup makes it, the first
+    // one records the outcome and clears the span-cell level counters of the next pass
+    const uint32_t C = st->need_cells;
+    if (blockIdx.x == 0) {
+        if (C > d.cap) {
+            if (threadIdx.x == 0) st->halt = 1;
+        } else {
+            if (threadIdx.x == 0) st->n_cells = C;
+            if (threadIdx.x < 66) d.span_lcnt[threadIdx.x] = 0;
+        }
+    }
+    if (C > d.cap) return;
     uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
     const uint32_t G = st->n_groups;
     if (g >= G) return;
     const int L = (int)d.levels;
-    const uint32_t C = st->n_cells, nf = st->n_finite;
+    const uint32_t nf = st->n_finite;
     const uint64_t *gkey = d.gkey;
     uint64_t key = gkey[g];
     int left = g > 0 ? key_lcp<D>(gkey[g - 1], key, L) : -1;
@@ -1704,6 +1703,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
     T *s_pj = (T *)fin_lds;     // [STAGE][D]
     T *s_w = s_pj + STAGE * D;  // [STAGE], weighted graphs only
     __shared__ uint32_t s_lr_lo[GR_BLOCK], s_lr_hi[GR_BLOCK], s_wl[GR_BLOCK / GR_WAVE];
+    __shared__ uint32_t s_g[2];
     const uint32_t lb = gr_xcd_block(blockIdx.x, gridDim.x);
     const uint32_t r0 = d.part_rows[lb], r1 = d.part_rows[lb + 1];
     const double grav = d.gravity;
@@ -1724,16 +1724,19 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         }
         const T mt = live ? ((const T *)d.mass)[v] : (T)0;
         const bool attr = mode == 0 && (parts & PART_ATTRACTION);
-        uint32_t e = live ? d.row_off[v] : 0;
+        const uint32_t e0 = live ? d.row_off[v] : 0;
         const uint32_t e1 = live ? d.row_off[v + 1] : 0;
-        const bool longrow = e1 - e > LONG_ROW;  // summed by k_long_rows
-        if (longrow) e = e1;
+        const bool longrow = e1 - e0 > LONG_ROW;  // summed by k_long_rows
+        uint32_t e = longrow ? e1 : e0;
         if (attr) {
             // the chunk's long rows, ascending: their entries are left out of the staging
             const uint64_t lm = gr_ballot(longrow);
             const uint32_t wid = threadIdx.x / GR_WAVE;
-            __syncthreads();  // the previous chunk is done with s_lr / s_wl
+            __syncthreads();  // the previous chunk is done with s_lr / s_wl / s_g
             if (gr_lane() == 0) s_wl[wid] = (uint32_t)__popcll(lm);
+            // the chunk's entries [s_g[0], s_g[1]), from its first and last rows' lanes
+            if (live && v == base) s_g[0] = e0;
+            if (live && v + 1 == rend) s_g[1] = e1;
             __syncthreads();
             uint32_t nl = 0, off = 0;
 #pragma unroll
@@ -1743,13 +1746,13 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
             }
             if (longrow) {
                 const uint32_t slot = off + (uint32_t)__popcll(lm & gr_lanemask_lt());
-                s_lr_lo[slot] = d.row_off[v];
+                s_lr_lo[slot] = e0;
                 s_lr_hi[slot] = e1;
             }
             __syncthreads();
             // segments between long rows, each in rounds of STAGE entries, covering
             // [row_off[base], row_off[rend]); every short row lies inside one segment
-            const uint32_t g0 = d.row_off[base], g1 = d.row_off[rend];
+            const uint32_t g0 = s_g[0], g1 = s_g[1];
             uint32_t seg_lo = g0;
             for (uint32_t li = 0; li <= nl; li++) {
                 const uint32_t seg_hi = li < nl ? s_lr_lo[li] : g1;
@@ -1860,9 +1863,23 @@ __global__ __launch_bounds__(GR_BLOCK) void k_stats(const FaDev d) {
     if (halted(d)) return;
     const uint32_t c0 = d.chunk[blockIdx.x], c1 = d.chunk[blockIdx.x + 1];
     double v2[2] = {0.0, 0.0};
-    for (uint32_t v = c0 + threadIdx.x; v < c1; v += GR_BLOCK) {
-        v2[0] = v2[0] + d.nodeval[2 * (size_t)v];
-        v2[1] = v2[1] + d.nodeval[2 * (size_t)v + 1];
+    // lane t adds nodes c0 + t, + 256, ... in ascending order, eight loads in flight
+    constexpr int SI = 8;
+    const double2 *nv = (const double2 *)d.nodeval;
+    for (uint32_t vb = c0; vb < c1; vb += SI * GR_BLOCK) {
+        double2 q[SI];
+#pragma unroll
+        for (int k = 0; k < SI; k++) {
+            const uint32_t v = vb + k * GR_BLOCK + threadIdx.x;
+            q[k] = nv[v < c1 ? v : c0];
+        }
+#pragma unroll
+        for (int k = 0; k < SI; k++) {
+            if (vb + k * GR_BLOCK + threadIdx.x < c1) {
+                v2[0] = v2[0] + q[k].x;
+                v2[1] = v2[1] + q[k].y;
+            }
+        }
     }
     __shared__ double lds[4 * 2];
     block_reduce<2, 0>(v2, lds);
@@ -1928,37 +1945,56 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
         sum[a] = 0.0;
     }
     const uint32_t c0 = d.chunk[blockIdx.x], c1 = d.chunk[blockIdx.x + 1];
-    for (uint32_t v = c0 + threadIdx.x; v < c1; v += GR_BLOCK) {
-        if (frozen) {
+    // AI nodes per lane (v = c0 + lane, + 256, ...): all their loads first, in flight
+    // together, then their steps in that same ascending order
+    constexpr int AI = sizeof(T) == 4 ? 8 : 4;
+    for (uint32_t vb = c0; vb < c1; vb += AI * GR_BLOCK) {
+        T pv[AI][D], uu[AI][D], mm[AI];
+#pragma unroll
+        for (int k = 0; k < AI; k++) {
+            const uint32_t v = vb + k * GR_BLOCK + threadIdx.x;
+            const uint32_t vi = v < c1 ? v : c0;  // an idle lane re-reads a valid node
+#pragma unroll
+            for (int a = 0; a < D; a++) pv[k][a] = ((const T *)d.pos[a])[vi];
+            if (!frozen) {
+#pragma unroll
+                for (int a = 0; a < D; a++) uu[k][a] = ((const T *)d.u[a])[vi];
+                mm[k] = ((const T *)d.mass)[vi];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < AI; k++) {
+            const uint32_t v = vb + k * GR_BLOCK + threadIdx.x;
+            if (v >= c1) continue;
             double p[D];
+            if (frozen) {
 #pragma unroll
-            for (int a = 0; a < D; a++) p[a] = (double)((const T *)d.pos[a])[v];
+                for (int a = 0; a < D; a++) p[a] = (double)pv[k][a];
+                bounds_take<D>(p, lo, hi, sum, nfin);
+                continue;
+            }
+            double uv[D], nn = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                uv[a] = (double)uu[k][a];
+                nn += uv[a] * uv[a];
+            }
+            const double norm = sqrt(nn);
+            const double factor = speed / (1.0 + sqrt(speed * (double)mm[k] * norm));
+            double step = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                const double dd = uv[a] * factor;
+                const T np = (T)((double)pv[k][a] + dd);
+                ((T *)d.pos[a])[v] = np;
+                ((T *)d.pi)[(size_t)v * D + a] = np;
+                ((T *)d.pk)[(size_t)v * 4 + a] = np;
+                p[a] = (double)np;
+                step += fabs(dd);
+            }
+            moved += step;
             bounds_take<D>(p, lo, hi, sum, nfin);
-            continue;
         }
-        double uv[D], nn = 0.0;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-            uv[a] = (double)((const T *)d.u[a])[v];
-            nn += uv[a] * uv[a];
-        }
-        const double norm = sqrt(nn);
-        const double factor =
-            speed / (1.0 + sqrt(speed * (double)((const T *)d.mass)[v] * norm));
-        double step = 0.0, p[D];
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-            const double dd = uv[a] * factor;
-            T *P = (T *)d.pos[a];
-            T np = (T)((double)P[v] + dd);
-            P[v] = np;
-            ((T *)d.pi)[(size_t)v * D + a] = np;
-            ((T *)d.pk)[(size_t)v * 4 + a] = np;
-            p[a] = (double)np;
-            step += fabs(dd);
-        }
-        moved += step;
-        bounds_take<D>(p, lo, hi, sum, nfin);
     }
     bounds_write<D>(lo, hi, sum, nfin, d.bpart);
     __shared__ double lds[4];
@@ -1971,17 +2007,26 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
 __global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
     if (halted(d)) return;
     GrGraph *st = d.gr + blockIdx.x;
-    if (st->done) return;  // block-uniform
-    __shared__ double lds[4];
-    double v[1] = {0.0};
-    for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
-         c += GR_BLOCK)
-        v[0] = v[0] + d.mpart[c];
-    block_reduce<1, 0>(v, lds);
-    if (threadIdx.x == 0) {
-        st->moved = v[0];
-        st->iters_done += 1;
-        if (v[0] < 1e-10) st->done = 1;
+    if (!st->done) {  // block-uniform
+        __shared__ double lds[4];
+        double v[1] = {0.0};
+        for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
+             c += GR_BLOCK)
+            v[0] = v[0] + d.mpart[c];
+        block_reduce<1, 0>(v, lds);
+        if (threadIdx.x == 0) {
+            st->moved = v[0];
+            st->iters_done += 1;
+            if (v[0] < 1e-10) st->done = 1;
+        }
+    }
+    // one graph: the session's bookkeeping (k_tail_global's) from the same thread
+    if (d.ngraphs == 1 && threadIdx.x == 0) {
+        GrState *gs = d.st;
+        const uint32_t live = st->done ? 0u : 1u;
+        if (gs->live > 0) gs->iters_done += 1;
+        gs->live = live;
+        if (live == 0) gs->done = 1;
     }
 }
 
@@ -2039,7 +2084,6 @@ static void tree_impl(FaDev &d) {
     hipLaunchKernelGGL((k_group_compact<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_group_info<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.gcells, d.gbase, n, &d.st->n_groups, &d.st->need_cells, d.scan_tmp, s);
-    hipLaunchKernelGGL(k_cells_ready, dim3(1), dim3(1), 0, s, d);
     hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
     hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
@@ -2105,7 +2149,7 @@ static void iteration_impl(FaDev &d) {
     hipLaunchKernelGGL(k_estimate, dim3(d.ngraphs), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_apply<T, D>), dim3(d.nchunks), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL(k_tail, dim3(d.ngraphs), dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL(k_tail_global, dim3(1), dim3(GR_BLOCK), 0, s, d);
+    if (d.ngraphs > 1) hipLaunchKernelGGL(k_tail_global, dim3(1), dim3(GR_BLOCK), 0, s, d);
 }
 
 #define GR_DISPATCH(FN, ...)                                                                   \

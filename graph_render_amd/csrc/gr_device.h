@@ -101,6 +101,17 @@ __device__ __forceinline__ uint64_t wave_lt(double a, double b) {
     return __builtin_amdgcn_fcmp(a, b, 4);
 }
 
+// A raw buffer resource over [p, p + bytes): a load through it at a byte offset at or past
+// `bytes` returns 0 and never faults, so a tile's loads can all be issued unconditionally
+// (straight-line code keeps them in flight together; a per-load bounds branch makes each
+// one wait for the last).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t gr_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t gr_load_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8). This
 // maps them so that XCD x owns logical blocks [x*q, (x+1)*q): consecutive bodies or rows
 // share one XCD's L2. Launch gr_xcd_grid(nb) workgroups; a logical index >= nb is idle.

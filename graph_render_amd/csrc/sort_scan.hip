@@ -38,12 +38,12 @@ __global__ __launch_bounds__(GR_BLOCK) void k_scan_reduce(const uint32_t *in, ui
     __shared__ uint32_t lds[8];
     const uint32_t n = load_count(d_count, max_n);
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint32_t tn = base < n ? (n - base < SCAN_TILE ? (uint32_t)(n - base) : SCAN_TILE) : 0;
+    const __amdgpu_buffer_rsrc_t rin = gr_rsrc(in + base, tn * 4);
     uint32_t s = 0;
 #pragma unroll
-    for (int r = 0; r < SCAN_ITEMS; r++) {
-        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
-        if (i < n) s += in[i];
-    }
+    for (int r = 0; r < SCAN_ITEMS; r++)
+        s += gr_load_u32(rin, ((uint32_t)r * GR_BLOCK + threadIdx.x) * 4);
     s = gr_wave_sum_u32(s);
     if ((threadIdx.x & 63u) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -77,12 +77,18 @@ __global__ __launch_bounds__(GR_BLOCK) void k_scan_down(const uint32_t *in, uint
     __shared__ uint32_t carry;
     const uint32_t n = load_count(d_count, max_n);
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint32_t tn = base < n ? (n - base < SCAN_TILE ? (uint32_t)(n - base) : SCAN_TILE) : 0;
+    const __amdgpu_buffer_rsrc_t rin = gr_rsrc(in + base, tn * 4);
+    uint32_t vin[SCAN_ITEMS];  // the whole tile in flight before the first block scan
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; r++)
+        vin[r] = gr_load_u32(rin, ((uint32_t)r * GR_BLOCK + threadIdx.x) * 4);
     if (threadIdx.x == 0) carry = partials[blockIdx.x];
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; r++) {
         uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
-        uint32_t v = i < n ? in[i] : 0;
+        uint32_t v = vin[r];
         uint32_t inc = block_inclusive_scan(v, lds);
         uint32_t c = carry;
         if (i < n) out[i] = c + inc - v;

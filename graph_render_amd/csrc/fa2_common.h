@@ -78,6 +78,29 @@ __device__ __forceinline__ bool halted(const FaDev &d) {
     return st->halt != 0 || (!d.ignore_done && st->done != 0);
 }
 
+// The first eight words of GrState, read in one go at a kernel's start: the halt test and
+// the counts the kernel sizes itself by then cost one memory round trip, not two.
+struct StateHead {
+    uint32_t iters_done, done, halt, n_finite, n_groups, n_cells, need_cells, pool_used;
+};
+static_assert(offsetof(GrState, done) == 4 && offsetof(GrState, halt) == 8 &&
+                  offsetof(GrState, n_finite) == 12 && offsetof(GrState, n_groups) == 16 &&
+                  offsetof(GrState, n_cells) == 20 && offsetof(GrState, need_cells) == 24,
+              "StateHead mirrors the head of GrState");
+__device__ __forceinline__ StateHead state_head(const FaDev &d) {
+    const uint4 *q = (const uint4 *)d.st;
+    const uint4 a = q[0], b = q[1];
+    // an empty statement that takes all eight words: both loads are issued and waited for
+    // here, ahead of the halt branch (a word used only past the branch would otherwise be
+    // loaded there, a second round trip)
+    asm volatile("" ::"s"(a.x), "s"(a.y), "s"(a.z), "s"(a.w), "s"(b.x), "s"(b.y), "s"(b.z),
+                 "s"(b.w));
+    return StateHead{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+__device__ __forceinline__ bool halted(const FaDev &d, const StateHead &h) {
+    return h.halt != 0 || (!d.ignore_done && h.done != 0);
+}
+
 // m_a*m_b/d2*k. FP64 divides exactly as the reference does; the FP32 precisions use the
 // hardware reciprocal (about 1 ulp), well inside their 1e-5 force tolerance.
 // a * b + c: FP32 fuses it into one rounding (FP32 results are held to a tolerance, not

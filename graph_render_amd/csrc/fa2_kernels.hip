@@ -422,15 +422,17 @@ __global__ __launch_bounds__(GR_BLOCK) void k_keys(const FaDev d) {
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_gather(const FaDev d) {
     const GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (p >= d.n) return;
-    const uint32_t nf = st->n_finite;
+    const uint32_t nf = sh.n_finite;
     // Non-finite points follow the tree's points in ascending index (preorder.rs:54-59);
     // the sort left them descending, so that tail is read backwards.
     uint32_t src = p < nf ? p : nf + (d.n - 1 - p);
     uint32_t i = d.vals[src];
     uint64_t key = d.keys[src];
+    const uint64_t kprev = d.keys[p > 0 ? p - 1 : 0];  // loaded with the others
     d.order[p] = i;
     // position and mass of point i from one 4-slot record (one cache line)
     const T *pk = (const T *)d.pk + (size_t)i * 4;
@@ -440,15 +442,16 @@ __global__ __launch_bounds__(GR_BLOCK) void k_gather(const FaDev d) {
 #pragma unroll
     for (int a = 0; a < D; a++) ((T *)d.sp[a])[p] = rec[a];
     ((T *)d.sm)[p] = rec[D];
-    d.flag[p] = (p < nf && (p == 0 || d.keys[p - 1] != key)) ? 1u : 0u;
+    d.flag[p] = (p < nf && (p == 0 || kprev != key)) ? 1u : 0u;
 }
 
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_group_compact(const FaDev d) {
     const GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     uint32_t p = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (p >= st->n_finite) return;
+    if (p >= sh.n_finite) return;
     if (!d.flag[p]) {
         // a later point of a group: does its position differ from the previous one's?
         uint32_t distinct = 0;
@@ -686,16 +689,19 @@ __device__ bool local_build(const FaDev &d, uint32_t s, uint32_t e, Box<D> rootb
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_group_info(const FaDev d) {
     GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
-    const uint32_t G = st->n_groups;
+    const uint32_t G = sh.n_groups;
     if (g >= G) return;
     const int L = (int)d.levels;
-    uint64_t key = d.gkey[g];
-    int left = g > 0 ? key_lcp<D>(d.gkey[g - 1], key, L) : -1;
-    int right = g + 1 < G ? key_lcp<D>(key, d.gkey[g + 1], L) : -1;
-    uint32_t s = d.gstart[g];
-    uint32_t e = g + 1 < G ? d.gstart[g + 1] : st->n_finite;
+    // the neighbouring groups' keys and starts, loaded together (clamped, unconditional)
+    const uint32_t gp = g > 0 ? g - 1 : g, gn = g + 1 < G ? g + 1 : g;
+    const uint64_t key = d.gkey[g], kprev = d.gkey[gp], knext = d.gkey[gn];
+    const uint32_t s = d.gstart[g], snext = d.gstart[gn];
+    int left = g > 0 ? key_lcp<D>(kprev, key, L) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, knext, L) : -1;
+    uint32_t e = g + 1 < G ? snext : sh.n_finite;
     bool distinct = false;
     for (uint32_t q = s + 1; q < e; q++)
         if (d.dflag[q]) {
@@ -850,10 +856,11 @@ __device__ __forceinline__ void emit_cell(const FaDev &d, uint32_t k, const Box<
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     // capacity check of the cells the scan asked for: every workgroup makes it, the first
     // one records the outcome and clears the span-cell level counters of the next pass
-    const uint32_t C = st->need_cells;
+    const uint32_t C = sh.need_cells;
     if (blockIdx.x == 0) {
         if (C > d.cap) {
             if (threadIdx.x == 0) st->halt = 1;
@@ -864,21 +871,24 @@ __global__ __launch_bounds__(GR_BLOCK) void k_emit(const FaDev d) {
     }
     if (C > d.cap) return;
     uint32_t g = blockIdx.x * GR_BLOCK + threadIdx.x;
-    const uint32_t G = st->n_groups;
+    const uint32_t G = sh.n_groups;
     if (g >= G) return;
     const int L = (int)d.levels;
-    const uint32_t nf = st->n_finite;
+    const uint32_t nf = sh.n_finite;
     const uint64_t *gkey = d.gkey;
-    uint64_t key = gkey[g];
-    int left = g > 0 ? key_lcp<D>(gkey[g - 1], key, L) : -1;
-    int right = g + 1 < G ? key_lcp<D>(key, gkey[g + 1], L) : -1;
-    uint32_t info = d.ginfo[g], type = info & 0xFF;
+    // this group's and its neighbours' records, loaded together (clamped, unconditional)
+    const uint32_t gp = g > 0 ? g - 1 : g, gn = g + 1 < G ? g + 1 : g;
+    const uint64_t key = gkey[g], kprev = gkey[gp], knext = gkey[gn];
+    const uint32_t info = d.ginfo[g], start = d.gstart[g], snext = d.gstart[gn];
+    const uint32_t gbase0 = d.gbase[g], gbase1 = d.gbase[gn];
+    int left = g > 0 ? key_lcp<D>(kprev, key, L) : -1;
+    int right = g + 1 < G ? key_lcp<D>(key, knext, L) : -1;
+    const uint32_t type = info & 0xFF;
     int level = (int)(info >> 8);
     int last_internal = type == G_SIMPLE ? level - 1 : (type == G_STALL ? level - 1 : L - 1);
-    const uint32_t start = d.gstart[g];
-    uint32_t gend = g + 1 < G ? d.gstart[g + 1] : nf;
-    uint32_t gskip = g + 1 < G ? d.gbase[g + 1] : C;
-    uint32_t idx = d.gbase[g];
+    uint32_t gend = g + 1 < G ? snext : nf;
+    uint32_t gskip = g + 1 < G ? gbase1 : C;
+    uint32_t idx = gbase0;
     const uint32_t gb = graph_of(d, d.order[start]);
     Box<D> b = root_box<D>(d.gr[gb]);
     int top = type == G_DEEP ? L - 1 : level;
@@ -983,13 +993,14 @@ constexpr uint32_t COM_THREADS = 1024;
 template <typename T, int D>
 __global__ __launch_bounds__(COM_THREADS) void k_com_tiles(const FaDev d) {
     const GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     __shared__ double sums[COM_TILE][1 + D];
     __shared__ uint32_t skips[COM_TILE];
     __shared__ uint16_t bylevel[COM_TILE];  // the tile's internal cells, bucketed by level
     __shared__ uint32_t lcnt[32], loff[33];
     __shared__ uint32_t nspan;
-    const uint32_t C = st->n_cells;
+    const uint32_t C = sh.n_cells;
     const uint32_t t0 = blockIdx.x * COM_TILE;
     if (t0 >= C) return;
     const uint32_t t1 = t0 + COM_TILE < C ? t0 + COM_TILE : C;
@@ -1132,10 +1143,11 @@ __device__ __forceinline__ void span_offsets(const FaDev &d, uint32_t *loff) {
 template <typename T, int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_span_prep(const FaDev d) {
     const GrState *st = d.st;
-    if (halted(d)) return;
+    const StateHead sh = state_head(d);
+    if (halted(d, sh)) return;
     __shared__ uint32_t loff[34];
     span_offsets(d, loff);
-    const uint32_t C = st->n_cells;
+    const uint32_t C = sh.n_cells;
     const uint32_t ntiles = (C + COM_TILE - 1) / COM_TILE;
     const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (i >= ntiles * SPAN_SLOTS || (i % SPAN_SLOTS) >= d.span_n[i / SPAN_SLOTS]) return;

@@ -1703,10 +1703,23 @@ __global__ __launch_bounds__(GR_BLOCK) void k_long_rows(const FaDev d) {
 // order. Entries of rows longer than LONG_ROW are not staged (k_long_rows sums them).
 // The row split (part_rows) is fixed by the CSR and computed once per session; each
 // workgroup gets about one round's worth of rows + entries. Dynamic LDS: FIN_LDS bytes.
+// waves per SIMD k_finish is built for: six fit its LDS, and the FP32 2D build is held to
+// the registers that allow them (it is latency-bound, so occupancy is its speed; the Kahan
+// and FP64 builds would spill)
 template <typename T, int D, bool KAHAN>
-__global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mode,
-                                                      uint32_t parts) {
-    const GrState *st = d.st;
+constexpr int fin_waves() {
+    return sizeof(T) == 4 && D == 2 && !KAHAN ? 6 : 1;
+}
+template <typename T, int D, bool KAHAN>
+__global__ __launch_bounds__(GR_BLOCK, (fin_waves<T, D, KAHAN>())) void k_finish(
+    const FaDev d, uint32_t mode, uint32_t parts) {
+    // the row range and (one graph) the centroid are read along with the halt flag, not
+    // after it: both reads are in bounds whatever the flag says
+    const uint32_t lb = gr_xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t r0 = d.part_rows[lb], r1 = d.part_rows[lb + 1];
+    double mean1[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) mean1[a] = d.gidn ? 0.0 : d.gr[0].mean[a];
     if (halted(d)) return;
     const T *pi = (const T *)d.pi;
     const T *wgt = (const T *)d.w;
@@ -1716,8 +1729,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
     T *s_w = s_pj + STAGE * D;  // [STAGE], weighted graphs only
     __shared__ uint32_t s_lr_lo[GR_BLOCK], s_lr_hi[GR_BLOCK], s_wl[GR_BLOCK / GR_WAVE];
     __shared__ uint32_t s_g[2];
-    const uint32_t lb = gr_xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t r0 = d.part_rows[lb], r1 = d.part_rows[lb + 1];
     const double grav = d.gravity;
     for (uint32_t base = r0; base < r1; base += GR_BLOCK) {
         const uint32_t rend = base + GR_BLOCK < r1 ? base + GR_BLOCK : r1;
@@ -1741,30 +1752,39 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         const bool longrow = e1 - e0 > LONG_ROW;  // summed by k_long_rows
         uint32_t e = longrow ? e1 : e0;
         if (attr) {
-            // the chunk's long rows, ascending: their entries are left out of the staging
-            const uint64_t lm = gr_ballot(longrow);
-            const uint32_t wid = threadIdx.x / GR_WAVE;
-            __syncthreads();  // the previous chunk is done with s_lr / s_wl / s_g
-            if (gr_lane() == 0) s_wl[wid] = (uint32_t)__popcll(lm);
-            // the chunk's entries [s_g[0], s_g[1]), from its first and last rows' lanes
-            if (live && v == base) s_g[0] = e0;
-            if (live && v + 1 == rend) s_g[1] = e1;
-            __syncthreads();
-            uint32_t nl = 0, off = 0;
+            // the chunk's entries [g0, g1) and its long rows, ascending, whose entries are
+            // left out of the staging; a graph without long rows reads the range directly
+            // (alongside the rows' own loads) and needs no exchange between lanes
+            uint32_t nl = 0, g0, g1;
+            if (d.nlong == 0) {
+                g0 = d.row_off[base];
+                g1 = d.row_off[rend];
+            } else {
+                const uint64_t lm = gr_ballot(longrow);
+                const uint32_t wid = threadIdx.x / GR_WAVE;
+                __syncthreads();  // the previous chunk is done with s_lr / s_wl / s_g
+                if (gr_lane() == 0) s_wl[wid] = (uint32_t)__popcll(lm);
+                // from the chunk's first and last rows' lanes
+                if (live && v == base) s_g[0] = e0;
+                if (live && v + 1 == rend) s_g[1] = e1;
+                __syncthreads();
+                uint32_t off = 0;
 #pragma unroll
-            for (uint32_t w = 0; w < GR_BLOCK / GR_WAVE; w++) {
-                if (w < wid) off += s_wl[w];
-                nl += s_wl[w];
+                for (uint32_t w = 0; w < GR_BLOCK / GR_WAVE; w++) {
+                    if (w < wid) off += s_wl[w];
+                    nl += s_wl[w];
+                }
+                if (longrow) {
+                    const uint32_t slot = off + (uint32_t)__popcll(lm & gr_lanemask_lt());
+                    s_lr_lo[slot] = e0;
+                    s_lr_hi[slot] = e1;
+                }
+                __syncthreads();
+                g0 = s_g[0];
+                g1 = s_g[1];
             }
-            if (longrow) {
-                const uint32_t slot = off + (uint32_t)__popcll(lm & gr_lanemask_lt());
-                s_lr_lo[slot] = e0;
-                s_lr_hi[slot] = e1;
-            }
-            __syncthreads();
             // segments between long rows, each in rounds of STAGE entries, covering
             // [row_off[base], row_off[rend]); every short row lies inside one segment
-            const uint32_t g0 = s_g[0], g1 = s_g[1];
             uint32_t seg_lo = g0;
             for (uint32_t li = 0; li <= nl; li++) {
                 const uint32_t seg_hi = li < nl ? s_lr_lo[li] : g1;
@@ -1841,11 +1861,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_finish(const FaDev d, uint32_t mod
         }
         const double mv = (double)mt;
         if (parts & PART_GRAVITY) {
-            const GrGraph &gg = d.gr[graph_of(d, v)];
             double cen[D], nn = 0.0;
 #pragma unroll
             for (int a = 0; a < D; a++) {
-                cen[a] = (double)pv[a] - gg.mean[a];
+                const double ma = d.gidn ? d.gr[d.gidn[v]].mean[a] : mean1[a];
+                cen[a] = (double)pv[a] - ma;
                 nn += cen[a] * cen[a];
             }
             double norm = sqrt(nn);

@@ -117,6 +117,8 @@ struct FaDev {
     float2 *pm_k = nullptr;   // kernel spectrum, transposed layout, 1/side^2 folded in
     float2 *pm_tw = nullptr;  // twiddles exp(-2 pi i t / side), t < side/2
     unsigned long long *pm_acc = nullptr;  // fixed-point deposit
+    uint32_t *pm_tiles = nullptr;  // deposit tiles: first sorted particle of each, (side/16)^2 + 1
+    unsigned long long *pm_halo = nullptr;  // deposit tiles: 33 cells beyond each tile
     void *pm_fr = nullptr;    // frame and the rung the spectrum was built for
     double pm_fix = 0.0;      // fixed-point scale of the deposit (a power of two)
     // law: 0 FA2 (mass deposit, G = r/|r|^2, infinite reach, read times k*m_i); 1 d3's

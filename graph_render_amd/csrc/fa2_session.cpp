@@ -672,6 +672,8 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
         TRY_ALLOC2(d.pm_k, P2 * sizeof(float2));
         TRY_ALLOC2(d.pm_acc, P2 * sizeof(unsigned long long));
         TRY_ALLOC2(d.pm_tw, (d.pm_side / 2) * sizeof(float2));
+        TRY_ALLOC2(d.pm_tiles, ((size_t)(d.pm_side / 16) * (d.pm_side / 16) + 2) * 4);
+        TRY_ALLOC2(d.pm_halo, (size_t)(d.pm_side / 16) * (d.pm_side / 16) * 33 * 8);
         TRY_ALLOC2(d.pm_fr, gr::pm_frame_bytes());
         std::vector<float2> tw(d.pm_side / 2);
         gr::pm_twiddles(d.pm_side, tw.data());

@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include "fa2_kernels.h"
+#include "sort_scan.h"
 
 namespace gr {
 
@@ -207,34 +208,121 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_transpose(const FaDev d, const 
 
 // ---------------------------------------------------------------- deposit / read
 
-__global__ __launch_bounds__(GR_BLOCK) void k_pm_clear(const FaDev d) {
-    const PmFrame *f = (const PmFrame *)d.pm_fr;
-    if (!f->valid) return;
-    const uint32_t P = d.pm_side, c = (uint32_t)f->cells;
-    for (uint64_t q = (uint64_t)blockIdx.x * GR_BLOCK + threadIdx.x; q < (uint64_t)c * P;
-         q += (uint64_t)gridDim.x * GR_BLOCK)
-        d.pm_acc[q] = 0ull;
+// The deposit works tile by tile. Particles are sorted by the PM_TILE x PM_TILE tile of
+// cells their stencil's lower corner (cx, cy) falls in; the workgroup that owns a tile sums
+// its particles' stencil weights in LDS over the tile plus one column and one row beyond it,
+// stores the tile's own cells with plain stores and leaves the 2 * PM_TILE + 1 cells beyond
+// it in the tile's halo record; k_pm_to_grid adds the halo records of the tiles left of and
+// below a cell when it reads the cell. No global atomics and no clearing pass: every cell of
+// the frame's cells x cells region is stored. The weights are the same fixed-point integers
+// a per-particle deposit would add, so the sums are exact and independent of order.
+constexpr uint32_t PM_TILE = 16;
+constexpr uint32_t PM_HALO = 2 * PM_TILE + 1;  // column beyond, row beyond, the corner
+
+__device__ __forceinline__ uint32_t pm_tiles_per_side(const PmFrame *f) {
+    return ((uint32_t)f->cells + PM_TILE - 1) / PM_TILE;
 }
 
+// the fixed-point CIC weights of particle i (cells (cx, cy), (cx+1, cy), (cx, cy+1),
+// (cx+1, cy+1)); false for a non-finite position
 template <typename T>
-__global__ __launch_bounds__(GR_BLOCK) void k_pm_deposit(const FaDev d) {
-    const PmFrame *f = (const PmFrame *)d.pm_fr;
-    if (!f->valid) return;
-    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (i >= d.n) return;
+__device__ __forceinline__ bool pm_weights(const FaDev &d, const PmFrame *f, uint32_t i,
+                                           int &cx, int &cy, unsigned long long w[4]) {
     const T *pi = (const T *)d.pi;
     const double px = (double)pi[2 * (size_t)i], py = (double)pi[2 * (size_t)i + 1];
-    if (!isfinite(px) || !isfinite(py)) return;
-    int cx, cy;
+    if (!isfinite(px) || !isfinite(py)) return false;
     double fx, fy;
     pm_stencil(f, px, py, cx, cy, fx, fy);
     const double m = (d.pm_law == 1 ? 1.0 : (double)((const T *)d.mass)[i]) * d.pm_fix;
+    w[0] = (unsigned long long)llrint(m * ((1.0 - fx) * (1.0 - fy)));
+    w[1] = (unsigned long long)llrint(m * (fx * (1.0 - fy)));
+    w[2] = (unsigned long long)llrint(m * ((1.0 - fx) * fy));
+    w[3] = (unsigned long long)llrint(m * (fx * fy));
+    return true;
+}
+
+// sort keys: the tile of each particle (tiles row-major, tiles_per_side wide); non-finite
+// particles and an invalid frame get `none`, past every tile
+template <typename T>
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_keys(const FaDev d, uint64_t *keys,
+                                                      uint32_t *vals, uint32_t none) {
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= d.n) return;
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    uint32_t key = none;
+    if (f->valid) {
+        const T *pi = (const T *)d.pi;
+        const double px = (double)pi[2 * (size_t)i], py = (double)pi[2 * (size_t)i + 1];
+        if (isfinite(px) && isfinite(py)) {
+            int cx, cy;
+            double fx, fy;
+            pm_stencil(f, px, py, cx, cy, fx, fy);
+            key = ((uint32_t)cy / PM_TILE) * pm_tiles_per_side(f) + (uint32_t)cx / PM_TILE;
+        }
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// pm_tiles[t] = the first sorted particle of tile t (t = 0 .. tiles, the last one the end of
+// the finite particles): position j writes the entries of the tiles in (key[j-1], key[j]],
+// so every entry is written exactly once
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_tile_starts(const FaDev d,
+                                                             const uint64_t *keys) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    const uint32_t j = blockIdx.x * GR_BLOCK + threadIdx.x;
+    const uint32_t n = d.n;
+    if (j > n) return;
+    const uint32_t nt = pm_tiles_per_side(f);
+    const int64_t tiles = (int64_t)nt * nt;
+    const int64_t prev = j > 0 ? (int64_t)keys[j - 1] : -1;
+    const int64_t cur = j < n ? (int64_t)keys[j] : tiles;
+    const int64_t hi = cur < tiles ? cur : tiles;
+    for (int64_t t = prev + 1; t <= hi; t++) d.pm_tiles[t] = j;
+}
+
+// one workgroup per tile: its particles' weights summed in LDS over (PM_TILE + 1)^2 cells,
+// the tile's cells stored, the rest left in its halo record (column PM_TILE rows 0..15, row
+// PM_TILE columns 0..15, then the corner)
+template <typename T>
+__global__ __launch_bounds__(GR_BLOCK) void k_pm_deposit(const FaDev d, const uint32_t *order) {
+    const PmFrame *f = (const PmFrame *)d.pm_fr;
+    if (!f->valid) return;
+    const uint32_t nt = pm_tiles_per_side(f);
+    const uint32_t t = blockIdx.x;
+    if (t >= nt * nt) return;
+    constexpr uint32_t W = PM_TILE + 1;
+    const uint32_t tx = t % nt, ty = t / nt;
+    const int x0 = (int)(tx * PM_TILE), y0 = (int)(ty * PM_TILE);
+    __shared__ unsigned long long acc[W * W];
+    for (uint32_t q = threadIdx.x; q < W * W; q += GR_BLOCK) acc[q] = 0ull;
+    __syncthreads();
+    const uint32_t j0 = d.pm_tiles[t], j1 = d.pm_tiles[t + 1];
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += GR_BLOCK) {
+        int cx, cy;
+        unsigned long long w[4];
+        if (!pm_weights<T>(d, f, order[j], cx, cy, w)) continue;
+        // the particle's tile is this one: cx - x0 and cy - y0 are in [0, PM_TILE)
+        const uint32_t at = (uint32_t)(cy - y0) * W + (uint32_t)(cx - x0);
+        atomicAdd(&acc[at], w[0]);  // LDS
+        atomicAdd(&acc[at + 1], w[1]);
+        atomicAdd(&acc[at + W], w[2]);
+        atomicAdd(&acc[at + W + 1], w[3]);
+    }
+    __syncthreads();
+    const int cells = f->cells;
     const uint32_t P = d.pm_side;
-    const size_t at = (size_t)cy * P + cx;
-    atomicAdd(&d.pm_acc[at], (unsigned long long)llrint(m * ((1.0 - fx) * (1.0 - fy))));
-    atomicAdd(&d.pm_acc[at + 1], (unsigned long long)llrint(m * (fx * (1.0 - fy))));
-    atomicAdd(&d.pm_acc[at + P], (unsigned long long)llrint(m * ((1.0 - fx) * fy)));
-    atomicAdd(&d.pm_acc[at + P + 1], (unsigned long long)llrint(m * (fx * fy)));
+    for (uint32_t q = threadIdx.x; q < PM_TILE * PM_TILE; q += GR_BLOCK) {
+        const uint32_t lx = q % PM_TILE, ly = q / PM_TILE;
+        const int X = x0 + (int)lx, Y = y0 + (int)ly;
+        if (X < cells && Y < cells) d.pm_acc[(size_t)Y * P + X] = acc[ly * W + lx];
+    }
+    unsigned long long *halo = d.pm_halo + (size_t)t * PM_HALO;
+    for (uint32_t q = threadIdx.x; q < PM_HALO; q += GR_BLOCK)
+        halo[q] = q < PM_TILE       ? acc[q * W + PM_TILE]
+                  : q < 2 * PM_TILE ? acc[PM_TILE * W + (q - PM_TILE)]
+                                    : acc[PM_TILE * W + PM_TILE];
 }
 
 __global__ __launch_bounds__(GR_BLOCK) void k_pm_to_grid(const FaDev d) {
@@ -242,10 +330,23 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_to_grid(const FaDev d) {
     if (!f->valid) return;
     const uint32_t P = d.pm_side, c = (uint32_t)f->cells;
     const double inv = 1.0 / d.pm_fix;
+    const uint32_t nt = pm_tiles_per_side(f);
     for (uint64_t q = (uint64_t)blockIdx.x * GR_BLOCK + threadIdx.x; q < (uint64_t)c * P;
          q += (uint64_t)gridDim.x * GR_BLOCK) {
-        const uint32_t col = (uint32_t)(q % P);
-        const float v = col < c ? (float)((double)d.pm_acc[q] * inv) : 0.f;
+        const uint32_t col = (uint32_t)(q % P), row = (uint32_t)(q / P);
+        float v = 0.f;
+        if (col < c) {
+            // the tile's own sum, plus what the tiles left of and below it left over
+            unsigned long long a = d.pm_acc[q];
+            const uint32_t tx = col / PM_TILE, ty = row / PM_TILE;
+            const uint32_t lx = col % PM_TILE, ly = row % PM_TILE;
+            if (lx == 0 && tx > 0) a += d.pm_halo[(size_t)(ty * nt + tx - 1) * PM_HALO + ly];
+            if (ly == 0 && ty > 0)
+                a += d.pm_halo[(size_t)((ty - 1) * nt + tx) * PM_HALO + PM_TILE + lx];
+            if (lx == 0 && ly == 0 && tx > 0 && ty > 0)
+                a += d.pm_halo[(size_t)((ty - 1) * nt + tx - 1) * PM_HALO + 2 * PM_TILE];
+            v = (float)((double)a * inv);
+        }
         d.pm_a[q] = make_float2(v, 0.f);
     }
 }
@@ -307,13 +408,33 @@ void pm_repulsion(FaDev &d) {
     hipLaunchKernelGGL(k_pm_commit, dim3(1), dim3(1), 0, s, d);
     // density, forward along x (rows with data), transpose, y-lines forward * K inverse,
     // transpose back (rows read), inverse along x, CIC read
-    hipLaunchKernelGGL(k_pm_clear, dim3(fill), dim3(GR_BLOCK), 0, s, d);
-    if (d.precision == PREC_FP64)
-        hipLaunchKernelGGL(k_pm_deposit<double>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK),
-                           0, s, d);
-    else
-        hipLaunchKernelGGL(k_pm_deposit<float>, dim3(gr_blocks(d.n, GR_BLOCK)), dim3(GR_BLOCK),
-                           0, s, d);
+    {
+        // particles sorted by deposit tile; the keys are written where an odd number of
+        // sort passes starts, so the sorted pairs end in (keys, vals)
+        const uint32_t side_tiles = P / PM_TILE;
+        const uint32_t none = side_tiles * side_tiles;  // past every tile of any frame
+        int bits = 1;
+        while ((1u << bits) <= none) bits++;
+        const bool odd = radix_digits(bits) & 1;
+        uint64_t *k0 = odd ? d.keys_alt : d.keys, *k1 = odd ? d.keys : d.keys_alt;
+        uint32_t *v0 = odd ? d.vals_alt : d.vals, *v1 = odd ? d.vals : d.vals_alt;
+        const uint32_t nb = gr_blocks(d.n, GR_BLOCK);
+        if (d.precision == PREC_FP64)
+            hipLaunchKernelGGL(k_pm_keys<double>, dim3(nb), dim3(GR_BLOCK), 0, s, d, k0, v0,
+                               none);
+        else
+            hipLaunchKernelGGL(k_pm_keys<float>, dim3(nb), dim3(GR_BLOCK), 0, s, d, k0, v0,
+                               none);
+        radix_sort_pairs(k0, v0, k1, v1, d.n, bits, d.sort_tmp, s);
+        hipLaunchKernelGGL(k_pm_tile_starts, dim3(gr_blocks((uint64_t)d.n + 1, GR_BLOCK)),
+                           dim3(GR_BLOCK), 0, s, d, (const uint64_t *)d.keys);
+        if (d.precision == PREC_FP64)
+            hipLaunchKernelGGL(k_pm_deposit<double>, dim3(none), dim3(GR_BLOCK), 0, s, d,
+                               (const uint32_t *)d.vals);
+        else
+            hipLaunchKernelGGL(k_pm_deposit<float>, dim3(none), dim3(GR_BLOCK), 0, s, d,
+                               (const uint32_t *)d.vals);
+    }
     hipLaunchKernelGGL(k_pm_to_grid, dim3(fill), dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_a, 0, -1);
     hipLaunchKernelGGL(k_pm_transpose, dim3(tiles, tiles), dim3(GR_BLOCK), 0, s, d,

@@ -10,9 +10,9 @@
 //   solve    mesh.rs:151-185 + kernel.rs: packed kernel Gx + i*Gy, one forward transform of
 //            the density, multiply, one inverse; Ex = re, Ey = im; 1/side^2 folded into G
 //   read     mesh.rs:276-288 (CIC gather)
-// The FFT is a hand-written radix-2 Stockham transform per line in LDS; 2D transforms are
-// row passes with tiled LDS transposes between them. Only the rows that hold density (and
-// the rows that are read back) are transformed along x.
+// The FFT is a hand-written radix-4 (+ one radix-2) Stockham transform per line in LDS; 2D
+// transforms are row passes with tiled LDS transposes between them. Only the rows that hold
+// density (and the rows that are read back) are transformed along x.
 #include <math.h>
 
 #include "fa2_kernels.h"
@@ -126,19 +126,58 @@ __global__ void k_pm_commit(const FaDev d) {
 
 // ---------------------------------------------------------------- FFT
 
-// Radix-2 Stockham over one line of P points held in LDS (x, y: P float2 each). Returns the
-// buffer holding the result. tw[t] = exp(-2 pi i t / P), t < P/2; dir +1 is the inverse
-// (conjugate twiddles, unnormalised).
+// exp(-2 pi i t / P) for t < P from the half table tw (t < P/2): w(t) = -w(t - P/2);
+// conjugated for the inverse (dir +1)
+__device__ __forceinline__ float2 tw_at(const float2 *__restrict__ tw, uint32_t t, uint32_t P,
+                                        int dir) {
+    const uint32_t h = P >> 1;
+    float2 w = t < h ? tw[t] : make_float2(-tw[t - h].x, -tw[t - h].y);
+    if (dir > 0) w.y = -w.y;
+    return w;
+}
+
+// Stockham (self-sorting) FFT over one line of P points held in LDS (x, y: P float2 each):
+// radix-4 stages while four sub-transforms fit, then one radix-2 stage when log2 P is odd.
+// Stage with sub-transform size ns and radix R: butterfly j takes in[j + r P/R] times the
+// twiddle w(r k P/(R ns)), k = j mod ns, and writes out[(j - k) R + k + r ns]. Returns the
+// buffer holding the result; dir +1 is the inverse (conjugate twiddles, unnormalised).
 __device__ float2 *stockham(float2 *x, float2 *y, uint32_t P, const float2 *__restrict__ tw,
                             int dir) {
     float2 *in = x, *out = y;
-    const uint32_t half = P >> 1;
-    for (uint32_t ns = 1; ns < P; ns <<= 1) {
-        const uint32_t tstride = P / (2 * ns);
+    uint32_t ns = 1;
+    const uint32_t q = P >> 2;
+    for (; ns * 4 <= P; ns <<= 2) {
+        const uint32_t tstride = P / (4 * ns);
+        for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) {
+            const uint32_t k = j & (ns - 1);
+            float2 v0 = in[j], v1 = in[j + q], v2 = in[j + 2 * q], v3 = in[j + 3 * q];
+            if (ns > 1) {  // block-uniform: the first stage's twiddles are all 1
+                v1 = cmul(v1, tw_at(tw, k * tstride, P, dir));
+                v2 = cmul(v2, tw_at(tw, 2 * k * tstride, P, dir));
+                v3 = cmul(v3, tw_at(tw, 3 * k * tstride, P, dir));
+            }
+            // 4-point DFT: y1 = a1 - i d, y3 = a1 + i d (forward; signs swap for the inverse)
+            const float2 a0 = make_float2(v0.x + v2.x, v0.y + v2.y);
+            const float2 a1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+            const float2 a2 = make_float2(v1.x + v3.x, v1.y + v3.y);
+            const float2 dd = make_float2(v1.x - v3.x, v1.y - v3.y);
+            const float2 a3 = dir < 0 ? make_float2(dd.y, -dd.x) : make_float2(-dd.y, dd.x);
+            const uint32_t idx = ((j - k) << 2) + k;
+            out[idx] = make_float2(a0.x + a2.x, a0.y + a2.y);
+            out[idx + ns] = make_float2(a1.x + a3.x, a1.y + a3.y);
+            out[idx + 2 * ns] = make_float2(a0.x - a2.x, a0.y - a2.y);
+            out[idx + 3 * ns] = make_float2(a1.x - a3.x, a1.y - a3.y);
+        }
+        __syncthreads();
+        float2 *t = in;
+        in = out;
+        out = t;
+    }
+    if (ns < P) {  // the radix-2 stage, ns = P / 2
+        const uint32_t half = P >> 1;
         for (uint32_t j = threadIdx.x; j < half; j += blockDim.x) {
             const uint32_t k = j & (ns - 1);
-            float2 w = tw[k * tstride];
-            if (dir > 0) w.y = -w.y;
+            const float2 w = tw_at(tw, k * (P / (2 * ns)), P, dir);
             const float2 v0 = in[j];
             const float2 v1 = cmul(in[j + half], w);
             const uint32_t idx = ((j - k) << 1) + k;

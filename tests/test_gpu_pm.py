@@ -76,3 +76,30 @@ def test_d3_charge_pass_matches_the_reference_fixture(state):
     err = np.abs(got - want).max() / np.abs(want).max()
     print(f"  max error / max |delta| = {err:.2e}", flush=True)
     assert err <= 1e-5
+
+
+def test_pm_skips_non_finite_points_and_repeats_bit_for_bit():
+    """Points at NaN / inf take no part in the deposit: every finite point's force equals,
+    bit for bit, its force in a session without them (no edges, so every mass is 1, and the
+    fixed-point scale, set by the total mass, is the same for 3000 and 3003 points); theirs is
+    NaN. The deposit is an exact integer sum, so a second session repeats the forces exactly."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    xy = np.concatenate([rng.normal(0, 40.0, (n // 2, 2)),
+                         rng.uniform(-150, 150, (n - n // 2, 2))])
+    bad = np.array([[np.nan, 1.0], [np.inf, 0.0], [2.0, -np.inf]])
+    at = np.array([5, 1700, 2999])
+    mixed = np.insert(xy, at - np.arange(3), bad, axis=0)
+    finite = np.isfinite(mixed).all(axis=1)
+    assert finite.sum() == n and (~finite).sum() == 3
+    e = np.zeros(0, np.uint32)
+    print("pm non-finite case n=3003 side=1024", flush=True)
+    with pm_session(n + 3, e, e, mixed, 1024) as sess:
+        f = sess.forces(2)
+    with pm_session(n + 3, e, e, mixed, 1024) as sess:
+        f2 = sess.forces(2)
+    with pm_session(n, e, e, mixed[finite], 1024) as sess:
+        ref = sess.forces(2)
+    assert np.array_equal(f, f2, equal_nan=True)
+    assert np.array_equal(f[finite], ref)
+    assert np.isnan(f[~finite]).all()

@@ -1225,7 +1225,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_span_link(const FaDev d) {
 // cells' sums stay in LDS and a level costs only LDS traffic and a barrier; otherwise a
 // span child's sum is read back from d.raw, where the level before wrote it.
 template <typename T, int D>
-__global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
+__global__ __launch_bounds__(1024) void k_com_span(const FaDev d, uint32_t levelwise) {
     const GrState *st = d.st;
     if (halted(d)) return;
     __shared__ uint32_t loff[34];
@@ -1290,6 +1290,7 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
         }
         return;
     }
+    if (levelwise) return;  // k_com_span_level takes the cells that overflow LDS
     for (int l = 31; l >= 0; l--) {
         const uint32_t a = loff[l], b = loff[l + 1];
         if (a == b) continue;  // block-uniform
@@ -1314,6 +1315,43 @@ __global__ __launch_bounds__(1024) void k_com_span(const FaDev d) {
             body_final<T, D>(d, cells[q], s);
         }
         __syncthreads();
+    }
+}
+
+// One level of the span pass spread over the GPU, for sessions whose span cells overflow
+// k_com_span's LDS path (batches): every span cell of level l sums its children, which are
+// final cells or span cells of deeper levels finished by the previous launch. Launched once
+// per level, deepest first.
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_com_span_level(const FaDev d, int l) {
+    if (halted(d)) return;
+    __shared__ uint32_t loff[34];
+    span_offsets(d, loff);
+    if (D == 2 && loff[33] <= SPAN_LDS) return;  // k_com_span's LDS path did them all
+    constexpr int KIDS = 1 << D;
+    const uint32_t a = loff[l], b = loff[l + 1];
+    const uint32_t *cells = d.span_cells;
+    const uint32_t *link = d.span_kids + (size_t)d.span_cap * KIDS;
+    for (uint32_t q = a + blockIdx.x * GR_BLOCK + threadIdx.x; q < b;
+         q += gridDim.x * GR_BLOCK) {
+        double s[1 + D];
+#pragma unroll
+        for (int y = 0; y <= D; y++) s[y] = 0.0;
+#pragma unroll
+        for (int x = 0; x < KIDS; x++) {
+            const uint32_t r = link[(size_t)x * d.span_cap + q];
+            if (r == NO_KID) continue;
+            if (r & SPAN_REF) {
+                const double *v = d.raw + (size_t)cells[r & ~SPAN_REF] * (1 + D);
+#pragma unroll
+                for (int y = 0; y <= D; y++) s[y] += v[y];
+            } else {
+#pragma unroll
+                for (int y = 0; y <= D; y++)
+                    s[y] += d.span_kval[(size_t)(x * (1 + D) + y) * d.span_cap + q];
+            }
+        }
+        body_final<T, D>(d, cells[q], s);
     }
 }
 
@@ -2123,7 +2161,15 @@ static void tree_impl(FaDev &d) {
                        dim3(GR_BLOCK), 0, s, d);
     hipLaunchKernelGGL((k_span_link<D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
                        dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d);
+    // a batch's span cells overflow the one-workgroup LDS pass: it then runs level by level
+    // over the whole GPU
+    const bool levelwise = d.ngraphs > 1;
+    hipLaunchKernelGGL((k_com_span<T, D>), dim3(1), dim3(1024), 0, s, d, (uint32_t)levelwise);
+    if (levelwise) {
+        for (int l = d.levels < 31 ? (int)d.levels : 31; l >= 0; l--)
+            hipLaunchKernelGGL((k_com_span_level<T, D>), dim3(gr_xcd_grid(64)), dim3(GR_BLOCK), 0,
+                               s, d, l);
+    }
 }
 
 template <typename T, int D, bool KAHAN>

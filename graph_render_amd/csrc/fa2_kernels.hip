@@ -1728,6 +1728,44 @@ __global__ __launch_bounds__(GR_BLOCK) void k_long_rows(const FaDev d) {
     }
 }
 
+// One row's attraction terms from the staged entries [e, lim) (entry q = e - s0), in
+// ascending entry order: (p_j - p_v) [* w] added to the row's sum per axis. Four entries'
+// LDS reads are issued before their adds, so the reads do not wait one by one.
+template <typename T, int D, bool KAHAN, bool W>
+__device__ __forceinline__ void row_sum(const T *s_pj, const T *s_w, uint32_t &e, uint32_t lim,
+                                        uint32_t s0, const T (&pv)[D], T (&uv)[D],
+                                        T (&cc)[D]) {
+    constexpr int U = 4;
+    for (; e + U <= lim; e += U) {
+        T pj[U][D], ww[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t q = e + u - s0;
+#pragma unroll
+            for (int a = 0; a < D; a++) pj[u][a] = s_pj[(size_t)q * D + a];
+            if (W) ww[u] = s_w[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                T dd = pj[u][a] - pv[a];
+                if (W) dd = dd * ww[u];
+                acc_add<T, KAHAN>(uv[a], cc[a], dd);
+            }
+        }
+    }
+    for (; e < lim; e++) {
+        const uint32_t q = e - s0;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            T dd = s_pj[(size_t)q * D + a] - pv[a];
+            if (W) dd = dd * s_w[q];
+            acc_add<T, KAHAN>(uv[a], cc[a], dd);
+        }
+    }
+}
+
 // Completes u (attraction + repulsion, then gravity, state.rs:130-148 order) and the
 // swing/traction partials (schedule.rs:16-30). mode 0: repulsion from the walk (u = attr +
 // rep); mode 1: u already holds attraction (+ dense pairs).
@@ -1862,15 +1900,10 @@ __global__ __launch_bounds__(GR_BLOCK, (fin_waves<T, D, KAHAN>())) void k_finish
                     }
                     __syncthreads();
                     const uint32_t lim = e1 < s1 ? e1 : s1;
-                    for (; e < lim; e++) {
-                        const uint32_t q = e - s0;
-#pragma unroll
-                        for (int a = 0; a < D; a++) {
-                            T dd = s_pj[(size_t)q * D + a] - pv[a];
-                            if (wgt) dd = dd * s_w[q];
-                            acc_add<T, KAHAN>(uv[a], cc[a], dd);
-                        }
-                    }
+                    if (wgt)
+                        row_sum<T, D, KAHAN, true>(s_pj, s_w, e, lim, s0, pv, uv, cc);
+                    else
+                        row_sum<T, D, KAHAN, false>(s_pj, s_w, e, lim, s0, pv, uv, cc);
                 }
                 if (li < nl) seg_lo = s_lr_hi[li];
             }

@@ -69,3 +69,33 @@ def test_batch_forces_match_single():
             f1 = s1.forces(7)
         assert np.array_equal(fb[off:off + t.n], f1)
         off += t.n
+
+
+def test_large_batch_equals_graphs_run_alone():
+    """A batch of about two million nodes: its centre-of-mass tiles (4096 cells each, about
+    nine span cells per tile) leave several thousand span cells, more than the one-workgroup
+    pass holds in LDS, so the span pass runs level by level over the GPU. Graphs picked from
+    across the batch are still bit-identical (FP64) to themselves run alone."""
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(20000, 30000, 80)
+    topos = [G.Topology(int(n), *bh_graph(int(n))) for n in sizes]
+    p = G.Fa2Params(max_iter=40)
+    print(f"large batch: {len(topos)} graphs, {int(sizes.sum())} nodes", flush=True)
+    with G.Session.batch(topos, p) as bs:
+        assert bs.step(6) == 6
+        pb = bs.positions()
+        tr = bs.tree(bounds=False)
+    # span cells: a cell whose subtree [k, skip) runs past the end of its 4096-cell tile; the
+    # one-workgroup pass holds 3072 of them
+    c = tr["ncells"]
+    k = np.arange(c)
+    tile_end = np.minimum((k // 4096 + 1) * 4096, c)
+    spans = int((tr["skip"] > tile_end).sum())
+    print(f"large batch: {c} cells, {spans} span cells", flush=True)
+    assert spans > 3072
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    for b in (0, 27, 53, 79):
+        with G.Session(topos[b], p) as s1:
+            assert s1.step(6) == 6
+            p1 = s1.positions()
+        assert np.array_equal(pb[offs[b]:offs[b + 1]], p1), b

@@ -168,33 +168,41 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scan(uint32_t *hist, uint32_t n
 // LDS counts, into the tile's digit-sorted order in LDS; the tile is then written out as one
 // contiguous run per digit, so global stores are coalesced. The tile's per-digit counts come
 // from the (already scanned) block histogram.
+// Stable scatter of one digit pass. Wave w takes the tile's keys [w * 512, (w + 1) * 512) in
+// eight rounds of 64 and ranks each key among the same-digit keys before it in the wave,
+// with wave-private digit counters in LDS (a wave's LDS reads and writes happen in program
+// order, so no block barrier is needed between rounds); one prefix over the waves then
+// gives each wave's first slot per digit, the keys go to their tile-local slots in LDS in
+// input order, and from there to their global slots.
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin,
                                                          uint64_t *kout, uint32_t *vout,
                                                          uint32_t n, int shift,
                                                          const uint32_t *hist, uint32_t nb,
                                                          const uint32_t *totals) {
+    constexpr uint32_t WAVES = GR_BLOCK / GR_WAVE;
+    constexpr uint32_t PER_WAVE = RS_ITEMS * GR_WAVE;  // consecutive keys per wave
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
     __shared__ uint32_t gbase[RS_BUCKETS];  // global position of the tile's first key per digit
     __shared__ uint32_t lbase[RS_BUCKETS];  // tile-local position of the same
-    __shared__ uint32_t run[RS_BUCKETS];
-    __shared__ uint32_t wcnt[GR_BLOCK / GR_WAVE][RS_BUCKETS];
+    __shared__ uint32_t wcnt[WAVES][RS_BUCKETS];  // per wave and digit: count, then first slot
     __shared__ uint32_t lds[8];
-    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, wave = tid / GR_WAVE, lane = tid % GR_WAVE;
     const uint32_t b = blockIdx.x;
     const uint64_t tile = (uint64_t)b * RS_TILE;
     const uint32_t tn = tile + RS_TILE <= n ? RS_TILE : (uint32_t)(n - tile);
-    // the tile's keys and values, all loads in flight at once; the ranking rounds below
-    // then run from registers and LDS only
+    // the wave's keys and values, all loads in flight at once (64 consecutive per round)
     uint64_t kk[RS_ITEMS];
     uint32_t vv[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const uint32_t li = (uint32_t)r * GR_BLOCK + tid;
+        const uint32_t li = wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane;
         const uint64_t at = tile + (li < tn ? li : tn - 1);  // tn >= 1: unconditional loads
         kk[r] = kin[at];
         vv[r] = vin[at];
     }
+#pragma unroll
+    for (uint32_t w = 0; w < WAVES; w++) wcnt[w][tid] = 0;
     {
         const uint32_t t = totals[tid];
         const uint32_t h = hist[(size_t)tid * nb + b];
@@ -205,41 +213,50 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, co
         const uint32_t cnt = next - h;
         const uint32_t linc = block_inclusive_scan(cnt, lds);
         lbase[tid] = linc - cnt;
-        run[tid] = linc - cnt;
     }
-    __syncthreads();
+    __syncthreads();  // counters cleared (block_inclusive_scan's barriers come before it too)
     const uint64_t below = gr_lanemask_lt();
+    uint32_t rank_in_wave[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        if ((uint32_t)r * GR_BLOCK >= tn) break;  // block-uniform
-        const uint32_t li = (uint32_t)r * GR_BLOCK + tid;
+        const uint32_t li = wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane;
         const bool valid = li < tn;
-        const uint64_t key = kk[r];
-        const uint32_t val = vv[r];
-        const uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
-#pragma unroll
-        for (int q = 0; q < GR_BLOCK / GR_WAVE; q++) wcnt[q][tid] = 0;
-        __syncthreads();
+        const uint32_t d = (uint32_t)(kk[r] >> shift) & (RS_BUCKETS - 1);
         uint64_t peers = gr_ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < RS_BITS; bit++) {
-            uint64_t bb = gr_ballot((d >> bit) & 1u);
+            const uint64_t bb = gr_ballot((d >> bit) & 1u);
             peers &= ((d >> bit) & 1u) ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & below);
         const uint32_t cnt = (uint32_t)__popcll(peers);
-        if (valid && rank == cnt - 1) wcnt[wave][d] = cnt;
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
-            skey[pos] = key;
-            sval[pos] = val;
-        }
-        __syncthreads();
-        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        __syncthreads();
+        // the digit's count in this wave so far; the group's last lane then advances it
+        const uint32_t before = valid ? wcnt[wave][d] : 0u;
+        rank_in_wave[r] = before + rank;
+        if (valid && rank == cnt - 1) wcnt[wave][d] = before + cnt;
     }
+    __syncthreads();
+    {  // thread = digit: each wave's first slot for it, in wave order
+        uint32_t acc = lbase[tid];
+#pragma unroll
+        for (uint32_t w = 0; w < WAVES; w++) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint32_t li = wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane;
+        if (li < tn) {
+            const uint32_t d = (uint32_t)(kk[r] >> shift) & (RS_BUCKETS - 1);
+            const uint32_t pos = wcnt[wave][d] + rank_in_wave[r];
+            skey[pos] = kk[r];
+            sval[pos] = vv[r];
+        }
+    }
+    __syncthreads();
     for (uint32_t i = tid; i < tn; i += GR_BLOCK) {
         const uint64_t key = skey[i];
         const uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);

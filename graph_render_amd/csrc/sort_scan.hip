@@ -70,31 +70,51 @@ __global__ __launch_bounds__(GR_BLOCK) void k_scan_partials(uint32_t *partials, 
     if (threadIdx.x == 0 && d_total) *d_total = carry;
 }
 
+// Wave-wide inclusive scan of one value per lane (shuffles, no barriers).
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+// Each wave scans its own SCAN_ITEMS * 64 consecutive values of the tile in rounds of 64,
+// carrying its running total; one exchange of the wave totals then gives every wave its
+// base, so the tile takes one barrier instead of two per round.
 __global__ __launch_bounds__(GR_BLOCK) void k_scan_down(const uint32_t *in, uint32_t *out,
                                                         uint32_t max_n, const uint32_t *d_count,
                                                         const uint32_t *partials) {
-    __shared__ uint32_t lds[8];
-    __shared__ uint32_t carry;
+    constexpr uint32_t WAVES = GR_BLOCK / GR_WAVE;
+    constexpr uint32_t PER_WAVE = SCAN_ITEMS * GR_WAVE;
+    __shared__ uint32_t wtot[WAVES];
     const uint32_t n = load_count(d_count, max_n);
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
     const uint32_t tn = base < n ? (n - base < SCAN_TILE ? (uint32_t)(n - base) : SCAN_TILE) : 0;
     const __amdgpu_buffer_rsrc_t rin = gr_rsrc(in + base, tn * 4);
-    uint32_t vin[SCAN_ITEMS];  // the whole tile in flight before the first block scan
+    const uint32_t wave = threadIdx.x / GR_WAVE, lane = threadIdx.x % GR_WAVE;
+    uint32_t vin[SCAN_ITEMS];  // the wave's values in flight before the first round
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; r++)
-        vin[r] = gr_load_u32(rin, ((uint32_t)r * GR_BLOCK + threadIdx.x) * 4);
-    if (threadIdx.x == 0) carry = partials[blockIdx.x];
-    __syncthreads();
+        vin[r] = gr_load_u32(rin, (wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane) * 4);
+    const uint32_t tile_base = partials[blockIdx.x];
+    uint32_t run = 0, ex[SCAN_ITEMS];
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; r++) {
-        uint64_t i = base + (uint64_t)r * GR_BLOCK + threadIdx.x;
-        uint32_t v = vin[r];
-        uint32_t inc = block_inclusive_scan(v, lds);
-        uint32_t c = carry;
-        if (i < n) out[i] = c + inc - v;
-        __syncthreads();
-        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
-        __syncthreads();
+        const uint32_t inc = wave_inclusive_scan(vin[r]);
+        ex[r] = run + inc - vin[r];
+        run += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) wtot[wave] = run;
+    __syncthreads();
+    uint32_t wbase = tile_base;
+    for (uint32_t w = 0; w < wave; w++) wbase += wtot[w];
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; r++) {
+        const uint64_t i = base + wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane;
+        if (i < n) out[i] = wbase + ex[r];
     }
 }
 

@@ -50,24 +50,43 @@ __global__ __launch_bounds__(GR_BLOCK) void k_scan_reduce(const uint32_t *in, ui
     if (threadIdx.x == 0) partials[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
 }
 
+// Exclusive scan of row[0, cnt) in place by one workgroup, in rounds of 4 * GR_BLOCK values,
+// four consecutive per thread: their local scan, then one block scan of the thread totals.
+// Returns the grand total (in every thread).
+__device__ uint32_t block_scan_row(uint32_t *row, uint32_t cnt, uint32_t *lds) {
+    constexpr uint32_t PER = 4;
+    const __amdgpu_buffer_rsrc_t r = gr_rsrc(row, cnt * 4);
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < cnt; b += PER * GR_BLOCK) {
+        const uint32_t i0 = b + PER * threadIdx.x;
+        uint32_t v[PER], loc = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) v[k] = gr_load_u32(r, (i0 + k) * 4);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t t = v[k];
+            v[k] = loc;
+            loc += t;
+        }
+        const uint32_t inc = block_inclusive_scan(loc, lds);
+        const uint32_t ex = carry + inc - loc;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++)
+            if (i0 + k < cnt) row[i0 + k] = ex + v[k];
+        // the round's total from the wave totals block_inclusive_scan left in lds, read by
+        // every thread before the next round writes them again
+        carry += lds[0] + lds[1] + lds[2] + lds[3];
+        __syncthreads();
+    }
+    return carry;
+}
+
 // One block scans the per-tile partials (exclusive) and writes the grand total.
 __global__ __launch_bounds__(GR_BLOCK) void k_scan_partials(uint32_t *partials, uint32_t nparts,
                                                             uint32_t *d_total) {
     __shared__ uint32_t lds[8];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t b = 0; b < nparts; b += GR_BLOCK) {
-        uint32_t i = b + threadIdx.x;
-        uint32_t v = i < nparts ? partials[i] : 0;
-        uint32_t inc = block_inclusive_scan(v, lds);
-        uint32_t c = carry;
-        if (i < nparts) partials[i] = c + inc - v;
-        __syncthreads();
-        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && d_total) *d_total = carry;
+    const uint32_t total = block_scan_row(partials, nparts, lds);
+    if (threadIdx.x == 0 && d_total) *d_total = total;
 }
 
 // Wave-wide inclusive scan of one value per lane (shuffles, no barriers).
@@ -166,21 +185,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_scan(uint32_t *hist, uint32_t nb,
                                                       uint32_t *totals) {
     __shared__ uint32_t lds[8];
-    __shared__ uint32_t carry;
-    uint32_t *row = hist + (size_t)blockIdx.x * nb;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t b = 0; b < nb; b += GR_BLOCK) {
-        uint32_t i = b + threadIdx.x;
-        uint32_t v = i < nb ? row[i] : 0;
-        uint32_t inc = block_inclusive_scan(v, lds);
-        uint32_t c = carry;
-        if (i < nb) row[i] = c + inc - v;
-        __syncthreads();
-        if (threadIdx.x == GR_BLOCK - 1) carry = c + inc;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    const uint32_t total = block_scan_row(hist + (size_t)blockIdx.x * nb, nb, lds);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
 
 // Stable scatter. A tile's keys are ranked round by round (256 consecutive keys per round),

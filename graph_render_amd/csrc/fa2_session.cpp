@@ -566,10 +566,11 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     // batch tables: node offsets, graph-aligned node chunks, node -> graph
     d.ngraphs = ngraphs;
     {
-        // The sort takes whole 8-bit digits. The key holds the fewest digits that still give
-        // WANT levels (the flag and graph id included); points sharing all of them are
-        // separated by the deep-group builder, which replays the reference insertion below
-        // the last key level, so the tree is the same for any level count.
+        // The sort takes whole digits of radix_bits() bits. The key holds the fewest digits
+        // (at least 30 bits) that still give WANT levels (the flag and graph id included);
+        // points sharing all of them are separated by the deep-group builder, which replays
+        // the reference insertion below the last key level, so the tree is the same for any
+        // level count.
         const uint32_t gbits = ngraphs > 1 ? (uint32_t)std::ceil(std::log2((double)ngraphs)) : 0;
         const uint32_t maxl = d.dim == 2 ? 31 : 21;
         uint32_t biggest = 1;
@@ -577,10 +578,11 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
         // levels to spare beyond the depth an even spread of the biggest graph fills
         const uint32_t fill = (uint32_t)std::ceil(std::log2((double)biggest) / d.dim);
         const uint32_t want = fill + (d.dim == 2 ? 8 : 5);
-        uint32_t digits = 4, levels = 0;
+        const uint32_t rb = (uint32_t)gr::radix_bits(), max_digits = 64 / rb;
+        uint32_t digits = (30 + rb - 1) / rb, levels = 0;
         for (;; digits++) {
-            levels = (8 * digits - 1 - gbits) / d.dim;
-            if (levels >= want || levels >= maxl || digits == 8) break;
+            levels = (rb * digits - 1 - gbits) / d.dim;
+            if (levels >= want || levels >= maxl || digits == max_digits) break;
         }
         d.levels = levels < maxl ? levels : maxl;
         d.key_bits = d.dim * d.levels + gbits + 1;

@@ -17,6 +17,7 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uin
 // (keys_alt, vals_alt) after an odd number.
 size_t radix_tmp_bytes(uint32_t n);
 int radix_digits(int end_bit);
+int radix_bits();  // key bits per sort pass
 void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                       uint32_t n, int end_bit, void *tmp, hipStream_t stream);
 

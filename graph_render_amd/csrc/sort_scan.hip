@@ -159,11 +159,14 @@ constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
 constexpr int RS_BITS = 8;
 constexpr int RS_BUCKETS = 1 << RS_BITS;
+constexpr int RS_DPT = RS_BUCKETS / GR_BLOCK;  // digits per thread in the per-digit steps
+static_assert(RS_BUCKETS % GR_BLOCK == 0, "whole digits per thread");
 
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint32_t n, int shift,
                                                       uint32_t *hist, uint32_t nb) {
     __shared__ uint32_t h[RS_BUCKETS];
-    h[threadIdx.x] = 0;
+#pragma unroll
+    for (int q = 0; q < RS_DPT; q++) h[q * GR_BLOCK + threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     uint32_t dg[RS_ITEMS];  // every key's digit first (all loads in flight), then the counts
@@ -177,7 +180,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint
     for (int r = 0; r < RS_ITEMS; r++)
         if (dg[r] < RS_BUCKETS) atomicAdd(&h[dg[r]], 1u);
     __syncthreads();
-    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < RS_DPT; q++) {
+        const uint32_t dgt = q * GR_BLOCK + threadIdx.x;
+        hist[(size_t)dgt * nb + blockIdx.x] = h[dgt];
+    }
 }
 
 // Block d scans row d of the digit-major histogram in place (exclusive) and writes the
@@ -228,17 +235,33 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, co
         vv[r] = vin[at];
     }
 #pragma unroll
-    for (uint32_t w = 0; w < WAVES; w++) wcnt[w][tid] = 0;
+    for (uint32_t w = 0; w < WAVES; w++)
+#pragma unroll
+        for (int q = 0; q < RS_DPT; q++) wcnt[w][q * GR_BLOCK + tid] = 0;
     {
-        const uint32_t t = totals[tid];
-        const uint32_t h = hist[(size_t)tid * nb + b];
-        const uint32_t hn = hist[(size_t)tid * nb + (b + 1 < nb ? b + 1 : b)];
-        const uint32_t inc = block_inclusive_scan(t, lds);
-        gbase[tid] = inc - t + h;
-        const uint32_t next = b + 1 < nb ? hn : t;
-        const uint32_t cnt = next - h;
-        const uint32_t linc = block_inclusive_scan(cnt, lds);
-        lbase[tid] = linc - cnt;
+        // thread tid takes digits [tid * RS_DPT, (tid + 1) * RS_DPT): the digits' totals and
+        // this tile's counts, scanned locally and then across the block
+        uint32_t t[RS_DPT], h[RS_DPT], c[RS_DPT], tsum = 0, csum = 0;
+#pragma unroll
+        for (int q = 0; q < RS_DPT; q++) {
+            const uint32_t dgt = tid * RS_DPT + q;
+            t[q] = totals[dgt];
+            h[q] = hist[(size_t)dgt * nb + b];
+            const uint32_t hn = hist[(size_t)dgt * nb + (b + 1 < nb ? b + 1 : b)];
+            c[q] = (b + 1 < nb ? hn : t[q]) - h[q];
+            tsum += t[q];
+            csum += c[q];
+        }
+        uint32_t tex = block_inclusive_scan(tsum, lds) - tsum;
+        uint32_t cex = block_inclusive_scan(csum, lds) - csum;
+#pragma unroll
+        for (int q = 0; q < RS_DPT; q++) {
+            const uint32_t dgt = tid * RS_DPT + q;
+            gbase[dgt] = tex + h[q];
+            lbase[dgt] = cex;
+            tex += t[q];
+            cex += c[q];
+        }
     }
     __syncthreads();  // counters cleared (block_inclusive_scan's barriers come before it too)
     const uint64_t below = gr_lanemask_lt();
@@ -262,12 +285,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, co
         if (valid && rank == cnt - 1) wcnt[wave][d] = before + cnt;
     }
     __syncthreads();
-    {  // thread = digit: each wave's first slot for it, in wave order
-        uint32_t acc = lbase[tid];
+#pragma unroll
+    for (int q = 0; q < RS_DPT; q++) {  // per digit: each wave's first slot, in wave order
+        const uint32_t dgt = q * GR_BLOCK + tid;
+        uint32_t acc = lbase[dgt];
 #pragma unroll
         for (uint32_t w = 0; w < WAVES; w++) {
-            const uint32_t c = wcnt[w][tid];
-            wcnt[w][tid] = acc;
+            const uint32_t c = wcnt[w][dgt];
+            wcnt[w][dgt] = acc;
             acc += c;
         }
     }
@@ -297,7 +322,7 @@ size_t radix_tmp_bytes(uint32_t n) {
     return ((size_t)RS_BUCKETS * nb + RS_BUCKETS) * sizeof(uint32_t) + 256;
 }
 
-// Sorts bits [0, end_bit) of the keys, rounded up to whole 8-bit digits, ping-ponging
+// Sorts bits [0, end_bit) of the keys, rounded up to whole RS_BITS digits, ping-ponging
 // between (keys, vals) and (keys_alt, vals_alt): after an even number of digits the result
 // is in (keys, vals), after an odd number in the alternates. Nothing is copied.
 void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
@@ -326,5 +351,6 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32
 }
 
 int radix_digits(int end_bit) { return (end_bit + RS_BITS - 1) / RS_BITS; }
+int radix_bits() { return RS_BITS; }
 
 }  // namespace gr

@@ -753,7 +753,8 @@ __device__ uint32_t run_end(const uint64_t *gkey, uint32_t G, uint64_t key, uint
             uint32_t pr[P];
 #pragma unroll
             for (int u = 0; u < P; u++) {
-                const uint64_t at = (uint64_t)lo + (1ull << (shift + 2 * u < 62 ? shift + 2 * u : 62));
+                const uint32_t sh = shift + 2 * u < 62 ? shift + 2 * u : 62;
+                const uint64_t at = (uint64_t)lo + (1ull << sh);
                 pr[u] = at < G ? (uint32_t)at : G - 1;
             }
             bool m[P];
@@ -980,7 +981,8 @@ __device__ __forceinline__ void com_cell(const FaDev &d, uint32_t k) {
 }
 
 constexpr uint32_t COM_TILE = 4096;
-constexpr uint32_t SPAN_SLOTS = 32;   // spanning cells are ancestors of the tile's end: <= 1 per level
+// spanning cells are ancestors of the tile's end: at most one per level
+constexpr uint32_t SPAN_SLOTS = 32;
 
 // One workgroup per tile of COM_TILE consecutive cells, its sums held in LDS. Leaves (and
 // cells the deep builder already summed) are read in one parallel pass; then every cell
@@ -1436,7 +1438,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         {
             uint32_t w[W];
 #pragma unroll
-            for (uint32_t q = 0; q < W; q++) w[q] = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)q);
+            for (uint32_t q = 0; q < W; q++)
+                w[q] = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)q);
             __builtin_memcpy(&cell, w, sizeof(cell));
         }
         const uint32_t k1 = k + 1;
@@ -2190,10 +2193,9 @@ static void tree_impl(FaDev &d) {
     hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
     hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
-    hipLaunchKernelGGL((k_span_prep<T, D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
-                       dim3(GR_BLOCK), 0, s, d);
-    hipLaunchKernelGGL((k_span_link<D>), dim3(gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK)),
-                       dim3(GR_BLOCK), 0, s, d);
+    const uint32_t span_blocks = gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK);
+    hipLaunchKernelGGL((k_span_prep<T, D>), dim3(span_blocks), dim3(GR_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_span_link<D>), dim3(span_blocks), dim3(GR_BLOCK), 0, s, d);
     // a batch's span cells overflow the one-workgroup LDS pass: it then runs level by level
     // over the whole GPU
     const bool levelwise = d.ngraphs > 1;
@@ -2213,9 +2215,11 @@ static void walk_impl(FaDev &d) {
     if (buf && one)
         hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
     else if (buf)
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false>), grid, dim3(GR_BLOCK), 0, d.stream,
+                           d);
     else
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false, false>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false, false>), grid, dim3(GR_BLOCK), 0, d.stream,
+                           d);
 }
 
 void pm_repulsion(FaDev &d);  // pm.hip

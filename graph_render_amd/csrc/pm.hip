@@ -68,6 +68,11 @@ __global__ void k_pm_frame(const FaDev d) {
         f->oy = floor(g->box_lo[1] / h) * h;
         f->valid = 1;
         f->rebuild = !(f->kvalid && f->kstep == step && f->kreach == reach);
+        // the spectrum launches that follow read only `rebuild`: the rung they build it for
+        // is recorded here
+        f->kstep = step;
+        f->kreach = reach;
+        f->kvalid = 1;
         return;
     }
 }
@@ -114,14 +119,6 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pm_sample(const FaDev d) {
         }
         d.pm_b[q] = g;
     }
-}
-
-__global__ void k_pm_commit(const FaDev d) {
-    PmFrame *f = (PmFrame *)d.pm_fr;
-    if (!f->valid || !f->rebuild) return;
-    f->kstep = f->step;
-    f->kreach = f->reach;
-    f->kvalid = 1;
 }
 
 // ---------------------------------------------------------------- FFT
@@ -444,7 +441,6 @@ void pm_repulsion(FaDev &d) {
     hipLaunchKernelGGL(k_pm_transpose, dim3(tiles, tiles), dim3(GR_BLOCK), 0, s, d,
                        (const float2 *)d.pm_b, d.pm_k, 0, 0, 1);
     hipLaunchKernelGGL(k_pm_lines, dim3(P), dim3(GR_BLOCK), lds, s, d, d.pm_k, 0, -2);
-    hipLaunchKernelGGL(k_pm_commit, dim3(1), dim3(1), 0, s, d);
     // density, forward along x (rows with data), transpose, y-lines forward * K inverse,
     // transpose back (rows read), inverse along x, CIC read
     {

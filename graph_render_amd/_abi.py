@@ -76,6 +76,9 @@ SIGNATURES = {
     "gr_pm_side_for": (C.c_uint32, [C.c_uint32]),
     "gr_pm_charge": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
+    "gr_pm_charge_noalloc": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                       C.c_void_p]),
     "gr_snapshot_encode": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,

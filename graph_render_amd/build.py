@@ -8,13 +8,14 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libgraph_render_fa2.so")
 SOURCES = ["fa2_session.cpp", "fa2_kernels.hip", "sort_scan.hip", "pm.hip", "adjoint.hip",
-           "snapshot.cpp"]
+           "snapshot.cpp", "noalloc.hip"]
 ARCH = os.environ.get("GR_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: FP64 kernels must round as the reference's Rust does (no FMA fusion).
@@ -30,16 +31,22 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [hipcc, *[f for f in FLAGS if f != "-shared"], "-c", *lang,
+        cmd = [hipcc, *[f for f in FLAGS if f != "-shared"], "-c", "-x", "hip",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as pool:
+        for cmd, res in zip(cmds, pool.map(
+                lambda c: subprocess.run(c, capture_output=True, text=True), cmds)):
+            sys.stderr.write(res.stderr)
+            if res.returncode != 0:
+                raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
     tmp = LIB + ".tmp"
     cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
     if verbose:

@@ -151,6 +151,10 @@ void launch_tree(FaDev &d);
 void launch_walk(FaDev &d);
 // Clears the halt flag left by a failed capacity check.
 void launch_clear_halt(FaDev &d);
+// One particle-mesh charge pass of a 2D FP64 single-graph state whose planar positions,
+// masses and tables are in place: bounds partials, the graph's extent, then the mesh
+// (pm.hip), leaving the field times the law's strength in d.rep.
+void launch_pm_charge(FaDev &d);
 
 // ---- reverse mode (adjoint.hip) ----
 

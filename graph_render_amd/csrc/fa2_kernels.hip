@@ -2317,5 +2317,12 @@ void launch_walk(FaDev &dd) {
 void launch_clear_halt(FaDev &d) {
     hipLaunchKernelGGL(k_clear_halt, dim3(1), dim3(1), 0, d.stream, d);
 }
+void launch_pm_charge(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    bounds_impl<double, 2, false>(d);
+    head_impl<2>(d);
+    pm_repulsion(d);
+}
 
 }  // namespace gr

@@ -218,6 +218,20 @@ gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charg
                        double distance_min, double distance_max, double *out,
                        double *frame_out);
 
+/* gr_pm_charge without device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, hosts that forbid hipMalloc). Same arguments, checks and result bits as
+ * gr_pm_charge, but every device buffer lives in one array inside the library (64 MiB per
+ * GPU, held until the process ends), so the sizes are capped: n <= GR_NOALLOC_MAX_NODES and
+ * a mesh side <= GR_NOALLOC_MAX_SIDE; larger calls are refused with GR_ERR_CAPACITY. Calls
+ * on one GPU share that array: they are serialised inside one process, and calls that
+ * overlap on one GPU from several processes race. Runs on the null stream. */
+#define GR_NOALLOC_MAX_NODES 262144u
+#define GR_NOALLOC_MAX_SIDE 1024u
+gr_status gr_pm_charge_noalloc(uint32_t n, const double *xy, uint32_t side, double charge,
+                               double alpha, double distance_min, double distance_max,
+                               double *out, double *frame_out);
+
 /* Last failure of the calling thread: code, parameter/column name, rule text. */
 gr_status gr_last_error(const char **name, const char **rule);
 

@@ -228,7 +228,7 @@ def bench_adjoint(args):
     each other is what says the finite difference measures a derivative at all."""
     import torch
     import graph_render_amd as G
-    from graph_render_amd import autograd as AG
+    from tests import fa2_autograd as AG
     from graph_render_amd import synth
     n = args.nodes or 10_000
     s, t = synth.sbm_equal(n, 16, 0.01, 0.0005, seed=3)

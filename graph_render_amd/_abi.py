@@ -27,6 +27,7 @@ GR_PREC_FP64, GR_PREC_FP32, GR_PREC_FP32_KAHAN = 0, 1, 2
 GR_REP_DEFAULT, GR_REP_TREE, GR_REP_DENSE, GR_REP_PM = 0, 1, 2, 3
 GR_EXT_EXPORT_TREE = 1
 GR_EXT_DIRECT_LAUNCH = 2
+GR_EXT_BLOCK_SUMS = 4
 
 
 class GrFa2Params(C.Structure):

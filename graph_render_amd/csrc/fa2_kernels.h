@@ -128,6 +128,10 @@ struct FaDev {
     double pm_dmin2 = 0.0, pm_dmax2 = 0.0, pm_strength = 0.0;
 
     int ignore_done = 0;  // set on force evaluations, which run after an early exit too
+    // The global sums (centroid, swing, traction, moved) added one term after another in
+    // node order, as the reference does, instead of by block reductions: FP64 trajectories
+    // are then the reference's bit for bit.
+    int seq_sums = 0;
 
     hipStream_t stream = nullptr;
 };

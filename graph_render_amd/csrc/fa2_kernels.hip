@@ -1964,6 +1964,71 @@ __global__ __launch_bounds__(GR_BLOCK, (fin_waves<T, D, KAHAN>())) void k_finish
     }
 }
 
+// ------------------------------------------------------------------ sequential sums
+
+// The reference's sums in its own order (state.rs:208-220, schedule.rs:16-30 and :62-77):
+// each one a chain from 0.0 adding the nodes' terms one after another in ascending node
+// order. The block stages SEQ_TILE nodes' terms in LDS with coalesced loads, then thread 0
+// adds them in order; the chains of one sum kind run side by side in that thread. The
+// result is valid in thread 0.
+constexpr uint32_t SEQ_TILE = 2048;
+
+// K interleaved doubles per node at src[K * v + k]: sums k = 0..K-1 over v in [v0, v1)
+template <int K>
+__device__ void seq_sum_nodes(const double *src, uint32_t v0, uint32_t v1, double (&s)[K]) {
+    __shared__ double tile[K * SEQ_TILE];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (uint32_t base = v0; base < v1; base += SEQ_TILE) {
+        const uint32_t cnt = v1 - base < SEQ_TILE ? v1 - base : SEQ_TILE;
+        for (uint32_t q = threadIdx.x; q < K * cnt; q += GR_BLOCK)
+            tile[q] = src[(size_t)K * base + q];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (uint32_t q = 0; q < cnt; q++) {
+#pragma unroll
+                for (int k = 0; k < K; k++) s[k] = s[k] + tile[K * q + k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The centroid as state.rs:208-220 forms it (one block per graph), over the planar
+// positions; it replaces the block-reduced one k_head_graph wrote.
+template <typename T, int D>
+__global__ __launch_bounds__(GR_BLOCK) void k_seq_mean(const FaDev d) {
+    if (halted(d)) return;
+    const uint32_t b = blockIdx.x;
+    const uint32_t v0 = d.gnode[b], v1 = d.gnode[b + 1];
+    __shared__ double tile[D][SEQ_TILE];
+    double s[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) s[a] = 0.0;
+    for (uint32_t base = v0; base < v1; base += SEQ_TILE) {
+        const uint32_t cnt = v1 - base < SEQ_TILE ? v1 - base : SEQ_TILE;
+        for (uint32_t q = threadIdx.x; q < cnt; q += GR_BLOCK) {
+#pragma unroll
+            for (int a = 0; a < D; a++) tile[a][q] = (double)((const T *)d.pos[a])[base + q];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (uint32_t q = 0; q < cnt; q++) {
+#pragma unroll
+                for (int a = 0; a < D; a++) s[a] = s[a] + tile[a][q];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double nb = (double)(v1 - v0);
+#pragma unroll
+        for (int a = 0; a < D; a++) d.gr[b].mean[a] = s[a] / nb;
+    }
+}
+
 // swing and traction partial sums per node chunk (chunks never cross graphs), fixed order
 __global__ __launch_bounds__(GR_BLOCK) void k_stats(const FaDev d) {
     if (halted(d)) return;
@@ -2002,12 +2067,16 @@ __global__ __launch_bounds__(GR_BLOCK) void k_estimate(const FaDev d) {
     if (st->done) return;  // block-uniform
     __shared__ double lds[4 * 2];
     double v[2] = {0.0, 0.0};
-    for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
-         c += GR_BLOCK) {
-        v[0] = v[0] + d.spart[2 * c];
-        v[1] = v[1] + d.spart[2 * c + 1];
+    if (d.seq_sums) {  // block-uniform
+        seq_sum_nodes<2>(d.nodeval, d.gnode[blockIdx.x], d.gnode[blockIdx.x + 1], v);
+    } else {
+        for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
+             c += GR_BLOCK) {
+            v[0] = v[0] + d.spart[2 * c];
+            v[1] = v[1] + d.spart[2 * c + 1];
+        }
+        block_reduce<2, 0>(v, lds);
     }
-    block_reduce<2, 0>(v, lds);
     if (threadIdx.x != 0) return;
     st->swing += v[0];
     st->traction += v[1];
@@ -2099,6 +2168,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_apply(const FaDev d) {
                 step += fabs(dd);
             }
             moved += step;
+            // the node's step for k_tail's sequential sum (its swing term is spent)
+            if (d.seq_sums) d.nodeval[2 * (size_t)v] = step;
             bounds_take<D>(p, lo, hi, sum, nfin);
         }
     }
@@ -2116,10 +2187,16 @@ __global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
     if (!st->done) {  // block-uniform
         __shared__ double lds[4];
         double v[1] = {0.0};
-        for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
-             c += GR_BLOCK)
-            v[0] = v[0] + d.mpart[c];
-        block_reduce<1, 0>(v, lds);
+        if (d.seq_sums) {  // block-uniform: each node's step, left in nodeval by k_apply
+            double sv[2];
+            seq_sum_nodes<2>(d.nodeval, d.gnode[blockIdx.x], d.gnode[blockIdx.x + 1], sv);
+            v[0] = sv[0];
+        } else {
+            for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
+                 c += GR_BLOCK)
+                v[0] = v[0] + d.mpart[c];
+            block_reduce<1, 0>(v, lds);
+        }
         if (threadIdx.x == 0) {
             st->moved = v[0];
             st->iters_done += 1;
@@ -2162,9 +2239,11 @@ __global__ void k_clear_halt(const FaDev d) {
 
 // ------------------------------------------------------------------ launchers
 
-template <int D>
+template <typename T, int D>
 static void head_impl(FaDev &d) {
     hipLaunchKernelGGL((k_head_graph<D>), dim3(d.ngraphs), dim3(GR_BLOCK), 0, d.stream, d);
+    if (d.seq_sums)
+        hipLaunchKernelGGL((k_seq_mean<T, D>), dim3(d.ngraphs), dim3(GR_BLOCK), 0, d.stream, d);
     if (d.ngraphs > 1)
         hipLaunchKernelGGL(k_head_global, dim3(1), dim3(GR_BLOCK), 0, d.stream, d);
 }
@@ -2174,7 +2253,7 @@ static void tree_impl(FaDev &d) {
     const uint32_t n = d.n;
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
     hipStream_t s = d.stream;
-    head_impl<D>(d);
+    head_impl<T, D>(d);
     // keys are written where an odd number of sort passes starts, so the sorted pairs
     // always end in (d.keys, d.vals)
     FaDev dk = d;
@@ -2233,7 +2312,7 @@ static void forces_impl(FaDev &d, uint32_t parts) {
         tree_impl<T, D>(d);
         if (parts & PART_REPULSION) walk_impl<T, D, KAHAN>(d);
     } else if (d.repulsion == REP_DENSE) {
-        head_impl<D>(d);
+        head_impl<T, D>(d);
         if (parts & PART_ATTRACTION)
             hipLaunchKernelGGL((k_attract_only<T, D, KAHAN>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
         if (parts & PART_REPULSION) {
@@ -2244,10 +2323,10 @@ static void forces_impl(FaDev &d, uint32_t parts) {
         }
         mode = 1;
     } else if (d.repulsion == REP_PM) {
-        head_impl<D>(d);
+        head_impl<T, D>(d);
         if (parts & PART_REPULSION) pm_repulsion(d);
     } else {
-        head_impl<D>(d);
+        head_impl<T, D>(d);
         parts &= ~(uint32_t)PART_REPULSION;
     }
     if (mode == 0 && (parts & PART_ATTRACTION) && d.nlong > 0)
@@ -2321,7 +2400,7 @@ void launch_pm_charge(FaDev &dd) {
     FaDev d = dd;
     d.ignore_done = 1;
     bounds_impl<double, 2, false>(d);
-    head_impl<2>(d);
+    head_impl<double, 2>(d);
     pm_repulsion(d);
 }
 

@@ -146,7 +146,8 @@ struct gr_fa2_session {
     }
 
     // Enqueues one iteration: the captured graph when there is one, else direct launches.
-    void iteration() {
+    // Returns the launch's status (a direct launch's errors surface in hipGetLastError).
+    hipError_t iteration() {
         if (use_graph && !it_exec) {
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
@@ -161,10 +162,9 @@ struct gr_fa2_session {
             (void)hipGetLastError();
             if (!it_exec) use_graph = false;  // capture unavailable: launch directly
         }
-        if (it_exec)
-            (void)hipGraphLaunch(it_exec, d.stream);
-        else
-            gr::launch_iteration(d);
+        if (it_exec) return hipGraphLaunch(it_exec, d.stream);
+        gr::launch_iteration(d);
+        return hipSuccess;
     }
 
     // The last gr_fa2_session_record(): state checkpoints at the start of each segment of
@@ -514,6 +514,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     d.jitter_tolerance = p.jitter_tolerance;
     d.seed = p.seed;
     d.export_bounds = (x.flags & GR_EXT_EXPORT_TREE) ? 1 : 0;
+    d.seq_sums = (x.precision == GR_PREC_FP64 && !(x.flags & GR_EXT_BLOCK_SUMS)) ? 1 : 0;
     s->use_graph = (x.flags & GR_EXT_DIRECT_LAUNCH) == 0;
     d.pm_side = x.pm_grid;
     (void)hipGetDevice(&s->device);
@@ -789,9 +790,14 @@ static gr_status run_iters(gr_fa2_session *s, uint32_t iters, uint32_t *applied)
     int recoveries = 0;
     while (remaining > 0 && !st.done) {
         uint32_t k = remaining < batch ? remaining : batch;
-        for (uint32_t q = 0; q < k; q++) s->iteration();
+        const uint32_t before = st.iters_done;
+        for (uint32_t q = 0; q < k; q++) HIP_TRY(s->iteration());
         HIP_TRY(hipGetLastError());
         if ((r = s->read_state(st))) return r;
+        // launches that applied nothing, with neither a halt nor an early exit to say why,
+        // would otherwise be repeated forever
+        if (!st.halt && !st.done && st.iters_done == before)
+            return fail(GR_ERR_DEVICE, "iteration", "launched iterations applied nothing");
         if (st.halt) {
             if (++recoveries > 8) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
             if ((r = s->recover(st))) return r;
@@ -939,7 +945,10 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
     };
     gr::AdjDev a;
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
-    const size_t every = rec.every;
+    // the tape holds one segment: size it by the longest one recorded (checkpoint_every
+    // may exceed the iterations run)
+    size_t every = 1;
+    for (uint32_t L : rec.seg_len) every = std::max<size_t>(every, L);
     const size_t pb = s->pos_bytes();
     void *tape_pos = nullptr, *tape_g = nullptr, *tape_st = nullptr, *cacc = nullptr;
     uint32_t cacc_cap = 0;
@@ -986,7 +995,7 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
             for (uint32_t j = 0; j < L; j++) {
                 if ((r = s->save_state((char *)tape_pos + (size_t)j * pb, tg + j, ts + j)))
                     return r;
-                s->iteration();
+                HIP_TRY(s->iteration());
             }
             HIP_TRY(hipMemcpyAsync(tg + L, d.gr, sizeof(GrGraph), hipMemcpyDeviceToDevice,
                                    d.stream));
@@ -1132,7 +1141,12 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
     if (ncells_out) *ncells_out = C;
     if (order && s->d.n)
         HIP_TRY(hipMemcpy(order, s->d.order, (size_t)s->d.n * 4, hipMemcpyDeviceToHost));
-    if (C == 0 || cap < C) return ok();
+    if (C == 0) return ok();
+    if (cap < C) {  // a size query (no cell output) is the first call of two
+        if (bounds || skip || start || end || bodies)
+            return fail(GR_ERR_CAPACITY, "cap", "cap >= *ncells_out cells in every output");
+        return ok();
+    }
     if (bounds) {
         if (!s->d.export_bounds)
             return fail(GR_ERR_UNSUPPORTED, "bounds", "create the session with GR_EXT_EXPORT_TREE");
@@ -1194,7 +1208,7 @@ gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t rep
     for (uint32_t q = 0; q < reps; q++) {
         if (walk) gr::launch_walk(s->d);
         if (tree) gr::launch_tree(s->d);
-        if (it) s->iteration();
+        if (it) HIP_TRY(s->iteration());
         if (pm) gr::launch_forces(s->d, 2);
     }
     HIP_TRY(hipEventRecord(e1, s->d.stream));
@@ -1238,7 +1252,7 @@ gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charg
     x.precision = GR_PREC_FP64;
     x.pm_grid = side;
     x.start = xy;
-    x.flags = GR_EXT_DIRECT_LAUNCH;
+    x.flags = GR_EXT_DIRECT_LAUNCH | GR_EXT_BLOCK_SUMS;  // the mesh reads no global sum
     gr_fa2_session *s = gr_fa2_session_create(n, nullptr, nullptr, 0, &p, &x);
     if (!s) return g_err.code;
     gr::FaDev &d = s->d;
@@ -1268,6 +1282,9 @@ gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *sourc
                         const uint32_t *target, size_t m, const uint8_t *params,
                         size_t params_len, const gr_fa2_ext *ext, float *out_x, float *out_y,
                         float *out_z) {
+    // index.rs: the u32 index space holds at most u32::MAX - 1 nodes; no output buffer of
+    // this length exists to clear
+    if (n >= 0xFFFFFFFFu) return fail(GR_ERR_CAPACITY, "nodes", "node count fits the u32 index space");
     auto zero = [&]() {
         if (out_x) std::memset(out_x, 0, (size_t)n * 4);
         if (out_y) std::memset(out_y, 0, (size_t)n * 4);

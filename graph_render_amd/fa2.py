@@ -123,13 +123,16 @@ class Fa2Ext:
     pm_grid: int = 0
     export_tree: bool = False
     direct_launch: bool = False            # True: no HIP-graph replay of iterations
+    block_sums: bool = False               # FP64: block-reduced global sums (faster, not
+                                           # the reference's rounding order)
     _keep: list = field(default_factory=list, repr=False)
 
     def _c(self, n: int, m: int, dim: int) -> A.GrFa2Ext:
         x = A.GrFa2Ext()
         x.struct_size = C.sizeof(A.GrFa2Ext)
         x.flags = ((A.GR_EXT_EXPORT_TREE if self.export_tree else 0) |
-                   (A.GR_EXT_DIRECT_LAUNCH if self.direct_launch else 0))
+                   (A.GR_EXT_DIRECT_LAUNCH if self.direct_launch else 0) |
+                   (A.GR_EXT_BLOCK_SUMS if self.block_sums else 0))
         x.theta = -1.0 if self.theta is None else float(self.theta)
         x.precision = PRECISIONS[self.precision]
         x.repulsion = REPULSIONS[self.repulsion]

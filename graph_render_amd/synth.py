@@ -173,3 +173,32 @@ def batch_workload(count: int = 4096, lo: int = 50, hi: int = 20000):
         dsts.append(t)
         starts.append(rng.uniform(-100.0, 100.0, (n, 2)))
     return sizes, srcs, dsts, np.concatenate(starts)
+
+
+# The named workloads of the benchmark configurations. Each returns the graph (raw edge
+# columns in edge-index order), its edge weights (None: unit), the start positions (None:
+# the reference's Mulberry32 starts from `seed`), the opening angle and the parameters.
+
+def workload(name: str) -> dict:
+    if name == "ba2000":  # preferential attachment, 2000 nodes, m = 5, seed 0; FP64
+        s, t = barabasi_albert(2000, 5, seed=0)
+        return dict(name=name, n=2000, dim=2, source=s, target=t, weights=None,
+                    start=uniform_start(2000, 2, seed=0), seed=0, theta=1.2,
+                    precision="fp64", iterations=100)
+    if name == "sbm1m":  # 1M-node SBM, 64 Zipf(1.1) blocks, mean degree 10, seed 1; FP32
+        n = 1_000_000
+        s, t, w = sbm_zipf(n, 64, 1.1, 10.0, seed=1)
+        return dict(name=name, n=n, dim=2, source=s, target=t, weights=w,
+                    start=uniform_start(n, 2, seed=1), seed=0, theta=1.2, precision="fp32",
+                    iterations=500)
+    if name == "rgg250k":  # 3D random geometric graph, unit cube, mean degree 8, seed 2
+        n = 250_000
+        s, t, _ = random_geometric_3d(n, 8.0, 2)
+        return dict(name=name, n=n, dim=3, source=s, target=t, weights=None, start=None,
+                    seed=2, theta=0.8, precision="fp32", iterations=50)
+    if name == "sbm10k":  # 16 equal blocks, p_in 0.01, p_out 0.0005, seed 3 (the adjoint's)
+        n = 10_000
+        s, t = sbm_equal(n, 16, 0.01, 0.0005, seed=3)
+        return dict(name=name, n=n, dim=2, source=s, target=t, weights=None, start=None,
+                    seed=0, theta=0.8, precision="fp32", iterations=200)
+    raise KeyError(name)

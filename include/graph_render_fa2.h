@@ -54,7 +54,9 @@ typedef struct gr_fa2_params {
 } gr_fa2_params;
 
 typedef enum gr_precision {
-    GR_PREC_FP64 = 0,       /* the reference's arithmetic, bit for bit */
+    GR_PREC_FP64 = 0,       /* the reference's arithmetic, bit for bit: trees, per-node forces,
+                               the global sums in node order, steps, trajectories (see
+                               GR_EXT_BLOCK_SUMS for the faster, reassociated sums) */
     GR_PREC_FP32 = 1,       /* FP32 forces; FP64 centres of mass and global reductions */
     GR_PREC_FP32_KAHAN = 2  /* as FP32, with compensated per-body accumulation */
 } gr_precision;
@@ -69,6 +71,12 @@ typedef enum gr_repulsion {
 #define GR_EXT_EXPORT_TREE 1u   /* keep cell bounds for gr_fa2_session_tree() */
 #define GR_EXT_DIRECT_LAUNCH 2u /* launch each iteration's kernels one by one instead of
                                    replaying the iteration as a captured HIP graph */
+#define GR_EXT_BLOCK_SUMS 4u    /* FP64: form the global sums (centroid, swing, traction,
+                                   moved) by block reductions instead of the reference's
+                                   sequential node-order sums; faster on large graphs, and
+                                   the trajectory then agrees with the reference to rounding
+                                   (~1e-12 per step, which FA2 amplifies), not bit for bit.
+                                   FP32 precisions always use block reductions. */
 
 /* Extensions the reference does not have. A NULL gr_fa2_ext pointer means exactly the
  * reference's semantics: theta 0.8 (barnes_hut.rs:24), FP64, unit edge weights, Mulberry32
@@ -158,7 +166,9 @@ gr_status gr_fa2_session_graph(gr_fa2_session *s, uint32_t *m_out, uint32_t *lo,
 
 /* The tree of the last force evaluation or iteration (GR_REP_TREE only). Any output pointer
  * may be NULL. bounds: 2*dim f64 per cell (lo axes, then hi axes; needs GR_EXT_EXPORT_TREE);
- * bodies: (mass, centre...) f64 per cell; order: n tree positions -> point index. */
+ * bodies: (mass, centre...) f64 per cell; order: n tree positions -> point index. Two calls:
+ * the first with cap 0 and no cell output gives *ncells_out; a call whose cell outputs hold
+ * fewer than *ncells_out entries (cap) is refused with GR_ERR_CAPACITY. */
 gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t cap,
                               double *bounds, uint32_t *skip, uint32_t *start, uint32_t *end,
                               double *bodies, uint32_t *order);

@@ -1,4 +1,6 @@
 """torch.autograd wrapper around the device adjoint, and the stress loss it is checked with.
+Test and measurement harness: the product is the C ABI's record/adjoint pair; this module
+only drives it.
 
 `Fa2Unrolled.apply(weights, scaling_ratio, gravity, spec)` runs `spec.iters` FA2 iterations
 on the device (gr_fa2_session_record) and returns the final positions; backward() is the
@@ -18,7 +20,7 @@ from typing import Optional
 
 import numpy as np
 
-from .fa2 import Fa2Ext, Fa2Params, Session, Topology
+from graph_render_amd.fa2 import Fa2Ext, Fa2Params, Session, Topology
 
 
 @dataclass

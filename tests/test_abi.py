@@ -7,6 +7,7 @@ import os
 import re
 import struct
 
+import numpy as np
 import pytest
 
 import graph_render_amd as G
@@ -103,3 +104,43 @@ def test_an_edge_past_n_is_refused():
     with pytest.raises(G.ParamError) as e:
         G.Session(G.Topology(3, [0, 1], [1, 3]))
     assert e.value.name == "edges"
+
+
+# ---- StageError::Capacity through the C ABI (stage.rs:35-37, arena.rs:68-77): refused on
+# the counts alone, before any array is read or any device memory is touched
+
+
+def test_capacity_edges_past_the_index_space():
+    src = np.zeros(4, np.uint32)
+    x = (C.c_float * 4)()
+    m = (1 << 31) + 1  # 2m CSR entries would not fit u32; the arrays are never read
+    code = A.lib().gr_layout_run(b"layout.forceatlas2.barnes_hut", 4, src.ctypes.data,
+                                 src.ctypes.data, m, None, 0, None, x, x, None)
+    c, name, rule = A.last_error()
+    assert code == c == A.GR_ERR_CAPACITY and name == "edges"
+    assert list(x) == [0.0] * 4  # a refused run leaves no stale output
+
+
+def test_capacity_nodes_past_the_index_space():
+    code = A.lib().gr_layout_run(b"layout.forceatlas2", 0xFFFFFFFF, None, None, 0, None, 0,
+                                 None, None, None, None)
+    assert code == A.GR_ERR_CAPACITY and A.last_error()[1] == "nodes"
+    counts = np.array([0x80000000, 0x80000000], np.uint32)
+    edges = np.zeros(2, np.uint64)
+    p = A.GrFa2Params()
+    A.lib().gr_fa2_params_default(C.byref(p))
+    h = A.lib().gr_fa2_batch_create(2, counts.ctypes.data, edges.ctypes.data, None, None,
+                                    C.byref(p), None)
+    assert not h and A.last_error()[0] == A.GR_ERR_CAPACITY
+    with pytest.raises(G.CapacityError):
+        G.fa2._raise()
+
+
+def test_noalloc_charge_pass_refuses_sizes_past_its_array():
+    from graph_render_amd import pm
+    with pytest.raises(G.CapacityError) as e:
+        pm.charge_pass(np.zeros((pm.MAX_NODES_NOALLOC + 1, 2)), 0, allocate=False)
+    assert e.value.name == "n"
+    with pytest.raises(G.CapacityError) as e:
+        pm.charge_pass(np.zeros((10, 2)), 2 * pm.MAX_SIDE_NOALLOC, allocate=False)
+    assert e.value.name == "side"

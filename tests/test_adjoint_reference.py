@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from graph_render_amd.autograd import stress1, stress1_np, stress_sources
+from tests.fa2_autograd import stress1, stress1_np, stress_sources
 from tests import torch_fa2
 
 
@@ -77,7 +77,7 @@ def test_torch_gradient_matches_its_central_differences():
     ([0.0, 1.0, 100.0, 101.0], [0.0] * 4, [0, 2], [1, 3], 0.0),     # two components
 ])
 def test_stress_matches_the_reference_cases(xs, ys, src, dst, want):
-    from graph_render_amd.autograd import bfs_hops
+    from tests.fa2_autograd import bfs_hops
     n = len(xs)
     pos = np.stack([xs, ys], 1).astype(np.float64)
     srcs = stress_sources(n)

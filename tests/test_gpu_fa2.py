@@ -397,3 +397,88 @@ def test_set_params_refuses_out_of_range_values():
         with pytest.raises(G.StageError):
             sess.set_params(G.Fa2Params(dim=3))
         sess.step(2)  # still usable
+
+
+# ------------------------------------------------------------------ controls and errors
+
+
+def test_negative_control_one_ulp_of_scaling_ratio_goes_red(oracle):
+    """The bit-exact comparisons above would catch a one-ulp change of a parameter: with
+    the scaling ratio one ulp above 2 the repulsion differs from the oracle's (by about one
+    ulp), and with 2 it is equal."""
+    n = 3000
+    s, t = bh_graph(n)
+    ref = oracle_state(oracle, n, s, t)
+    ref.run(3)
+    p = ref.positions()
+    r = ref.forces(2)
+    with session(n, s, t, start=p) as good:
+        assert np.array_equal(good.forces(2), r)
+    with session(n, s, t, start=p, scaling_ratio=float(np.nextafter(2.0, 3.0))) as bad:
+        f = bad.forces(2)
+    assert not np.array_equal(f, r)
+    assert np.abs(f - r).max() <= 1e-14 * np.abs(r).max()
+
+
+def test_device_starts_are_mulberry32_at_any_seed(oracle):
+    e = np.zeros(0, np.uint32)
+    for seed, dim in ((7, 2), (123456789, 3), (0xFFFFFFFF, 2)):
+        with G.Session(G.Topology(40, e, e), G.Fa2Params(seed=seed, dim=dim)) as s:
+            assert np.array_equal(s.positions(), oracle.initial_positions(40, seed, dim))
+
+
+def test_non_finite_run_is_refused_as_node_x_and_clears_the_output(oracle):
+    """forceatlas2.rs:86-88: a run whose positions end non-finite is StageError::NonFinite
+    {column: "node.x"}, and gr_layout_run leaves no stale output (the columns are zeroed)."""
+    import ctypes as C
+    from graph_render_amd import _abi as A
+    n = 50
+    s, t = line(n)
+    start = np.random.default_rng(1).uniform(-1.0, 1.0, (n, 2))
+    start[0, 0] = np.nan  # settle() reads node 0's coordinates (the reference's quirk)
+    ref = oracle_state(oracle, n, s, t, start=start)
+    ref.run(3)
+    assert not np.isfinite(ref.positions()[0]).all()
+    with pytest.raises(G.NonFiniteError) as e:
+        G.ForceAtlas2BarnesHut.run(G.Topology(n, s, t), G.Fa2Params(max_iter=3),
+                                   G.Fa2Ext(start=start))
+    assert e.value.code == A.GR_ERR_NON_FINITE and e.value.name == "node.x"
+    x = np.ones(n, np.float32)
+    y = np.ones(n, np.float32)
+    ext = G.Fa2Ext(start=start)._c(n, s.size, 2)
+    code = A.lib().gr_layout_run(b"layout.forceatlas2.barnes_hut", n, s.ctypes.data,
+                                 t.ctypes.data, s.size, None, 0, C.addressof(ext),
+                                 x.ctypes.data, y.ctypes.data, None)
+    assert code == A.GR_ERR_NON_FINITE and not x.any() and not y.any()
+
+
+def test_layout_forceatlas2_3d_stage_tracks_the_oracle(oracle):
+    """The registry id layout.forceatlas2.3d (run_3d, dense, dim 3) through gr_layout_run,
+    against the oracle's 3D arm (pinned to the 2D arm in tests/test_oracle_pins.py)."""
+    n = 400
+    s, t = oracle.gate_edges(n)
+    geo = G.run_3d(G.Topology(n, s, t), G.Fa2Params(max_iter=2))
+    assert geo.dim == 3 and geo.z is not None
+    ref = oracle_state(oracle, n, s, t, dim=3, tree=False)
+    ref.run(2)
+    r = ref.positions()
+    ours = np.column_stack([geo.x, geo.y, geo.z]).astype(np.float64)
+    ext = (r.max(0) - r.min(0)).max()
+    assert np.abs(ours - r).max() <= 1e-7 * ext
+    assert np.array_equal(ours, r.astype(np.float32).astype(np.float64))
+
+
+def test_tree_export_refuses_a_short_buffer():
+    import ctypes as C
+    from graph_render_amd import _abi as A
+    n = 500
+    s, t = bh_graph(n)
+    with session(n, s, t) as sess:
+        sess.forces(2)
+        nc = C.c_uint32(0)
+        assert A.lib().gr_fa2_session_tree(sess._h, C.byref(nc), 0, None, None, None, None,
+                                           None, None) == A.GR_OK
+        short = np.zeros(max(nc.value - 1, 1), np.uint32)
+        code = A.lib().gr_fa2_session_tree(sess._h, C.byref(nc), nc.value - 1, None,
+                                           short.ctypes.data, None, None, None, None)
+        assert code == A.GR_ERR_CAPACITY

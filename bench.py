@@ -33,11 +33,19 @@ def errors(approx, exact):
 
 
 def walk_bytes(n: int, ncells: int, precision: str) -> int:
-    """Compulsory bytes of one walk launch: per body its sorted position and mass, its
-    point index and its force write; per cell one read of its walk record."""
-    es = 8 if precision == "fp64" else 4
-    cell = 48 if precision == "fp64" else 36  # FP32 cells carry a float-pair centre
-    return n * (3 * es + 4 + 2 * es) + ncells * cell
+    """Algorithmic bytes of one walk launch, the per-unit figures of the FA2-BH byte model
+    (2D, FP32 forces): per body 20 B (its sorted position, mass and point index read, its
+    force written) and per cell 48 B (one read of its record); doubled float sizes in FP64."""
+    if precision == "fp64":
+        return n * 36 + ncells * 72
+    return n * 20 + ncells * 48
+
+
+def iteration_bytes(n: int, ncells: int, m_simple: int) -> int:
+    """Algorithmic bytes of one whole FA2-BH iteration (2D, FP32 forces): keys 20n, sort
+    24n, build 8n + 32C, centres of mass 12n + 28C, walk 20n + 48C, attraction
+    4(n + 1) + 16 * 2m + 24n, centroid 8n, swing/traction 20n, apply 28n."""
+    return 168 * n + 108 * ncells + 32 * m_simple + 4
 
 
 def walk_traffic(n: int):
@@ -52,14 +60,40 @@ def walk_traffic(n: int):
     return None
 
 
-def cpu_baseline(topo, params, start, weights, budget_s: float):
+def cpu_info() -> dict:
+    """The host CPU as the baseline states it: model, logical cores the process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_cores": len(os.sched_getaffinity(0)),
+            "host_cpus_total": os.cpu_count()}
+
+
+def pin_one_core() -> int:
+    """Pins the calling thread (the one that runs the oracle) to the first core it may use:
+    the reference path is single-threaded, so its baseline runs on one core."""
+    core = sorted(os.sched_getaffinity(0))[0]
+    os.sched_setaffinity(0, {core})
+    return core
+
+
+def cpu_baseline(w, params, budget_s: float):
     """The oracle (the reference's FA2-BH restated in C, FP64, one thread) on the same graph
-    and start, iterated until about budget_s seconds of CPU time are spent."""
+    and start, pinned to one core, iterated until about budget_s seconds are spent; then the
+    speed check on the reference's own timed graph, seeded_model(0, 1e6, 8) at theta 0.8
+    with its Mulberry32 start (published: 123 031.69 ms for 100 iterations, 1.23 s each)."""
     from oracle import oracle as O
+    info = cpu_info()
+    core = pin_one_core()
     t0 = time.perf_counter()
-    st = O.Fa2(topo.n, topo.source, topo.target, start=start, weights=weights, tree=True,
-               theta2=1.2 * 1.2, scaling_ratio=params.scaling_ratio, gravity=params.gravity,
-               jitter_tolerance=params.jitter_tolerance)
+    st = O.Fa2(w["n"], w["source"], w["target"], start=w["start"], weights=w["weights"],
+               tree=True, theta2=w["theta"] ** 2, scaling_ratio=params.scaling_ratio,
+               gravity=params.gravity, jitter_tolerance=params.jitter_tolerance)
     setup = time.perf_counter() - t0
     its, t0 = 0, time.perf_counter()
     while True:
@@ -68,9 +102,26 @@ def cpu_baseline(topo, params, start, weights, budget_s: float):
         el = time.perf_counter() - t0
         if el >= budget_s or its >= 50:
             break
+    del st
+    # speed check against the published single-thread figure
+    gs, gt = O.gate_edges(1_000_000)
+    ref = O.Fa2(1_000_000, gs, gt, tree=True)
+    ref.iterate()  # warm
+    t1 = time.perf_counter()
+    for _ in range(3):
+        ref.iterate()
+    s_per_it = (time.perf_counter() - t1) / 3
+    del ref
     return {"value": its / el, "unit": "iters/s", "cores": 1, "kind": "port",
             "sample": f"{its} FA2-BH FP64 iterations of the same 1M-node graph and start "
-                      f"(oracle/fa2_oracle.c, gcc -O2, one thread; {setup:.1f}s setup untimed)"}
+                      f"(oracle/fa2_oracle.c, gcc -O2, one thread pinned to core {core}; "
+                      f"{setup:.1f}s setup untimed)",
+            **info,
+            "speed_check": {"graph": "seeded_model(0, 1000000, 8), 1 549 929 raw edges, "
+                                     "theta 0.8, Mulberry32 start, iterations 2-4",
+                            "edges": int(gs.size), "s_per_iter": round(s_per_it, 4),
+                            "published_s_per_iter": 1.2303,
+                            "ratio_to_published": round(s_per_it / 1.2303, 3)}}
 
 
 def batch_cpu_baseline(topos, starts, offs, params, budget_s: float):
@@ -78,6 +129,8 @@ def batch_cpu_baseline(topos, starts, offs, params, budget_s: float):
     iterations each after an untimed set-up; the batch's CPU time per iteration is the
     sample's time scaled by total nodes over sampled nodes."""
     from oracle import oracle as O
+    info = cpu_info()
+    core = pin_one_core()
     sample = list(range(0, len(topos), 32))
     spent, nodes, its = 0.0, 0, 8
     for b in sample:
@@ -97,7 +150,8 @@ def batch_cpu_baseline(topos, starts, offs, params, budget_s: float):
     return {"value": 1.0 / per_batch_iter, "unit": "batch iters/s", "cores": 1, "kind": "port",
             "sample": f"{its} FA2-BH FP64 iterations on each of {len(sample)} graphs (every "
                       f"32nd, {nodes} nodes), scaled by {total}/{nodes} nodes "
-                      f"(oracle/fa2_oracle.c, gcc -O2, one thread)"}
+                      f"(oracle/fa2_oracle.c, gcc -O2, one thread pinned to core {core})",
+            **info}
 
 
 def bench_batch(args):
@@ -148,6 +202,14 @@ def bench_batch(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def max_rel(approx, exact) -> float:
+    """The largest per-node relative error |a - e| / |e| (nodes with e != 0)."""
+    d = np.sqrt(((approx - exact) ** 2).sum(1))
+    e = np.sqrt((exact ** 2).sum(1))
+    ok = e > 0
+    return float((d[ok] / e[ok]).max())
 
 
 def rel_errors(approx, exact):
@@ -337,9 +399,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nodes", type=int, default=None)
-    ap.add_argument("--fp64-steps", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--fp64-steps", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
@@ -361,17 +425,21 @@ def main():
     import graph_render_amd as G
     from graph_render_amd import synth
 
-    n = args.nodes or 1_000_000
-    s, t, w = synth.sbm_zipf(n, 64, 1.1, 10.0, seed=1)
-    start = synth.uniform_start(n, 2, seed=1)
+    w = synth.workload("sbm1m")
+    if args.nodes:  # a smaller graph of the same family, for trials
+        s_, t_, w_ = synth.sbm_zipf(args.nodes, 64, 1.1, 10.0, seed=1)
+        w.update(n=args.nodes, source=s_, target=t_, weights=w_,
+                 start=synth.uniform_start(args.nodes, 2, seed=1))
+    n, s, t = w["n"], w["source"], w["target"]
     topo = G.Topology(n, s, t)
     params = G.Fa2Params(max_iter=500, scaling_ratio=2.0, gravity=1.0, jitter_tolerance=1.0)
 
-    def make(precision):
-        return G.Session(topo, params, G.Fa2Ext(theta=1.2, precision=precision, start=start,
-                                                 edge_weights=w))
+    def make(precision, **ext):
+        return G.Session(topo, params, G.Fa2Ext(theta=w["theta"], precision=precision,
+                                                 start=w["start"], edge_weights=w["weights"],
+                                                 **ext))
 
-    # FP32 (the headline)
+    # FP32 (the headline): W untimed iterations, then exactly K timed ones
     s32 = make("fp32")
     m_simple = len(s32.graph()[0])
     s32.step(args.warmup)
@@ -380,15 +448,33 @@ def main():
     el = time.perf_counter() - t0
     assert done == args.steps, (done, args.steps)
     fp32_its = done / el
-    # walk roofline on the current tree
-    walk_ms = s32.time("walk", 10)
-    tree_ms = s32.time("tree", 5)
+    # the walk (the dominant kernel) and the tree build, timed with HIP events on the
+    # session's stream, and the walk's interactions from its counting variant
+    walk_ms = s32.time("walk", 20)
+    tree_ms = s32.time("tree", 10)
+    ws = s32.walk_stats()
     ncells = s32.tree(bounds=False)["ncells"]
     wb = walk_bytes(n, ncells, "fp32")
     achieved = wb / (walk_ms * 1e-3) / 1e9
+    ib = iteration_bytes(n, ncells, m_simple)
     pos32 = s32.positions()
     f32 = s32.forces(7)
     s32.close()
+    if args.headline_only:
+        print(json.dumps({"metric": METRIC, "value": round(fp32_its, 3), "unit": "iters/s",
+                          "steps": args.steps, "walk_ms": round(walk_ms, 4),
+                          "tree_build_ms": round(tree_ms, 4), "tree_cells": int(ncells),
+                          "walk": ws}), flush=True)
+        return
+
+    # the whole 500-iteration run from the start, in blocks of 100
+    full = make("fp32")
+    blocks = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        d = full.step(100)
+        blocks.append(d / (time.perf_counter() - t0))
+    full.close()
 
     # FP32 with compensated accumulation, same start and steps
     sk = make("fp32_kahan")
@@ -400,21 +486,26 @@ def main():
     fk = sk.forces(7)
     sk.close()
 
-    # FP64 on the same start
-    s64 = make("fp64")
-    s64.step(2)
-    t0 = time.perf_counter()
-    d64 = s64.step(args.fp64_steps)
-    fp64_its = d64 / (time.perf_counter() - t0)
+    # FP64: the reference's arithmetic (node-order global sums, bit-exact trajectories) and
+    # with block-reduced sums, each over fp64_steps iterations after two untimed
+    def fp64_rate(**ext):
+        s64 = make("fp64", **ext)
+        s64.step(2)
+        t0 = time.perf_counter()
+        d64 = s64.step(args.fp64_steps)
+        r = d64 / (time.perf_counter() - t0)
+        return s64, r
+    s64, fp64_its = fp64_rate()
     s64.set_positions(pos32)
     f64 = s64.forces(7)
     s64.close()
+    s64b, fp64_block_its = fp64_rate(block_sums=True)
+    s64b.close()
     node_err, field_err = errors(f32, f64)
     knode_err, kfield_err = errors(fk, f64)
 
     # particle-mesh variant on a 2048^2 mesh, same graph and start, FP32
-    pm = G.Session(topo, params, G.Fa2Ext(repulsion="pm", pm_grid=2048, precision="fp32",
-                                          start=start, edge_weights=w))
+    pm = make("fp32", repulsion="pm", pm_grid=2048)
     pm.step(2)
     t0 = time.perf_counter()
     dpm = pm.step(args.steps)
@@ -431,7 +522,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(topo, params, start, w, args.cpu_seconds)
+        cpu = cpu_baseline(w, params, args.cpu_seconds)
 
     line = {
         "metric": METRIC,
@@ -453,17 +544,35 @@ def main():
             "nodes": n, "raw_edges": int(s.size), "simple_edges": int(m_simple),
             "edges_sha256": synth.digest(s, t),
         },
+        "full_run_500": {"iters_per_s_per_100": [round(b, 1) for b in blocks],
+                         "iters_per_s": round(500 / sum(100 / b for b in blocks), 3)},
         "fp64_iters_per_s": round(fp64_its, 3),
-        "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err},
+        "fp64_block_sums_iters_per_s": round(fp64_block_its, 3),
+        "fp64_steps": args.fp64_steps,
+        "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err,
+                                   "max_per_node": max_rel(f32, f64)},
         "fp32_kahan_iters_per_s": round(kahan_its, 3),
         "force_err_fp32_kahan_vs_fp64": {"per_node_rms": knode_err,
-                                         "whole_field": kfield_err},
+                                         "whole_field": kfield_err,
+                                         "max_per_node": max_rel(fk, f64)},
         "pm_2048_iters_per_s": round(pm_its, 3),
         "pm_2048_repulsion_ms": round(pm_ms, 4),
         "pm_vs_bh_fp64_repulsion_err": {"per_node_rms": pm_node, "whole_field": pm_field},
         "walk_ms": round(walk_ms, 4),
         "tree_build_ms": round(tree_ms, 4),
         "tree_cells": int(ncells),
+        "walk": {"interactions": ws["accepted"] + ws["pairs"],
+                 "interactions_per_s": round((ws["accepted"] + ws["pairs"]) /
+                                             (walk_ms * 1e-3), 1),
+                 "cells_accepted": ws["accepted"], "leaf_pairs": ws["pairs"],
+                 "scan_steps": ws["steps"],
+                 "steps_per_wave": round(ws["steps"] / max(1, -(-n // 64)), 1),
+                 "acting_lane_fraction": round(ws["acting_lanes"] / max(1, 64 * ws["steps"]),
+                                               4)},
+        "iteration_roofline": {"algorithmic_bytes": int(ib),
+                               "achieved": round(ib * fp32_its / 1e9, 2),
+                               "unit": "GB/s",
+                               "frac": round(ib * fp32_its / 1e9 / HBM_PEAK_GBS, 5)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": walk_traffic(n), "kernel": "k_walk<float,2>",

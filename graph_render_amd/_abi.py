@@ -72,6 +72,7 @@ SIGNATURES = {
                                         C.POINTER(C.c_uint32)]),
     "gr_fa2_session_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
+    "gr_fa2_session_walk_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_fa2_session_time": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32,
                                       C.POINTER(C.c_double)]),
     "gr_pm_side_for": (C.c_uint32, [C.c_uint32]),

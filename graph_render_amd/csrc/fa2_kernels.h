@@ -132,6 +132,7 @@ struct FaDev {
     // node order, as the reference does, instead of by block reductions: FP64 trajectories
     // are then the reference's bit for bit.
     int seq_sums = 0;
+    uint64_t *wstat = nullptr;  // the measuring walk's four counters
 
     hipStream_t stream = nullptr;
 };
@@ -153,6 +154,9 @@ void launch_forces(FaDev &d, uint32_t parts);
 void launch_tree(FaDev &d);
 // The walk alone over the current tree, for timing.
 void launch_walk(FaDev &d);
+// The walk's measuring variant: adds its scan steps, acting lanes, accepted cells and leaf
+// pairs to d.wstat (cleared by the caller).
+void launch_walk_count(FaDev &d);
 // Clears the halt flag left by a failed capacity check.
 void launch_clear_halt(FaDev &d);
 // One particle-mesh charge pass of a 2D FP64 single-graph state whose planar positions,

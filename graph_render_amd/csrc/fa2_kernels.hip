@@ -1367,7 +1367,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_com_span_level(const FaDev d, int 
 // BUF: the cell arena spans under 2 GiB, so records are read as buffer loads: a scalar byte
 // offset per record and the lane's dword offset, fixed for the whole scan.
 // ONE: a single-graph session, where every lane's tree ends at the scan's end.
-template <typename T, int D, bool KAHAN, bool BUF, bool ONE>
+// COUNT: the measuring variant, which also counts per wave its scan steps, the lanes acting
+// on each step, the cells accepted and the leaf pairs taken (d.wstat, four u64 totals).
+template <typename T, int D, bool KAHAN, bool BUF, bool ONE, bool COUNT = false>
 __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
@@ -1433,6 +1435,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         }
     };
     uint32_t cur = k < end ? record(k) : 0u;
+    uint64_t c_steps = 0, c_act = 0, c_acc = 0, c_pairs = 0;
     while (k < end) {
         WalkCell<T, D> cell;
         {
@@ -1471,6 +1474,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         const uint64_t accepted = acting & ~holding & wave_lt(cell.w2, open_rhs(d2, theta2));
         const bool accept = __builtin_amdgcn_inverse_ballot_w64(accepted);
         const bool act = __builtin_amdgcn_inverse_ballot_w64(acting);
+        if constexpr (COUNT) {
+            c_steps += 1;
+            c_act += (uint64_t)__builtin_popcountll(acting);
+            c_acc += (uint64_t)__builtin_popcountll(accepted);
+        }
         if (accepted != 0) {
             // the mass is read from the record only here, where a lane takes the force
             constexpr uint32_t MD = WalkCell<T, D>::MASS_DWORD;
@@ -1518,6 +1526,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
 #pragma unroll
                 for (int a = 0; a < D; a++) xj[a] = sp[a][q];
                 const T mj = sm[q];
+                if constexpr (COUNT) c_pairs += (uint64_t)__builtin_popcountll(__ballot(pairs && j != i));
                 if (pairs && j != i) {
                     // repel_delta from the lower index, negated for the higher one
                     // (barnes_hut.rs:170-180), the coincidence jiggle included.
@@ -1550,6 +1559,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
         // down to k + 1 when a lane opened k, else on to skip
         k = down ? k1 : ks;
         cur = down ? nx1 : nxs;
+    }
+    if constexpr (COUNT) {
+        if (gr_lane() == 0) {
+            atomicAdd((unsigned long long *)&d.wstat[0], (unsigned long long)c_steps);
+            atomicAdd((unsigned long long *)&d.wstat[1], (unsigned long long)c_act);
+            atomicAdd((unsigned long long *)&d.wstat[2], (unsigned long long)c_acc);
+            atomicAdd((unsigned long long *)&d.wstat[3], (unsigned long long)c_pairs);
+        }
     }
     if (live) {
 #pragma unroll
@@ -2287,10 +2304,19 @@ static void tree_impl(FaDev &d) {
 }
 
 template <typename T, int D, bool KAHAN>
-static void walk_impl(FaDev &d) {
+static void walk_impl(FaDev &d, bool count = false) {
     const dim3 grid(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK)));
     const bool buf = (uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31);
     const bool one = d.gidn == nullptr;
+    if (count) {  // the measuring variant (arenas under 2 GiB)
+        if (buf && one)
+            hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true, true>), grid, dim3(GR_BLOCK), 0,
+                               d.stream, d);
+        else if (buf)
+            hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false, true>), grid, dim3(GR_BLOCK), 0,
+                               d.stream, d);
+        return;
+    }
     if (buf && one)
         hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
     else if (buf)
@@ -2392,6 +2418,11 @@ void launch_walk(FaDev &dd) {
     FaDev d = dd;
     d.ignore_done = 1;
     GR_DISPATCH(walk_impl, d);
+}
+void launch_walk_count(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(walk_impl, d, true);
 }
 void launch_clear_halt(FaDev &d) {
     hipLaunchKernelGGL(k_clear_halt, dim3(1), dim3(1), 0, d.stream, d);

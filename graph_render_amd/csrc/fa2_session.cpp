@@ -564,6 +564,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     TRY_ALLOC(d.long_rows, ((size_t)d.nlong + 1) * 4);
     TRY_ALLOC(d.long_acc, ((size_t)d.nlong + 1) * d.dim * es);
     TRY_ALLOC(d.st, sizeof(GrState));
+    TRY_ALLOC(d.wstat, 4 * sizeof(uint64_t));
     // batch tables: node offsets, graph-aligned node chunks, node -> graph
     d.ngraphs = ngraphs;
     {
@@ -1222,6 +1223,29 @@ gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t rep
     gr_status r = s->read_state(st);
     if (r) return r;
     if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "halted while timing");
+    return ok();
+}
+
+gr_status gr_fa2_session_walk_stats(gr_fa2_session *s, uint64_t *out4) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (!out4) return fail(GR_ERR_PARAM, "out", "four counters");
+    if (s->d.repulsion != gr::REP_TREE)
+        return fail(GR_ERR_UNSUPPORTED, "walk", "a tree session");
+    for (int q = 0; q < 4; q++) out4[q] = 0;
+    if (s->d.n == 0) return ok();
+    if ((uint64_t)(s->d.cap + 1) * gr::walkcell_bytes(s->d.precision, s->d.dim) >= (1ull << 31))
+        return fail(GR_ERR_UNSUPPORTED, "walk", "a cell arena under 2 GiB");
+    if (!s->tree_valid) {
+        std::vector<double> tmp((size_t)s->d.n * s->d.dim);
+        gr_status r = gr_fa2_session_forces(s, 7, tmp.data());
+        if (r) return r;
+    }
+    HIP_TRY(hipMemsetAsync(s->d.wstat, 0, 4 * sizeof(uint64_t), s->d.stream));
+    gr::launch_walk_count(s->d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out4, s->d.wstat, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           s->d.stream));
+    HIP_TRY(hipStreamSynchronize(s->d.stream));
     return ok();
 }
 

@@ -445,6 +445,13 @@ class Session:
             out["bounds"] = b[: c * 2 * self.dim].reshape(c, 2 * self.dim)
         return out
 
+    def walk_stats(self) -> dict:
+        """One counting run of the walk on the current tree: scan steps (summed over waves),
+        lanes acting, cells accepted and leaf pairs taken."""
+        out = np.zeros(4, np.uint64)
+        _check(A.lib().gr_fa2_session_walk_stats(self._h, out.ctypes.data))
+        return dict(zip(("steps", "acting_lanes", "accepted", "pairs"), map(int, out)))
+
     def time(self, stage: str, reps: int) -> float:
         ms = C.c_double(0.0)
         _check(A.lib().gr_fa2_session_time(self._h, stage.encode(), int(reps), C.byref(ms)))

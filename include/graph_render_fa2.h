@@ -192,6 +192,11 @@ gr_status gr_fa2_session_record(gr_fa2_session *s, uint32_t iters, uint32_t chec
 gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, double *weight_bar,
                                  double *param_bar2, double *start_bar);
 
+/* The Barnes-Hut walk on the current tree, measured (no reference counterpart): runs a
+ * counting variant of the walk once and writes out4 = {wave scan steps, lanes acting summed
+ * over the steps, cells accepted, leaf pairs taken}; accepted + pairs are the interactions. */
+gr_status gr_fa2_session_walk_stats(gr_fa2_session *s, uint64_t *out4);
+
 /* Times `reps` launches of one stage on the session's stream with HIP events: "walk",
  * "tree", "iteration", "pm". Writes the mean milliseconds per launch. */
 gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,

@@ -144,3 +144,42 @@ def test_noalloc_charge_pass_refuses_sizes_past_its_array():
     with pytest.raises(G.CapacityError) as e:
         pm.charge_pass(np.zeros((10, 2)), 2 * pm.MAX_SIDE_NOALLOC, allocate=False)
     assert e.value.name == "side"
+
+
+# ---- the Rust binding (crates/graph-render-fa2-sys) declares what the header declares; no
+# Rust toolchain runs here, so its extern block is checked as text
+
+
+def _rust_externs():
+    text = open(os.path.join(ROOT, "crates", "graph-render-fa2-sys", "src", "lib.rs")).read()
+    block = text[text.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    fns = {}
+    for m in re.finditer(r"pub fn (gr_[a-z0-9_]+)\((.*?)\)", block, flags=re.S):
+        args = [a for a in m.group(2).split(",") if a.strip()]
+        fns[m.group(1)] = len(args)
+    return fns
+
+
+def _header_arity():
+    out = {}
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"\b(gr_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+            args = [a for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+            out[m.group(1)] = len(args)
+    return out
+
+
+def test_rust_binding_declares_every_header_function_with_its_arity():
+    assert _rust_externs() == _header_arity()
+
+
+def test_rust_structs_follow_the_header_field_order():
+    text = open(os.path.join(ROOT, "crates", "graph-render-fa2-sys", "src", "lib.rs")).read()
+    for name, fields in (("gr_fa2_params", A.GrFa2Params._fields_),
+                         ("gr_fa2_ext", A.GrFa2Ext._fields_)):
+        body = text[text.index(f"pub struct {name} {{"):]
+        body = body[:body.index("}")]
+        got = re.findall(r"pub (\w+):", body)
+        assert got == [f[0] for f in fields], name

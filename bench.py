@@ -393,6 +393,66 @@ def bench_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_autotune(args):
+    """The autotuner on each benchmark workload: iterations/s with the default schedule and
+    with the one gr_fa2_session_autotune picks (two sessions from the same start, 5 untimed
+    then `steps` timed iterations each), and the schedules themselves: the table
+    gr_fa2_schedule_select is written from."""
+    import graph_render_amd as G
+    from graph_render_amd import synth
+
+    def single(name, precision, theta):
+        w = synth.workload(name)
+        topo = G.Topology(w["n"], w["source"], w["target"])
+        p = G.Fa2Params(max_iter=10000, dim=w["dim"], seed=w["seed"])
+
+        def make():
+            return G.Session(topo, p, G.Fa2Ext(theta=theta, precision=precision,
+                                                start=w["start"], edge_weights=w["weights"]))
+        return w["n"], int(w["source"].size), make
+
+    def batch():
+        sizes, srcs, dsts, starts = synth.batch_workload(4096)
+        topos = [G.Topology(n, s_, t_) for n, s_, t_ in zip(sizes, srcs, dsts)]
+
+        def make():
+            return G.Session.batch(topos, G.Fa2Params(max_iter=10000),
+                                   G.Fa2Ext(theta=1.0, precision="fp32", start=starts))
+        return int(sum(sizes)), int(sum(s_.size for s_ in srcs)), make
+
+    cases = [("sbm10k", "fp32", 0.8), ("sbm10k", "fp64", 0.8), ("rgg250k", "fp32", 0.8),
+             ("sbm1m", "fp32", 1.2), ("batch4096", "fp32", 1.0)]
+    rows = []
+    for name, precision, theta in cases:
+        n, m, make = batch() if name == "batch4096" else single(name, precision, theta)
+        rates = {}
+        for mode in ("default", "autotuned"):
+            sess = make()
+            default = sess.schedule()
+            sess.step(5)
+            tuned = sess.autotune(5) if mode == "autotuned" else None
+            t0 = time.perf_counter()
+            d = sess.step(args.steps)
+            rates[mode] = d / (time.perf_counter() - t0)
+            sess.close()
+            if tuned is not None:
+                chosen = tuned
+        rows.append({"workload": name, "precision": precision, "nodes": n, "raw_edges": m,
+                     "default_schedule": default, "autotuned_schedule": chosen,
+                     "iters_per_s_default": round(rates["default"], 2),
+                     "iters_per_s_autotuned": round(rates["autotuned"], 2),
+                     "speedup": round(rates["autotuned"] / rates["default"], 3)})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    head = next(r for r in rows if r["workload"] == "sbm1m")
+    print(json.dumps({
+        "metric": "FA2-BH iterations/s with autotuned launch schedules",
+        "value": head["iters_per_s_autotuned"], "unit": "iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 5, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "the benchmark workloads, default vs autotuned schedule"},
+        "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -404,7 +464,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
-    ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint"],
+    ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
+                                           "autotune"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -420,6 +481,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "autotune":
+        bench_autotune(args)
         return
 
     import graph_render_amd as G

@@ -79,6 +79,16 @@ pub struct gr_fa2_ext {
     pub reserved: u32,
 }
 
+/// How the kernels are launched (never what they compute).
+#[repr(C)]
+#[derive(Clone, Copy, Debug, PartialEq, Eq)]
+pub struct gr_fa2_schedule {
+    pub walk_block: u32,
+    pub walk_xcd: u32,
+    pub sort_items: u32,
+    pub finish_split: u32,
+}
+
 /// An opaque device session (`struct gr_fa2_session`).
 #[repr(C)]
 pub struct gr_fa2_session {
@@ -122,6 +132,14 @@ extern "C" {
                                   weight_bar: *mut f64, param_bar2: *mut f64,
                                   start_bar: *mut f64) -> c_int;
     pub fn gr_fa2_session_walk_stats(s: *mut gr_fa2_session, out4: *mut u64) -> c_int;
+    pub fn gr_fa2_schedule_select(n: u32, m: usize, dim: u32, precision: i32, ngraphs: u32,
+                                  out: *mut gr_fa2_schedule) -> c_int;
+    pub fn gr_fa2_session_get_schedule(s: *mut gr_fa2_session, out: *mut gr_fa2_schedule)
+                                       -> c_int;
+    pub fn gr_fa2_session_set_schedule(s: *mut gr_fa2_session, sched: *const gr_fa2_schedule)
+                                       -> c_int;
+    pub fn gr_fa2_session_autotune(s: *mut gr_fa2_session, reps: u32,
+                                   out: *mut gr_fa2_schedule) -> c_int;
     pub fn gr_fa2_session_time(s: *mut gr_fa2_session, stage: *const c_char, reps: u32,
                                ms_out: *mut f64) -> c_int;
     pub fn gr_snapshot_encode(n: u32, node_ids: *const *const c_char, m: u32,

@@ -20,4 +20,5 @@ from .fa2 import (  # noqa: F401
     find,
     run_3d,
     run_params,
+    schedule_select,
 )

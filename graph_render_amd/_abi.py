@@ -36,6 +36,11 @@ class GrFa2Params(C.Structure):
                 ("seed", C.c_uint32), ("dim", C.c_uint32)]
 
 
+class GrFa2Schedule(C.Structure):
+    _fields_ = [("walk_block", C.c_uint32), ("walk_xcd", C.c_uint32),
+                ("sort_items", C.c_uint32), ("finish_split", C.c_uint32)]
+
+
 class GrFa2Ext(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("flags", C.c_uint32), ("theta", C.c_double),
                 ("precision", C.c_int32), ("repulsion", C.c_int32),
@@ -73,6 +78,11 @@ SIGNATURES = {
     "gr_fa2_session_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
     "gr_fa2_session_walk_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gr_fa2_schedule_select": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_int32,
+                                         C.c_uint32, C.POINTER(GrFa2Schedule)]),
+    "gr_fa2_session_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(GrFa2Schedule)]),
+    "gr_fa2_session_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(GrFa2Schedule)]),
+    "gr_fa2_session_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(GrFa2Schedule)]),
     "gr_fa2_session_time": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32,
                                       C.POINTER(C.c_double)]),
     "gr_pm_side_for": (C.c_uint32, [C.c_uint32]),

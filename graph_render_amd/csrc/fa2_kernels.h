@@ -134,6 +134,12 @@ struct FaDev {
     int seq_sums = 0;
     uint64_t *wstat = nullptr;  // the measuring walk's four counters
 
+    // the schedule (none of it moves a result bit): walk workgroup size, XCD-contiguous
+    // walk workgroups, radix-sort keys per thread (a tile is 256 of them)
+    uint32_t walk_block = 256;
+    int walk_xcd = 1;
+    uint32_t sort_items = 8;
+
     hipStream_t stream = nullptr;
 };
 
@@ -154,6 +160,9 @@ void launch_forces(FaDev &d, uint32_t parts);
 void launch_tree(FaDev &d);
 // The walk alone over the current tree, for timing.
 void launch_walk(FaDev &d);
+// The force finish alone (attraction, gravity and the swing/traction terms over the last
+// walk's repulsion), for timing.
+void launch_finish(FaDev &d);
 // The walk's measuring variant: adds its scan steps, acting lanes, accepted cells and leaf
 // pairs to d.wstat (cleared by the caller).
 void launch_walk_count(FaDev &d);

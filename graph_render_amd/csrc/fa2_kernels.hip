@@ -1369,13 +1369,15 @@ __global__ __launch_bounds__(GR_BLOCK) void k_com_span_level(const FaDev d, int 
 // ONE: a single-graph session, where every lane's tree ends at the scan's end.
 // COUNT: the measuring variant, which also counts per wave its scan steps, the lanes acting
 // on each step, the cells accepted and the leaf pairs taken (d.wstat, four u64 totals).
-template <typename T, int D, bool KAHAN, bool BUF, bool ONE, bool COUNT = false>
-__global__ __launch_bounds__(GR_BLOCK) void k_walk(const FaDev d) {
+// WB: threads per workgroup (64, 128 or 256), the schedule's choice; with d.walk_xcd the
+// consecutive workgroups of the grid share an XCD. Neither moves a result bit.
+template <typename T, int D, bool KAHAN, bool BUF, bool ONE, bool COUNT, int WB>
+__global__ __launch_bounds__(WB) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
     const uint32_t n = d.n;
-    const uint32_t lb = gr_xcd_block(blockIdx.x, (n + GR_BLOCK - 1) / GR_BLOCK);
-    const uint32_t p = lb * GR_BLOCK + threadIdx.x;
+    const uint32_t lb = d.walk_xcd ? gr_xcd_block(blockIdx.x, (n + WB - 1) / WB) : blockIdx.x;
+    const uint32_t p = lb * WB + threadIdx.x;
     if (__ballot(p < n) == 0) return;
     const bool live = p < n;
     const uint32_t pc = live ? p : n - 1;
@@ -2280,7 +2282,8 @@ static void tree_impl(FaDev &d) {
     }
     hipLaunchKernelGGL((k_keys<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, dk);
     radix_sort_pairs(dk.keys, dk.vals, dk.keys == d.keys ? d.keys_alt : d.keys,
-                     dk.vals == d.vals ? d.vals_alt : d.vals, n, (int)d.key_bits, d.sort_tmp, s);
+                     dk.vals == d.vals ? d.vals_alt : d.vals, n, (int)d.key_bits, d.sort_tmp, s,
+                     (int)d.sort_items);
     hipLaunchKernelGGL((k_gather<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.flag, d.gid, n, nullptr, &d.st->n_groups, d.scan_tmp, s);
     hipLaunchKernelGGL((k_group_compact<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
@@ -2303,28 +2306,34 @@ static void tree_impl(FaDev &d) {
     }
 }
 
-template <typename T, int D, bool KAHAN>
-static void walk_impl(FaDev &d, bool count = false) {
-    const dim3 grid(gr_xcd_grid(gr_blocks(d.n, GR_BLOCK)));
+template <typename T, int D, bool KAHAN, bool COUNT, int WB>
+static void walk_launch(FaDev &d) {
+    const uint32_t nb = gr_blocks(d.n, WB);
+    const dim3 grid(d.walk_xcd ? gr_xcd_grid(nb) : nb);
     const bool buf = (uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31);
     const bool one = d.gidn == nullptr;
-    if (count) {  // the measuring variant (arenas under 2 GiB)
-        if (buf && one)
-            hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true, true>), grid, dim3(GR_BLOCK), 0,
-                               d.stream, d);
-        else if (buf)
-            hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false, true>), grid, dim3(GR_BLOCK), 0,
-                               d.stream, d);
-        return;
-    }
     if (buf && one)
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true>), grid, dim3(GR_BLOCK), 0, d.stream, d);
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, true, COUNT, WB>), grid, dim3(WB), 0,
+                           d.stream, d);
     else if (buf)
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false>), grid, dim3(GR_BLOCK), 0, d.stream,
-                           d);
-    else
-        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false, false>), grid, dim3(GR_BLOCK), 0, d.stream,
-                           d);
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, true, false, COUNT, WB>), grid, dim3(WB), 0,
+                           d.stream, d);
+    else if (!COUNT)  // the measuring variant needs buffer loads (an arena under 2 GiB)
+        hipLaunchKernelGGL((k_walk<T, D, KAHAN, false, false, false, WB>), grid, dim3(WB), 0,
+                           d.stream, d);
+}
+
+template <typename T, int D, bool KAHAN>
+static void walk_impl(FaDev &d, bool count = false) {
+    if (count) {
+        walk_launch<T, D, KAHAN, true, 256>(d);
+    } else if (d.walk_block == 64) {
+        walk_launch<T, D, KAHAN, false, 64>(d);
+    } else if (d.walk_block == 128) {
+        walk_launch<T, D, KAHAN, false, 128>(d);
+    } else {
+        walk_launch<T, D, KAHAN, false, 256>(d);
+    }
 }
 
 void pm_repulsion(FaDev &d);  // pm.hip
@@ -2418,6 +2427,19 @@ void launch_walk(FaDev &dd) {
     FaDev d = dd;
     d.ignore_done = 1;
     GR_DISPATCH(walk_impl, d);
+}
+template <typename T, int D, bool KAHAN>
+static void finish_impl(FaDev &d) {
+    if (d.nlong > 0)
+        hipLaunchKernelGGL((k_long_rows<T, D, KAHAN>), dim3(d.nlong), dim3(GR_BLOCK), 0, d.stream,
+                           d);
+    hipLaunchKernelGGL((k_finish<T, D, KAHAN>), dim3(d.nparts), dim3(GR_BLOCK), FIN_LDS, d.stream,
+                       d, 0u, (uint32_t)(PART_ATTRACTION | PART_REPULSION | PART_GRAVITY));
+}
+void launch_finish(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(finish_impl, d);
 }
 void launch_walk_count(FaDev &dd) {
     FaDev d = dd;

@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -119,6 +122,27 @@ void build_simple(uint32_t n, const uint32_t *src, const uint32_t *dst, size_t m
     }
 }
 
+bool schedule_valid(const gr_fa2_schedule &sc) {
+    const bool wb = sc.walk_block == 64 || sc.walk_block == 128 || sc.walk_block == 256;
+    const bool si = sc.sort_items == 4 || sc.sort_items == 8 || sc.sort_items == 16;
+    const bool fs = sc.finish_split == 1 || sc.finish_split == 2 || sc.finish_split == 4 ||
+                    sc.finish_split == 8;
+    return wb && si && fs && sc.walk_xcd <= 1;
+}
+
+// A session's size class: log2 of its node count, its edges per node (rounded down to a
+// power of two), dimension, precision, and whether it is a batch.
+using SizeClass = std::tuple<int, int, uint32_t, int, int>;
+SizeClass size_class(uint32_t n, size_t m, uint32_t dim, int precision, uint32_t ngraphs) {
+    const int ln = n > 1 ? (int)std::floor(std::log2((double)n)) : 0;
+    const double per = n ? (double)m / n : 0.0;
+    const int lm = per >= 1.0 ? (int)std::floor(std::log2(per)) : 0;
+    return SizeClass{ln, lm, dim, precision, ngraphs > 1 ? 1 : 0};
+}
+
+std::mutex g_sched_lock;
+std::map<SizeClass, gr_fa2_schedule> g_sched_cache;  // autotuned schedules of this process
+
 }  // namespace
 
 struct gr_fa2_session {
@@ -129,6 +153,11 @@ struct gr_fa2_session {
     std::vector<size_t> simple_raw;
     size_t m_raw = 0;
     std::vector<double> mass;
+    std::vector<uint32_t> row_off;  // the CSR row offsets (the finish split is cut from them)
+    uint32_t part_cap = 0;          // k_finish workgroups d.part_rows has room for
+    gr_fa2_schedule sched{256, 1, 8, 1};
+    uint32_t size_key_n = 0;  // the sizes the session's schedule class is keyed on
+    size_t size_key_m = 0;
     std::vector<void *> allocs;
     int device = 0;
     bool tree_valid = false;
@@ -206,6 +235,49 @@ struct gr_fa2_session {
     }
 
     size_t pos_bytes() const { return (size_t)d.n * d.dim * esize(); }
+
+    // k_finish's row split: workgroup b takes the rows from the first one whose
+    // (first entry + row index) reaches b/nparts of (entries + rows), so that rows and
+    // entries are balanced together; a workgroup gets at most one LDS staging round of rows +
+    // entries, or 1/split of one (more, smaller workgroups).
+    static uint32_t finish_parts(uint32_t n, uint32_t m2, uint32_t es, uint32_t dim,
+                                 bool weighted, uint32_t split) {
+        const uint64_t units = (uint64_t)m2 + n;
+        const uint32_t per = gr::fin_stage(es, dim, weighted) / (split ? split : 1);
+        return gr_xcd_grid(gr_blocks(units, per > 0 ? per : 1));
+    }
+    // Installs a schedule: the walk's launch shape, the sort's tile and the finish split.
+    // The captured iteration holds the old launch parameters, so it is rebuilt.
+    gr_status apply_schedule(const gr_fa2_schedule &sc) {
+        gr_status r = set_finish_split(sc.finish_split);
+        if (r) return r;
+        d.walk_block = sc.walk_block;
+        d.walk_xcd = sc.walk_xcd ? 1 : 0;
+        d.sort_items = sc.sort_items;
+        sched = sc;
+        drop_graph();
+        return GR_OK;
+    }
+    gr_status set_finish_split(uint32_t split) {
+        const uint32_t n = d.n;
+        const uint64_t units = (uint64_t)d.m2 + n;
+        const uint32_t parts = finish_parts(n, d.m2, (uint32_t)esize(), d.dim, d.weighted != 0, split);
+        if (parts > part_cap) return fail(GR_ERR_PARAM, "finish_split", "1, 2, 4 or 8");
+        std::vector<uint32_t> part_rows((size_t)parts + 1, n);
+        uint32_t r = 0;
+        for (uint32_t b = 0; b < parts; b++) {
+            const uint64_t cut = units * b / parts;
+            while (r < n && (uint64_t)row_off[r] + r < cut) r++;
+            part_rows[b] = r;
+        }
+        part_rows[0] = 0;
+        part_rows[parts] = n;
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        HIP_TRY(hipMemcpy(d.part_rows, part_rows.data(), part_rows.size() * 4,
+                          hipMemcpyHostToDevice));
+        d.nparts = parts;
+        return GR_OK;
+    }
 
     // state -> snapshot (planar positions, graph scalars, session bookkeeping)
     gr_status save_state(void *pos, GrGraph *g, GrState *st) {
@@ -323,6 +395,30 @@ struct gr_fa2_session {
         return GR_OK;
     }
 };
+
+// The median of three timings of `reps` launches (HIP events on the session's stream).
+template <typename F>
+static gr_status time_median(gr_fa2_session *s, uint32_t reps, F launch, double *ms) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double t[3];
+    for (int k = 0; k < 3; k++) {
+        HIP_TRY(hipEventRecord(e0, s->d.stream));
+        for (uint32_t q = 0; q < reps; q++) launch();
+        HIP_TRY(hipEventRecord(e1, s->d.stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0.f;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        t[k] = (double)f / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::sort(t, t + 3);
+    *ms = t[1];
+    HIP_TRY(hipGetLastError());
+    return GR_OK;
+}
 
 extern "C" {
 
@@ -635,25 +731,9 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     TRY_ALLOC(d.lc_int, (size_t)d.lcell_cap * 16);
     TRY_ALLOC(d.lc_dbl, (size_t)d.lcell_cap * 2 * d.dim * 8);
     TRY_ALLOC(d.lc_sum, (size_t)d.lcell_cap * (1 + d.dim) * 8);
-    // k_finish's row split: workgroup b takes the rows from the first one whose
-    // (first entry + row index) reaches b/nparts of (entries + rows), so that rows and
-    // entries are balanced together and each workgroup stages its entries in one round
-    std::vector<uint32_t> part_rows;
-    {
-        const uint64_t units = (uint64_t)g.nbr.size() + n;
-        const uint32_t per = gr::fin_stage((uint32_t)es, d.dim, d.weighted != 0);
-        d.nparts = gr_xcd_grid(gr_blocks(units, per));
-        part_rows.assign((size_t)d.nparts + 1, n);
-        uint32_t r = 0;
-        for (uint32_t b = 0; b < d.nparts; b++) {
-            const uint64_t cut = units * b / d.nparts;
-            while (r < n && (uint64_t)g.row_off[r] + r < cut) r++;
-            part_rows[b] = r;
-        }
-        part_rows[0] = 0;
-        part_rows[d.nparts] = n;
-    }
-    TRY_ALLOC(d.part_rows, ((size_t)d.nparts + 1) * 4);
+    s->row_off = g.row_off;
+    s->part_cap = gr_fa2_session::finish_parts(n, d.m2, (uint32_t)es, d.dim, d.weighted != 0, 8);
+    TRY_ALLOC(d.part_rows, ((size_t)s->part_cap + 1) * 4);
     TRY_ALLOC(d.bpart, (size_t)d.nchunks * 10 * 8);
     TRY_ALLOC(d.spart, (size_t)d.nchunks * 2 * 8);
     TRY_ALLOC(d.mpart, (size_t)d.nchunks * 8);
@@ -702,8 +782,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
     };
     bool good = upload(d.row_off, g.row_off.data(), ((size_t)n + 1) * 4) &&
                 upload(d.nbr, g.nbr.data(), g.nbr.size() * 4) &&
-                upload(d.long_rows, long_rows.data(), long_rows.size() * 4) &&
-                upload(d.part_rows, part_rows.data(), part_rows.size() * 4);
+                upload(d.long_rows, long_rows.data(), long_rows.size() * 4);
     if (d.precision == gr::PREC_FP64) {
         good = good && upload(d.mass, s->mass.data(), (size_t)n * 8);
         if (d.weighted) good = good && upload(d.w, g.roww.data(), g.roww.size() * 8);
@@ -755,6 +834,25 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
                 start[q] = mulberry32_next(rs);
         }
     }
+    {
+        gr_fa2_schedule sc;
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> hold(g_sched_lock);
+            auto it = g_sched_cache.find(size_class(n, m, d.dim, d.precision, ngraphs));
+            if (it != g_sched_cache.end()) {
+                sc = it->second;
+                cached = true;
+            }
+        }
+        if (!cached) gr_fa2_schedule_select(n, m, d.dim, d.precision, ngraphs, &sc);
+        if (s->apply_schedule(sc) != GR_OK) {
+            delete s;
+            return nullptr;
+        }
+    }
+    s->size_key_n = n;
+    s->size_key_m = m;
     if (n > 0 && s->upload_positions(start.data()) != GR_OK) {
         delete s;
         return nullptr;
@@ -1179,6 +1277,115 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
                 bodies[(size_t)k * (1 + dim) + 1 + a] = raw[(size_t)k * (1 + dim) + 1 + a] / m;
         }
     }
+    return ok();
+}
+
+// The table of measured schedules: what gr_fa2_session_autotune chose on MI355X for the
+// benchmark workloads (`bench.py --workload autotune`): 10K-node SBM 1.057x (FP32) and
+// 1.020x (FP64) over the default {256, 1, 8, 1}, the 4096-graph batch 1.038x, the 250K 3D
+// and 1M graphs within noise. Keyed on node count, precision and batch; every entry gives
+// the same bits.
+gr_status gr_fa2_schedule_select(uint32_t n, size_t m, uint32_t dim, int32_t precision,
+                                 uint32_t ngraphs, gr_fa2_schedule *out) {
+    if (!out) return fail(GR_ERR_PARAM, "out", "a schedule to fill");
+    (void)m;
+    (void)dim;
+    if (ngraphs > 1)
+        *out = gr_fa2_schedule{64, 1, 16, 1};
+    else if (n <= 131072)
+        *out = gr_fa2_schedule{64, 1, 4, precision == GR_PREC_FP64 ? 2u : 8u};
+    else
+        *out = gr_fa2_schedule{64, 1, 8, 1};
+    return ok();
+}
+
+gr_status gr_fa2_session_get_schedule(gr_fa2_session *s, gr_fa2_schedule *out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (!out) return fail(GR_ERR_PARAM, "out", "a schedule to fill");
+    *out = s->sched;
+    return ok();
+}
+
+gr_status gr_fa2_session_set_schedule(gr_fa2_session *s, const gr_fa2_schedule *sched) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (!sched || !schedule_valid(*sched))
+        return fail(GR_ERR_PARAM, "schedule",
+                    "walk_block 64/128/256, walk_xcd 0/1, sort_items 4/8/16, finish_split 1/2/4/8");
+    gr_status r = s->apply_schedule(*sched);
+    return r ? r : ok();
+}
+
+gr_status gr_fa2_session_autotune(gr_fa2_session *s, uint32_t reps, gr_fa2_schedule *out) {
+    if (!s) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session handle");
+    if (s->d.repulsion != gr::REP_TREE)
+        return fail(GR_ERR_UNSUPPORTED, "autotune", "a tree session");
+    if (reps == 0) reps = 5;
+    gr_fa2_schedule best = s->sched;
+    if (s->d.n == 0) {
+        if (out) *out = best;
+        return ok();
+    }
+    s->drop_recording();
+    gr_status r;
+    std::vector<double> tmp((size_t)s->d.n * s->d.dim);
+    if ((r = gr_fa2_session_forces(s, 7, tmp.data()))) return r;  // a current tree
+    // walk: workgroup size x XCD mapping
+    double bt = 1e300;
+    for (uint32_t wb : {64u, 128u, 256u})
+        for (uint32_t xcd : {0u, 1u}) {
+            gr_fa2_schedule c = best;
+            c.walk_block = wb;
+            c.walk_xcd = xcd;
+            if ((r = s->apply_schedule(c))) return r;
+            double ms;
+            if ((r = time_median(s, reps, [&] { gr::launch_walk(s->d); }, &ms))) return r;
+            if (ms < bt) {
+                bt = ms;
+                best = c;
+            }
+        }
+    // the tree build's radix-sort tile
+    bt = 1e300;
+    gr_fa2_schedule base = best;
+    for (uint32_t items : {4u, 8u, 16u}) {
+        gr_fa2_schedule c = base;
+        c.sort_items = items;
+        if ((r = s->apply_schedule(c))) return r;
+        double ms;
+        if ((r = time_median(s, reps, [&] { gr::launch_tree(s->d); }, &ms))) return r;
+        if (ms < bt) {
+            bt = ms;
+            best = c;
+        }
+    }
+    // the force finish's split
+    bt = 1e300;
+    base = best;
+    if ((r = s->apply_schedule(base))) return r;
+    gr::launch_walk(s->d);  // the finish reads the walk's repulsion of the current tree
+    for (uint32_t split : {1u, 2u, 4u, 8u}) {
+        gr_fa2_schedule c = base;
+        c.finish_split = split;
+        if ((r = s->apply_schedule(c))) return r;
+        double ms;
+        if ((r = time_median(s, reps, [&] { gr::launch_finish(s->d); }, &ms))) return r;
+        if (ms < bt) {
+            bt = ms;
+            best = c;
+        }
+    }
+    if ((r = s->apply_schedule(best))) return r;
+    GrState st;
+    if ((r = s->read_state(st))) return r;
+    if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "halted while tuning");
+    // the timed stages rewrote the force buffers; the state (positions, speed) is untouched
+    if ((r = gr_fa2_session_forces(s, 7, tmp.data()))) return r;
+    {
+        std::lock_guard<std::mutex> hold(g_sched_lock);
+        g_sched_cache[size_class(s->size_key_n, s->size_key_m, s->d.dim, s->d.precision,
+                                 s->d.ngraphs)] = best;
+    }
+    if (out) *out = best;
     return ok();
 }
 

@@ -460,7 +460,7 @@ void pm_repulsion(FaDev &d) {
         else
             hipLaunchKernelGGL(k_pm_keys<float>, dim3(nb), dim3(GR_BLOCK), 0, s, d, k0, v0,
                                none);
-        radix_sort_pairs(k0, v0, k1, v1, d.n, bits, d.sort_tmp, s);
+        radix_sort_pairs(k0, v0, k1, v1, d.n, bits, d.sort_tmp, s, (int)d.sort_items);
         hipLaunchKernelGGL(k_pm_tile_starts, dim3(gr_blocks((uint64_t)d.n + 1, GR_BLOCK)),
                            dim3(GR_BLOCK), 0, s, d, (const uint64_t *)d.keys);
         if (d.precision == PREC_FP64)

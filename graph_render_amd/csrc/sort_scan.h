@@ -18,7 +18,8 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uin
 size_t radix_tmp_bytes(uint32_t n);
 int radix_digits(int end_bit);
 int radix_bits();  // key bits per sort pass
+// items: keys per thread of a tile (4, 8 or 16; a tile is 256 * items keys)
 void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                      uint32_t n, int end_bit, void *tmp, hipStream_t stream);
+                      uint32_t n, int end_bit, void *tmp, hipStream_t stream, int items = 8);
 
 }  // namespace gr

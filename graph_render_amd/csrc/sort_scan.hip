@@ -155,15 +155,17 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uin
 
 // ------------------------------------------------------------------ radix sort
 
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
+// RS_ITEMS keys per thread, RS_TILE = 256 * RS_ITEMS keys per tile: 4, 8 (default) or 16,
+// chosen per session by its schedule; the sort's result does not depend on it.
 constexpr int RS_BITS = 8;
 constexpr int RS_BUCKETS = 1 << RS_BITS;
 constexpr int RS_DPT = RS_BUCKETS / GR_BLOCK;  // digits per thread in the per-digit steps
 static_assert(RS_BUCKETS % GR_BLOCK == 0, "whole digits per thread");
 
+template <int RS_ITEMS>
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_hist(const uint64_t *keys, uint32_t n, int shift,
                                                       uint32_t *hist, uint32_t nb) {
+    constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
     __shared__ uint32_t h[RS_BUCKETS];
 #pragma unroll
     for (int q = 0; q < RS_DPT; q++) h[q * GR_BLOCK + threadIdx.x] = 0;
@@ -207,11 +209,13 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scan(uint32_t *hist, uint32_t n
 // order, so no block barrier is needed between rounds); one prefix over the waves then
 // gives each wave's first slot per digit, the keys go to their tile-local slots in LDS in
 // input order, and from there to their global slots.
+template <int RS_ITEMS>
 __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin,
                                                          uint64_t *kout, uint32_t *vout,
                                                          uint32_t n, int shift,
                                                          const uint32_t *hist, uint32_t nb,
                                                          const uint32_t *totals) {
+    constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
     constexpr uint32_t WAVES = GR_BLOCK / GR_WAVE;
     constexpr uint32_t PER_WAVE = RS_ITEMS * GR_WAVE;  // consecutive keys per wave
     __shared__ uint64_t skey[RS_TILE];
@@ -317,17 +321,19 @@ __global__ __launch_bounds__(GR_BLOCK) void k_rs_scatter(const uint64_t *kin, co
     }
 }
 
-size_t radix_tmp_bytes(uint32_t n) {
-    uint32_t nb = gr_blocks(n, RS_TILE);
+size_t radix_tmp_bytes(uint32_t n) {  // sized for the smallest tile (the most tiles)
+    uint32_t nb = gr_blocks(n, GR_BLOCK * 4);
     return ((size_t)RS_BUCKETS * nb + RS_BUCKETS) * sizeof(uint32_t) + 256;
 }
 
 // Sorts bits [0, end_bit) of the keys, rounded up to whole RS_BITS digits, ping-ponging
 // between (keys, vals) and (keys_alt, vals_alt): after an even number of digits the result
 // is in (keys, vals), after an odd number in the alternates. Nothing is copied.
-void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                      uint32_t n, int end_bit, void *tmp, hipStream_t stream) {
-    if (n == 0) return;
+template <int RS_ITEMS>
+static void radix_sort_impl(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                            uint32_t *vals_alt, uint32_t n, int end_bit, void *tmp,
+                            hipStream_t stream) {
+    constexpr int RS_TILE = GR_BLOCK * RS_ITEMS;
     uint32_t nb = gr_blocks(n, RS_TILE);
     uint32_t *hist = (uint32_t *)tmp;
     uint32_t *totals = hist + (size_t)RS_BUCKETS * nb;
@@ -336,11 +342,12 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32
     uint32_t *vi = vals, *vo = vals_alt;
     for (int p = 0; p < passes; p++) {
         int shift = p * RS_BITS;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_hist<RS_ITEMS>, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, n,
+                           shift, hist, nb);
         hipLaunchKernelGGL(k_rs_scan, dim3(RS_BUCKETS), dim3(GR_BLOCK), 0, stream, hist, nb,
                            totals);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, vi, ko, vo, n,
-                           shift, hist, nb, totals);
+        hipLaunchKernelGGL(k_rs_scatter<RS_ITEMS>, dim3(nb), dim3(GR_BLOCK), 0, stream, ki, vi,
+                           ko, vo, n, shift, hist, nb, totals);
         uint64_t *tk = ki;
         ki = ko;
         ko = tk;
@@ -348,6 +355,17 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32
         vi = vo;
         vo = tv;
     }
+}
+
+void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                      uint32_t n, int end_bit, void *tmp, hipStream_t stream, int items) {
+    if (n == 0) return;
+    if (items == 4)
+        radix_sort_impl<4>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, stream);
+    else if (items == 16)
+        radix_sort_impl<16>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, stream);
+    else
+        radix_sort_impl<8>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, stream);
 }
 
 int radix_digits(int end_bit) { return (end_bit + RS_BITS - 1) / RS_BITS; }

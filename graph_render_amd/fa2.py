@@ -275,6 +275,15 @@ def find(layout_id: str) -> Capability:
     return LAYOUTS[layout_id]
 
 
+def schedule_select(n: int, m: int, dim: int = 2, precision: str = "fp64",
+                    ngraphs: int = 1) -> dict:
+    """The measured table's launch schedule for these sizes (a pure function)."""
+    out = A.GrFa2Schedule()
+    _check(A.lib().gr_fa2_schedule_select(int(n), int(m), int(dim), PRECISIONS[precision],
+                                          int(ngraphs), C.byref(out)))
+    return {f: int(getattr(out, f)) for f, _ in A.GrFa2Schedule._fields_}
+
+
 def run_params(layout_id: str, topology: Topology, wire: bytes,
                ext: Optional[Fa2Ext] = None) -> Geometry:
     """registry/params.rs:144-160: the layout at the values a wire buffer carries (empty =
@@ -444,6 +453,25 @@ class Session:
         if b is not None:
             out["bounds"] = b[: c * 2 * self.dim].reshape(c, 2 * self.dim)
         return out
+
+    def schedule(self) -> dict:
+        """The session's launch schedule (it never changes a result bit)."""
+        out = A.GrFa2Schedule()
+        _check(A.lib().gr_fa2_session_get_schedule(self._h, C.byref(out)))
+        return {f: int(getattr(out, f)) for f, _ in A.GrFa2Schedule._fields_}
+
+    def set_schedule(self, **sched):
+        cur = self.schedule()
+        cur.update(sched)
+        sc = A.GrFa2Schedule(*(int(cur[f]) for f, _ in A.GrFa2Schedule._fields_))
+        _check(A.lib().gr_fa2_session_set_schedule(self._h, C.byref(sc)))
+
+    def autotune(self, reps: int = 5) -> dict:
+        """Times each schedule knob's candidates on this session, installs the fastest and
+        caches it for the session's size class; returns it."""
+        out = A.GrFa2Schedule()
+        _check(A.lib().gr_fa2_session_autotune(self._h, int(reps), C.byref(out)))
+        return {f: int(getattr(out, f)) for f, _ in A.GrFa2Schedule._fields_}
 
     def walk_stats(self) -> dict:
         """One counting run of the walk on the current tree: scan steps (summed over waves),

@@ -197,6 +197,34 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
  * over the steps, cells accepted, leaf pairs taken}; accepted + pairs are the interactions. */
 gr_status gr_fa2_session_walk_stats(gr_fa2_session *s, uint64_t *out4);
 
+/* ---- schedules and the autotuner (no reference counterpart) ----
+ * A schedule is how the kernels are launched, never what they compute: every schedule gives
+ * the same result bits. It is chosen per session: from the in-process cache of autotuned
+ * schedules for the session's size class when there is one, else by gr_fa2_schedule_select
+ * (a pure function of the sizes over a table measured on MI355X, as the reference's
+ * exec/select.rs picks tiers from measured thresholds). */
+typedef struct gr_fa2_schedule {
+    uint32_t walk_block;   /* threads per Barnes-Hut walk workgroup: 64, 128 or 256 */
+    uint32_t walk_xcd;     /* 1: consecutive walk workgroups (Morton-adjacent bodies) share an
+                              XCD and its L2; 0: the hardware's round-robin */
+    uint32_t sort_items;   /* radix-sort keys per thread: 4, 8 or 16 (a tile is 256x) */
+    uint32_t finish_split; /* force-finish workgroups per LDS round of rows: 1, 2, 4 or 8 */
+} gr_fa2_schedule;
+
+/* The measured table's schedule for a session of these sizes (n nodes over ngraphs graphs,
+ * m raw edges, dim, gr_precision). Reads no clock and no device. */
+gr_status gr_fa2_schedule_select(uint32_t n, size_t m, uint32_t dim, int32_t precision,
+                                 uint32_t ngraphs, gr_fa2_schedule *out);
+gr_status gr_fa2_session_get_schedule(gr_fa2_session *s, gr_fa2_schedule *out);
+/* Installs a schedule (values outside the listed sets are refused). */
+gr_status gr_fa2_session_set_schedule(gr_fa2_session *s, const gr_fa2_schedule *sched);
+/* Times each schedule knob's candidates on this session's current state (HIP events, the
+ * median of three runs of `reps` launches of the stage the knob drives: walk, tree build,
+ * force finish), installs the fastest, stores it in the process's cache under the session's
+ * size class (later sessions of that class start with it) and writes it to *out. Tree
+ * sessions only. Positions and the adaptive-speed state are left as they were. */
+gr_status gr_fa2_session_autotune(gr_fa2_session *s, uint32_t reps, gr_fa2_schedule *out);
+
 /* Times `reps` launches of one stage on the session's stream with HIP events: "walk",
  * "tree", "iteration", "pm". Writes the mean milliseconds per launch. */
 gr_status gr_fa2_session_time(gr_fa2_session *s, const char *stage, uint32_t reps,

@@ -183,3 +183,14 @@ def test_rust_structs_follow_the_header_field_order():
         body = body[:body.index("}")]
         got = re.findall(r"pub (\w+):", body)
         assert got == [f[0] for f in fields], name
+
+
+def test_schedule_select_is_a_pure_choice_from_the_candidate_sets():
+    for n, m, dim, prec, ng in ((10, 20, 2, "fp64", 1), (10_000, 54_547, 2, "fp32", 1),
+                                (250_000, 978_162, 3, "fp32", 1),
+                                (1_000_000, 4_997_552, 2, "fp32", 1),
+                                (13_543_864, 51_700_000, 2, "fp32", 4096)):
+        a = G.schedule_select(n, m, dim, prec, ng)
+        assert a == G.schedule_select(n, m, dim, prec, ng)
+        assert a["walk_block"] in (64, 128, 256) and a["walk_xcd"] in (0, 1)
+        assert a["sort_items"] in (4, 8, 16) and a["finish_split"] in (1, 2, 4, 8)

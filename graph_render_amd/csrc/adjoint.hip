@@ -67,11 +67,67 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_load(const FaDev d, const T *p
     }
 }
 
+// ---- exact accumulation: a term v = m * 2^e (m a 53-bit integer) lands as up to three
+// 32-bit chunks in the digit words of its place in a 256-bit fixed-point number (lowest bit
+// 2^ADJ_BASE). Integer additions commute, so the sum is the same whatever order the waves'
+// atomics land in; terms below 2^ADJ_BASE are truncated (toward zero), which is as exact a
+// rule as any. Each word takes chunks below 2^32: 2^31 terms cannot overflow it.
+__device__ __forceinline__ void exact_add(unsigned long long *acc, double v) {
+    if (v == 0.0) return;
+    if (!isfinite(v)) {
+        atomicAdd(&acc[ADJ_LIMBS - 1], 1ull);
+        return;
+    }
+    int e = 0;
+    const double fr = frexp(v, &e);               // v = fr * 2^e, 0.5 <= |fr| < 1
+    long long m = (long long)ldexp(fr, 53);       // exact: |m| < 2^53
+    int pos = e - 53 - ADJ_BASE;                  // the place of m's lowest bit
+    if (pos < 0) {
+        if (pos <= -64) return;
+        m = m >= 0 ? (long long)((unsigned long long)m >> -pos) : -(long long)((unsigned long long)(-m) >> -pos);
+        pos = 0;
+        if (m == 0) return;
+    }
+    const bool neg = m < 0;
+    const unsigned long long mag = (unsigned long long)(neg ? -m : m);
+    const int limb = pos >> 5, off = pos & 31;
+    const unsigned long long lo = mag << off;                        // bits [0, 64)
+    const unsigned long long hi = off ? mag >> (64 - off) : 0ull;    // bits [64, 85)
+    const unsigned long long c[3] = {lo & 0xFFFFFFFFull, lo >> 32, hi};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (c[q] == 0) continue;
+        if (limb + q >= ADJ_LIMBS - 1) {  // past the window's top: the sum reads as NaN
+            atomicAdd(&acc[ADJ_LIMBS - 1], 1ull);
+            continue;
+        }
+        atomicAdd(&acc[limb + q], neg ? (unsigned long long)(-(long long)c[q]) : c[q]);
+    }
+}
+
+// The accumulator's value: carries propagated through the digit words, then the number
+// read from its top word down (one rounding per word, the same for every run).
+__device__ __forceinline__ double exact_value(const unsigned long long *acc) {
+    if (acc[ADJ_LIMBS - 1] != 0) return NAN;
+    long long carry = 0;
+    unsigned long long dig[ADJ_LIMBS - 1];
+#pragma unroll
+    for (int q = 0; q < ADJ_LIMBS - 1; q++) {
+        const long long w = (long long)acc[q] + carry;
+        dig[q] = (unsigned long long)w & 0xFFFFFFFFull;
+        carry = w >> 32;  // arithmetic shift: floor(w / 2^32)
+    }
+    double r = (double)carry;
+#pragma unroll
+    for (int q = ADJ_LIMBS - 2; q >= 0; q--) r = r * 4294967296.0 + (double)dig[q];
+    return ldexp(r, ADJ_BASE);
+}
+
 template <int D>
 __global__ __launch_bounds__(GR_BLOCK) void k_adj_clear(const FaDev d, const AdjDev a) {
     const size_t q = (size_t)blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (q < (size_t)d.st->n_cells * D) a.cacc[q] = 0.0;
-    if (q < (size_t)d.n * D) a.pscat[q] = 0.0;
+    if (q < (size_t)d.st->n_cells * D * ADJ_LIMBS) a.cacc[q] = 0ull;
+    if (q < (size_t)d.n * D * ADJ_LIMBS) a.pscat[q] = 0ull;
 }
 
 // schedule.rs:62-77 reversed: p' = p + phi(u, s') u with phi = s'/(1 + q), q = sqrt(s' m |u|).
@@ -478,7 +534,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_walk(const FaDev d, const AdjD
                         if (single) {
                             cj[x] = cj[x] - kr * mi * jv[x];
                         } else {
-                            atomicAdd(&a.pscat[(size_t)q * D + x],
+                            exact_add(&a.pscat[((size_t)q * D + x) * ADJ_LIMBS],
                                       -(double)(kr * mi * mj * jv[x]));
                         }
                     }
@@ -490,7 +546,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_walk(const FaDev d, const AdjD
 #pragma unroll
             for (int x = 0; x < D; x++) {
                 const double s = gr_wave_sum((double)cj[x]);
-                if (gr_lane() == 0) atomicAdd(&a.cacc[(size_t)k * D + x], s);
+                if (gr_lane() == 0) exact_add(&a.cacc[((size_t)k * D + x) * ADJ_LIMBS], s);
             }
         }
         k = (__ballot(open) != 0 || cell.skip <= k) ? k + 1 : cell.skip;
@@ -521,7 +577,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_push(const FaDev d, const AdjD
     for (int x = 0; x < D; x++) s[x] = 0.0;
     for (;;) {
 #pragma unroll
-        for (int x = 0; x < D; x++) s[x] += a.cacc[(size_t)k * D + x];
+        for (int x = 0; x < D; x++) s[x] += exact_value(&a.cacc[((size_t)k * D + x) * ADJ_LIMBS]);
         const uint32_t end = wc[k].skip;
         if (end == k + 1) break;  // leaf
         uint32_t c = k + 1;
@@ -532,7 +588,8 @@ __global__ __launch_bounds__(GR_BLOCK) void k_adj_push(const FaDev d, const AdjD
     const uint32_t j = d.order[q];
     const double mj = (double)((const T *)d.sm)[q];
 #pragma unroll
-    for (int x = 0; x < D; x++) a.pb[x][j] += mj * s[x] + a.pscat[(size_t)q * D + x];
+    for (int x = 0; x < D; x++)
+        a.pb[x][j] += mj * s[x] + exact_value(&a.pscat[((size_t)q * D + x) * ADJ_LIMBS]);
 }
 
 __global__ void k_adj_scalar3(const AdjDev a, uint32_t nblk) {
@@ -562,7 +619,7 @@ static void step_impl(FaDev &d, AdjDev &a) {
     // the repulsion transposes read ub of every body and add into pb; the attraction
     // gather too. They only add to pb, so their order only affects rounding.
     if (d.repulsion == REP_TREE) {
-        const uint64_t cl = (uint64_t)(d.cap > d.n ? d.cap : d.n) * D;
+        const uint64_t cl = (uint64_t)(d.cap > d.n ? d.cap : d.n) * D * ADJ_LIMBS;
         const uint32_t nclr = gr_blocks(cl, GR_BLOCK);
         hipLaunchKernelGGL((k_adj_clear<D>), dim3(nclr), dim3(GR_BLOCK), 0, s, d, a);
         hipLaunchKernelGGL((k_adj_walk<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d, a);

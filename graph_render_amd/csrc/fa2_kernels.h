@@ -185,12 +185,20 @@ struct AdjScal {
 };
 
 constexpr int ADJ_NPART = 8;  // partial sums per block
+// One exact accumulator: ADJ_LIMBS - 1 signed 32-bit digit sums (held in 64-bit words) of a
+// fixed-point number with its lowest bit at 2^ADJ_BASE, and a count of non-finite terms.
+constexpr int ADJ_LIMBS = 9;
+constexpr int ADJ_BASE = -180;
 
 struct AdjDev {
     double *pb[3] = {};  // d loss / d positions, one column per axis
     double *ub[3] = {};  // d loss / d force u of the step being reversed
-    double *cacc = nullptr;  // per tree cell: sum over accepting bodies of -k*m_i*J(r)*ub_i
-    double *pscat = nullptr; // per tree position: pair terms for bodies of many-body leaves
+    // exact accumulators (ADJ_LIMBS integer limbs per value, adjoint.hip), so the order in
+    // which waves add into them moves no bit
+    unsigned long long *cacc = nullptr;  // per tree cell: sum over accepting bodies of
+                                         // -k*m_i*J(r)*ub_i
+    unsigned long long *pscat = nullptr; // per tree position: pair terms for bodies of
+                                         // many-body leaves
     double *wb = nullptr;    // d loss / d weight, per simple edge, summed over steps
     uint32_t *elo = nullptr, *ehi = nullptr;  // simple edges (lo < hi)
     uint32_t m = 0;

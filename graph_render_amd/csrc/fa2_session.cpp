@@ -1058,7 +1058,7 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
             return r;
     }
     a.m = s->m_simple;
-    if ((r = get((void **)&a.pscat, (size_t)n * dim * 8)) ||
+    if ((r = get((void **)&a.pscat, (size_t)n * dim * 8 * gr::ADJ_LIMBS)) ||
         (r = get((void **)&a.wb, (size_t)a.m * 8)) || (r = get((void **)&a.elo, (size_t)a.m * 4)) ||
         (r = get((void **)&a.ehi, (size_t)a.m * 4)) ||
         (r = get((void **)&a.sc, sizeof(gr::AdjScal))) ||
@@ -1105,9 +1105,9 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
             if ((r = s->recover(st))) return r;
         }
         if (cacc_cap < d.cap) {
-            if ((r = get(&cacc, (size_t)d.cap * dim * 8))) return r;
+            if ((r = get(&cacc, (size_t)d.cap * dim * 8 * gr::ADJ_LIMBS))) return r;
             cacc_cap = d.cap;
-            a.cacc = (double *)cacc;
+            a.cacc = (unsigned long long *)cacc;
         }
         for (uint32_t j = L; j-- > 0;) {
             if ((r = s->load_state((char *)tape_pos + (size_t)j * pb, tg + j, ts + j))) return r;

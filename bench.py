@@ -550,8 +550,9 @@ def main():
     fk = sk.forces(7)
     sk.close()
 
-    # FP64: the reference's arithmetic (node-order global sums, bit-exact trajectories) and
-    # with block-reduced sums, each over fp64_steps iterations after two untimed
+    # FP64: with block-reduced global sums (fp64_iters_per_s) and with the reference's own
+    # node-order sums (bit-exact trajectories), each over fp64_steps iterations after two
+    # untimed
     def fp64_rate(**ext):
         s64 = make("fp64", **ext)
         s64.step(2)
@@ -583,6 +584,16 @@ def main():
     f64r = s64r.forces(2)
     s64r.close()
     pm_node, pm_field = errors(fpm, f64r)
+    # the exact pair sum (device FP64 dense, O(n^2)) at the same positions: how far the
+    # Barnes-Hut approximation at theta 1.2 and the 2048^2 mesh each are from it
+    ex = make("fp64", repulsion="dense")
+    ex.set_positions(pos32)
+    t0 = time.perf_counter()
+    fex = ex.forces(2)
+    exact_s = time.perf_counter() - t0
+    ex.close()
+    pm_ex_node, pm_ex_field = errors(fpm, fex)
+    bh_ex_node, bh_ex_field = errors(f64r, fex)
 
     cpu = None
     if not args.no_cpu:
@@ -610,8 +621,8 @@ def main():
         },
         "full_run_500": {"iters_per_s_per_100": [round(b, 1) for b in blocks],
                          "iters_per_s": round(500 / sum(100 / b for b in blocks), 3)},
-        "fp64_iters_per_s": round(fp64_its, 3),
-        "fp64_block_sums_iters_per_s": round(fp64_block_its, 3),
+        "fp64_iters_per_s": round(fp64_block_its, 3),
+        "fp64_node_order_sums_iters_per_s": round(fp64_its, 3),
         "fp64_steps": args.fp64_steps,
         "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err,
                                    "max_per_node": max_rel(f32, f64)},
@@ -622,6 +633,11 @@ def main():
         "pm_2048_iters_per_s": round(pm_its, 3),
         "pm_2048_repulsion_ms": round(pm_ms, 4),
         "pm_vs_bh_fp64_repulsion_err": {"per_node_rms": pm_node, "whole_field": pm_field},
+        "pm_vs_exact_fp64_repulsion_err": {"per_node_rms": pm_ex_node,
+                                           "whole_field": pm_ex_field},
+        "bh_theta_1_2_vs_exact_fp64_repulsion_err": {"per_node_rms": bh_ex_node,
+                                                     "whole_field": bh_ex_field},
+        "exact_fp64_dense_repulsion_s": round(exact_s, 3),
         "walk_ms": round(walk_ms, 4),
         "tree_build_ms": round(tree_ms, 4),
         "tree_cells": int(ncells),

@@ -1987,31 +1987,78 @@ __global__ __launch_bounds__(GR_BLOCK, (fin_waves<T, D, KAHAN>())) void k_finish
 
 // The reference's sums in its own order (state.rs:208-220, schedule.rs:16-30 and :62-77):
 // each one a chain from 0.0 adding the nodes' terms one after another in ascending node
-// order. The block stages SEQ_TILE nodes' terms in LDS with coalesced loads, then thread 0
-// adds them in order; the chains of one sum kind run side by side in that thread. The
-// result is valid in thread 0.
+// order. Lane 0 runs the chains (K side by side) over SEQ_TILE nodes' terms at a time staged
+// in LDS, while the block's other waves stage the next tile (double-buffered), and reads its
+// tile eight nodes ahead of its adds, so the loads hide under the add chains. The result is
+// valid in thread 0. Launch with GR_BLOCK threads.
 constexpr uint32_t SEQ_TILE = 2048;
 
-// K interleaved doubles per node at src[K * v + k]: sums k = 0..K-1 over v in [v0, v1)
-template <int K>
-__device__ void seq_sum_nodes(const double *src, uint32_t v0, uint32_t v1, double (&s)[K]) {
-    __shared__ double tile[K * SEQ_TILE];
+template <int K, typename Term>
+__device__ void seq_sum_chain(uint32_t v0, uint32_t v1, double (&s)[K], Term term) {
+    __shared__ double tile[2][K * SEQ_TILE];
 #pragma unroll
     for (int k = 0; k < K; k++) s[k] = 0.0;
-    for (uint32_t base = v0; base < v1; base += SEQ_TILE) {
+    const uint32_t nt = (v1 - v0 + SEQ_TILE - 1) / SEQ_TILE;
+    auto fill = [&](uint32_t t) {
+        const uint32_t base = v0 + t * SEQ_TILE;
         const uint32_t cnt = v1 - base < SEQ_TILE ? v1 - base : SEQ_TILE;
-        for (uint32_t q = threadIdx.x; q < K * cnt; q += GR_BLOCK)
-            tile[q] = src[(size_t)K * base + q];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-#pragma unroll 8
-            for (uint32_t q = 0; q < cnt; q++) {
+        double *tb = tile[t & 1];
+        for (uint32_t q = threadIdx.x - GR_WAVE; q < cnt; q += GR_BLOCK - GR_WAVE) {
 #pragma unroll
-                for (int k = 0; k < K; k++) s[k] = s[k] + tile[K * q + k];
+            for (int k = 0; k < K; k++) tb[K * q + k] = term(base + q, k);
+        }
+    };
+    if (threadIdx.x >= GR_WAVE && nt > 0) fill(0);
+    __syncthreads();
+    for (uint32_t t = 0; t < nt; t++) {
+        if (threadIdx.x >= GR_WAVE) {
+            if (t + 1 < nt) fill(t + 1);
+        } else if (threadIdx.x == 0) {
+            const uint32_t base = v0 + t * SEQ_TILE;
+            const uint32_t cnt = v1 - base < SEQ_TILE ? v1 - base : SEQ_TILE;
+            const double *tb = tile[t & 1];
+            constexpr uint32_t B = 8;
+            uint32_t q = 0;
+            if (cnt >= B) {
+                double cur[B][K];
+#pragma unroll
+                for (uint32_t b = 0; b < B; b++)
+#pragma unroll
+                    for (int k = 0; k < K; k++) cur[b][k] = tb[K * b + k];
+                for (; q + 2 * B <= cnt; q += B) {
+                    double nxt[B][K];
+#pragma unroll
+                    for (uint32_t b = 0; b < B; b++)
+#pragma unroll
+                        for (int k = 0; k < K; k++) nxt[b][k] = tb[K * (q + B + b) + k];
+#pragma unroll
+                    for (uint32_t b = 0; b < B; b++)
+#pragma unroll
+                        for (int k = 0; k < K; k++) s[k] = s[k] + cur[b][k];
+#pragma unroll
+                    for (uint32_t b = 0; b < B; b++)
+#pragma unroll
+                        for (int k = 0; k < K; k++) cur[b][k] = nxt[b][k];
+                }
+#pragma unroll
+                for (uint32_t b = 0; b < B; b++)
+#pragma unroll
+                    for (int k = 0; k < K; k++) s[k] = s[k] + cur[b][k];
+                q += B;
+            }
+            for (; q < cnt; q++) {
+#pragma unroll
+                for (int k = 0; k < K; k++) s[k] = s[k] + tb[K * q + k];
             }
         }
         __syncthreads();
     }
+}
+
+// K interleaved doubles per node at src[K * v + k]: sums k = 0..K-1 over v in [v0, v1)
+template <int K>
+__device__ void seq_sum_nodes(const double *src, uint32_t v0, uint32_t v1, double (&s)[K]) {
+    seq_sum_chain<K>(v0, v1, s, [&](uint32_t v, int k) { return src[(size_t)K * v + k]; });
 }
 
 // The centroid as state.rs:208-220 forms it (one block per graph), over the planar
@@ -2021,26 +2068,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_seq_mean(const FaDev d) {
     if (halted(d)) return;
     const uint32_t b = blockIdx.x;
     const uint32_t v0 = d.gnode[b], v1 = d.gnode[b + 1];
-    __shared__ double tile[D][SEQ_TILE];
     double s[D];
-#pragma unroll
-    for (int a = 0; a < D; a++) s[a] = 0.0;
-    for (uint32_t base = v0; base < v1; base += SEQ_TILE) {
-        const uint32_t cnt = v1 - base < SEQ_TILE ? v1 - base : SEQ_TILE;
-        for (uint32_t q = threadIdx.x; q < cnt; q += GR_BLOCK) {
-#pragma unroll
-            for (int a = 0; a < D; a++) tile[a][q] = (double)((const T *)d.pos[a])[base + q];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-#pragma unroll 8
-            for (uint32_t q = 0; q < cnt; q++) {
-#pragma unroll
-                for (int a = 0; a < D; a++) s[a] = s[a] + tile[a][q];
-            }
-        }
-        __syncthreads();
-    }
+    seq_sum_chain<D>(v0, v1, s,
+                     [&](uint32_t v, int a) { return (double)((const T *)d.pos[a])[v]; });
     if (threadIdx.x == 0) {
         const double nb = (double)(v1 - v0);
 #pragma unroll
@@ -2207,9 +2237,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_tail(const FaDev d) {
         __shared__ double lds[4];
         double v[1] = {0.0};
         if (d.seq_sums) {  // block-uniform: each node's step, left in nodeval by k_apply
-            double sv[2];
-            seq_sum_nodes<2>(d.nodeval, d.gnode[blockIdx.x], d.gnode[blockIdx.x + 1], sv);
-            v[0] = sv[0];
+            const double *nv = d.nodeval;
+            seq_sum_chain<1>(d.gnode[blockIdx.x], d.gnode[blockIdx.x + 1], v,
+                             [&](uint32_t q, int) { return nv[2 * (size_t)q]; });
         } else {
             for (uint32_t c = d.gchunk[blockIdx.x] + threadIdx.x; c < d.gchunk[blockIdx.x + 1];
                  c += GR_BLOCK)

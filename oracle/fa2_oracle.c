@@ -8,6 +8,7 @@
  */
 #include "fa2_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -133,6 +134,7 @@ struct gro_tree {
     /* preorder arena */
     box_t *cbox;
     uint32_t *skip, *start, *end;
+    uint32_t *node_id; /* the shape slot each cell was flattened from (quadtree.rs:64-88) */
     size_t ncells, capcells;
     uint32_t *order;
     size_t norder;
@@ -311,18 +313,21 @@ static box_t quadrant(box_t b, uint32_t dim, int slot) {
     return r;
 }
 
-static void push_cell(gro_tree *t, box_t b, uint32_t skip, uint32_t start, uint32_t end) {
+static void push_cell(gro_tree *t, box_t b, uint32_t skip, uint32_t start, uint32_t end,
+                      int32_t node) {
     if (t->ncells == t->capcells) {
         t->capcells = t->capcells ? t->capcells * 2 : 64;
         t->cbox = (box_t *)realloc(t->cbox, t->capcells * sizeof(box_t));
         t->skip = (uint32_t *)realloc(t->skip, t->capcells * sizeof(uint32_t));
         t->start = (uint32_t *)realloc(t->start, t->capcells * sizeof(uint32_t));
         t->end = (uint32_t *)realloc(t->end, t->capcells * sizeof(uint32_t));
+        t->node_id = (uint32_t *)realloc(t->node_id, t->capcells * sizeof(uint32_t));
     }
     t->cbox[t->ncells] = b;
     t->skip[t->ncells] = skip;
     t->start[t->ncells] = start;
     t->end[t->ncells] = end;
+    t->node_id[t->ncells] = (uint32_t)node;
     t->ncells++;
 }
 
@@ -361,9 +366,9 @@ static void flatten(gro_tree *t) {
                 t->order[t->norder++] = (uint32_t)p;
                 p = t->chain_next[p];
             }
-            push_cell(t, v.b, k + 1, start, (uint32_t)t->norder);
+            push_cell(t, v.b, k + 1, start, (uint32_t)t->norder, v.node);
         } else {
-            push_cell(t, v.b, 0, start, 0);
+            push_cell(t, v.b, 0, start, 0, v.node);
             if (top + nslots + 1 >= cap) {
                 cap = cap * 2 + nslots + 1;
                 stack = (visit_t *)realloc(stack, cap * sizeof(visit_t));
@@ -434,6 +439,7 @@ void gro_tree_free(gro_tree *t) {
     free(t->skip);
     free(t->start);
     free(t->end);
+    free(t->node_id);
     free(t->order);
     free(t);
 }
@@ -456,6 +462,157 @@ void gro_tree_cells(const gro_tree *t, double *bounds, uint32_t *skip, uint32_t 
 
 void gro_tree_order(const gro_tree *t, uint32_t *order) {
     memcpy(order, t->order, t->n * sizeof(uint32_t));
+}
+
+void gro_tree_node_ids(const gro_tree *t, uint32_t *out) {
+    if (t->ncells) memcpy(out, t->node_id, t->ncells * sizeof(uint32_t));
+}
+
+/* ------------------------------------------- d3's many-body force over the quadtree */
+
+/* barnes_hut/charge/threshold.rs:20-41: w^2 / theta^2, descend everything at theta 0 */
+static double opening_threshold(double w, double theta, double theta2) {
+    if (theta == 0.0) return INFINITY;
+    double naive = w * w / theta2;
+    if (!isnan(naive)) return naive;
+    if (!isfinite(theta)) return INFINITY;
+    double ratio = w / theta;
+    return isfinite(ratio) ? ratio * ratio : INFINITY;
+}
+
+typedef struct {
+    double comx, comy, open;
+    uint32_t count, skip, start;
+} d3body_t;
+
+typedef struct {
+    const gro_tree *t;
+    const d3body_t *bodies;
+    const double *x, *y;
+    double dmin2, dmax2, charge, alpha;
+    uint32_t seed, tick;
+} d3ctx_t;
+
+/* barnes_hut/charge.rs:262-294 (settle): 0 past distanceMax, else the gap with an exactly
+ * zero axis jiggled (keyed on (i, key)) and l raised to distanceMin */
+static int d3_settle(const d3ctx_t *c, uint32_t i, uint32_t key, double *dx, double *dy,
+                     double *l) {
+    if (*l >= c->dmax2) return 0;
+    if (*dx == 0.0) {
+        *dx = gro_jiggle(c->seed, c->tick, 2, i, key);
+        *l += *dx * *dx;
+    }
+    if (*dy == 0.0) {
+        *dy = gro_jiggle(c->seed, c->tick, 3, i, key);
+        *l += *dy * *dy;
+    }
+    if (*l < c->dmin2) *l = sqrt(c->dmin2 * *l);
+    if (*l == 0.0) *l = c->dmin2 > DBL_MIN ? c->dmin2 : DBL_MIN;
+    return 1;
+}
+
+/* barnes_hut/charge.rs:196-240 (node_delta, approx, direct) */
+static void d3_node_delta(const d3ctx_t *c, uint32_t i, double *ox, double *oy) {
+    const double xi = c->x[i], yi = c->y[i];
+    double outx = 0.0, outy = 0.0;
+    uint32_t k = 0;
+    const uint32_t nc = (uint32_t)c->t->ncells;
+    while (k < nc) {
+        const d3body_t *b = &c->bodies[k];
+        double dx = b->comx - xi, dy = b->comy - yi;
+        double l = dx * dx + dy * dy;
+        double ddx = 0.0, ddy = 0.0;
+        uint32_t next;
+        if (b->open >= l) {
+            if (b->skip != k + 1) {
+                k++;
+                continue;
+            }
+            const uint32_t *chain = c->t->order + b->start;
+            if (d3_settle(c, i, chain[0], &dx, &dy, &l)) {
+                for (uint32_t q = 0; q < b->count; q++) {
+                    if (chain[q] == i) continue;
+                    double f = c->charge * c->alpha / l;
+                    ddx += dx * f;
+                    ddy += dy * f;
+                }
+            }
+            next = k + 1;
+        } else {
+            if (d3_settle(c, i, c->t->node_id[k], &dx, &dy, &l)) {
+                double m = (double)b->count;
+                double f = m * c->charge * c->alpha / l;
+                ddx = dx * f;
+                ddy = dy * f;
+            }
+            next = b->skip;
+        }
+        outx += ddx;
+        outy += ddy;
+        k = next;
+    }
+    *ox = outx;
+    *oy = outy;
+}
+
+/* The velocity each node gains from d3's many-body force over the quadtree of its
+ * positions (barnes_hut/charge.rs:41-53 + 120-140): bodies in reverse preorder (a leaf at
+ * its chain head's position, an internal cell at the count-weighted mean of its children's
+ * centres in slot order), then every node's walk. xy: n*2 row-major; out: n*2. */
+void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge, double alpha,
+                      double dmin, double dmax, uint32_t seed, uint32_t tick, double *out) {
+    double *xs = (double *)malloc((n ? n : 1) * sizeof(double));
+    double *ys = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint32_t i = 0; i < n; i++) {
+        xs[i] = xy[2 * (size_t)i];
+        ys[i] = xy[2 * (size_t)i + 1];
+    }
+    gro_tree *t = gro_tree_build(n, 2, xs, ys, NULL);
+    const size_t nc = t->ncells;
+    d3body_t *bodies = (d3body_t *)calloc(nc ? nc : 1, sizeof(d3body_t));
+    const double theta2 = theta * theta;
+    for (size_t kk = nc; kk-- > 0;) {
+        double cx, cy;
+        if (t->skip[kk] == kk + 1) {
+            uint32_t head = t->order[t->start[kk]];
+            cx = xs[head];
+            cy = ys[head];
+        } else {  /* charge/threshold.rs:54-68 */
+            double m = 0.0, sx = 0.0, sy = 0.0;
+            size_t ch = kk + 1;
+            while (ch < t->skip[kk]) {
+                const d3body_t *cb = &bodies[ch];
+                double cm = (double)cb->count;
+                m += cm;
+                sx += cm * cb->comx;
+                sy += cm * cb->comy;
+                ch = cb->skip;
+            }
+            cx = sx / m;
+            cy = sy / m;
+        }
+        double w = t->cbox[kk].hi[0] - t->cbox[kk].lo[0];
+        d3body_t *b = &bodies[kk];
+        b->comx = cx;
+        b->comy = cy;
+        b->open = opening_threshold(w, theta, theta2);
+        b->count = t->end[kk] - t->start[kk];
+        b->skip = t->skip[kk];
+        b->start = t->start[kk];
+    }
+    d3ctx_t c = {t, bodies, xs, ys, dmin * dmin, dmax * dmax, charge, alpha, seed, tick};
+    for (uint32_t i = 0; i < n; i++) {
+        if (!isfinite(xs[i]) || !isfinite(ys[i])) {
+            out[2 * (size_t)i] = NAN;
+            out[2 * (size_t)i + 1] = NAN;
+            continue;
+        }
+        d3_node_delta(&c, i, &out[2 * (size_t)i], &out[2 * (size_t)i + 1]);
+    }
+    free(bodies);
+    gro_tree_free(t);
+    free(xs);
+    free(ys);
 }
 
 /* barnes_hut.rs:62-110: reverse preorder; a leaf sums its chain, an internal cell its

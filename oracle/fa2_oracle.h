@@ -76,9 +76,17 @@ uint32_t gro_tree_ncells(const gro_tree *t);
 void gro_tree_cells(const gro_tree *t, double *bounds, uint32_t *skip, uint32_t *start,
                     uint32_t *end);
 void gro_tree_order(const gro_tree *t, uint32_t *order);
+/* each cell's node id: the shape slot the insertion created it as (quadtree.rs:64-88) */
+void gro_tree_node_ids(const gro_tree *t, uint32_t *out);
 /* bodies of the tree for per-point masses: mass then dim centre coordinates per cell */
 void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, const double *ys,
                      const double *zs, double *out);
+
+/* ---- d3's many-body force over the quadtree (layout/force/barnes_hut/charge.rs) ----
+ * parity: theta 0 equals the direct d3 pair sum (closed form); node ids are the insertion's
+ * own shape slots */
+void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge, double alpha,
+                      double dmin, double dmax, uint32_t seed, uint32_t tick, double *out);
 
 /* ---- particle mesh (pm_oracle.c) ---- */
 double gro_pm_h(int step);

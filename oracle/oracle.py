@@ -72,6 +72,9 @@ def lib():
         L.gro_tree_cells.argtypes = [C.c_void_p, f64p, u32p, u32p, u32p]
         L.gro_tree_order.argtypes = [C.c_void_p, u32p]
         L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
+        L.gro_tree_node_ids.argtypes = [C.c_void_p, u32p]
+        L.gro_d3_charge_bh.argtypes = [C.c_uint32, f64p, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_uint32, C.c_uint32, f64p]
         L.gro_pm_h.restype = C.c_double
         L.gro_pm_h.argtypes = [C.c_int]
         L.gro_fft2.argtypes = [f64p, f64p, C.c_int, C.c_int]
@@ -219,6 +222,12 @@ class Tree:
         lib().gro_tree_order(self._h, o)
         self.order = o[: self.n]
 
+    def node_ids(self) -> np.ndarray:
+        """Each cell's node id: the shape slot the insertion created it as."""
+        out = np.zeros(max(self.ncells, 1), np.uint32)
+        lib().gro_tree_node_ids(self._h, out)
+        return out[: self.ncells]
+
     def bodies(self, mass: np.ndarray) -> np.ndarray:
         m = np.ascontiguousarray(mass, np.float64)
         out = np.zeros(max(self.ncells, 1) * (1 + self.dim))
@@ -271,3 +280,16 @@ def pm_charge(xy: np.ndarray, side: int, charge: float = -90.0, alpha: float = 1
         frame = dict(h=fr[0], ox=fr[1], oy=fr[2], step=int(fr[3]), cells=int(fr[4]),
                      reach=int(fr[5]))
     return out[: 2 * n].reshape(n, 2), frame
+
+
+def d3_charge_bh(xy: np.ndarray, theta: float = 0.9, charge: float = -30.0,
+                 alpha: float = 1.0, distance_min: float = 1.0,
+                 distance_max: float = float("inf"), seed: int = 0, tick: int = 0):
+    """d3's many-body force over the quadtree (barnes_hut/charge.rs): the velocity each node
+    gains, (n, 2)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    lib().gro_d3_charge_bh(n, xy.reshape(-1) if n else np.zeros(2), theta, charge, alpha,
+                           distance_min, distance_max, seed, tick, out)
+    return out[: 2 * n].reshape(n, 2)

@@ -1,0 +1,64 @@
+// d3's many-body force over the quadtree (the reference's layout.force Barnes-Hut charge
+// pass, crates/graph-core/src/layout/force/barnes_hut/charge.rs) on a tree the FA2 build
+// made: the d3 bodies, the cells' insertion node ids and the walk (d3.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gr {
+
+// One cell as the walk reads it (charge.rs:86-94: centre, opening threshold, point count,
+// the index past its subtree, its first point), plus the node id its jiggle is keyed on.
+struct D3Body {
+    double comx, comy, open;
+    uint32_t count, skip, start, nid;
+};
+static_assert(sizeof(D3Body) == 40, "a d3 body is 40 bytes");
+
+struct D3Dev {
+    uint32_t n = 0;   // points
+    uint32_t nf = 0;  // finite points: tree positions [0, nf)
+    uint32_t C = 0;   // cells
+    // the tree (an FP64 2D FA2 session's arena): cell records, the point order, the
+    // positions in tree order and the exported cell bounds (x0, y0, x1, y1)
+    const void *wc = nullptr;
+    const uint32_t *order = nullptr;
+    const double *sx = nullptr, *sy = nullptr;
+    const double *cb = nullptr;
+    // the pass's own buffers
+    D3Body *body = nullptr;
+    uint32_t *diff = nullptr;    // C + 1: +1 past each internal cell, -1 at its skip
+    uint32_t *depth = nullptr;   // C: prefix of diff, then each cell's depth
+    uint32_t *amin = nullptr;    // C: the smallest point index of the cell's subtree
+    uint32_t *bmin = nullptr;    // C: the insertion that created the cell (see d3.hip)
+    uint32_t *hist = nullptr;    // C + 1: internal cells per depth, then the bucket fills
+    uint32_t *maxd = nullptr;    // 1: the deepest internal cell's depth
+    uint32_t *bucket = nullptr;  // C: internal cells by depth
+    uint64_t *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *vals = nullptr, *vals_alt = nullptr;
+    void *tmp = nullptr;         // scan and sort scratch
+    double *out = nullptr;       // n * 2: the velocity each node gains
+    uint32_t *nid_out = nullptr; // C, or null
+    double theta = 0.9, theta2 = 0.81;
+    double dmin2 = 1.0, dmax2 = 520.0 * 520.0, charge = -90.0, alpha = 1.0;
+    uint32_t seed = 0, tick = 0;
+    hipStream_t stream = nullptr;
+};
+
+size_t d3_tmp_bytes(uint32_t C);
+// Leaf bodies, the structure of every body, the leaves' creating points and the depth
+// difference array (d.diff cleared here).
+void d3_launch_cells(const D3Dev &d);
+// Depths, and per-depth counts of the internal cells in d.hist with the deepest in d.maxd.
+void d3_launch_depth(const D3Dev &d);
+// d.hist holds each depth's first bucket slot: buckets the internal cells by depth.
+void d3_launch_bucket(const D3Dev &d);
+// The internal cells of one depth, [off, off + cnt) of the buckets, from their children.
+void d3_launch_level(const D3Dev &d, uint32_t off, uint32_t cnt);
+// Node ids: each cell's rank by (the point whose insertion pushed it, preorder index).
+void d3_launch_ids(const D3Dev &d);
+// The walk: every node's delta into d.out (NaN for a non-finite node).
+void d3_launch_walk(const D3Dev &d);
+
+}  // namespace gr

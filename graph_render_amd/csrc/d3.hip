@@ -1,0 +1,311 @@
+// d3's many-body force over the quadtree on gfx950: the reference's Barnes-Hut charge pass
+// (paths relative to crates/graph-core/src/layout/force/):
+//   bodies   barnes_hut/charge.rs:120-140 + charge/threshold.rs: a leaf sits at its chain
+//            head, an internal cell at the count-weighted mean of its children's centres in
+//            slot order; open = w^2 / theta^2 with the threshold's NaN and overflow cases
+//   node ids quadtree/preorder.rs:64-88 records, per cell, the shape slot the insertion
+//            (quadtree/build.rs:25-115) pushed it as; the walk keys its jiggle on it
+//   walk     barnes_hut/charge.rs:196-294 (node_delta, approx, direct, settle)
+//
+// The tree is the FA2 build's (the same PR quadtree, bit for bit); what the bulk build does
+// not know is the order in which the insertion would have pushed the cells. It follows from
+// the arena: the insertion of point p pushes, in this order, the internal cells of the split
+// p causes (top down) and then p's own leaf, or only a leaf in an empty slot. So a leaf was
+// pushed by its smallest point a (later points are chained into it), and an internal cell
+// by the first point of its subtree that is not at the position of the subtree's smallest
+// point a (every point before it was chained into a's leaf, which then spanned the cell's
+// square). For an internal cell that point is min(its value for the child holding a, the
+// smallest point of each other child); for a leaf it is the first point chained there by a
+// split that stopped narrowing. The cells one insertion pushes lie on one root-to-leaf path,
+// in preorder. A cell's node id is therefore its rank by (pushing point, preorder index).
+#include <math.h>
+
+#include "d3.h"
+#include "fa2_common.h"
+#include "sort_scan.h"
+
+namespace gr {
+
+// charge/threshold.rs:20-41
+__device__ __forceinline__ double d3_open(double w, double theta, double theta2) {
+    if (theta == 0.0) return INFINITY;
+    const double naive = w * w / theta2;
+    if (!isnan(naive)) return naive;
+    if (!isfinite(theta)) return INFINITY;
+    const double r = w / theta;
+    return isfinite(r) ? r * r : INFINITY;
+}
+
+// One thread per cell: the leaf bodies (centre at the chain head), every cell's count,
+// skip, start and threshold, the leaves' smallest and pushing points, and the depth
+// difference array (an internal cell covers [k + 1, skip) of its descendants).
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_cells(const D3Dev d) {
+    const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k >= d.C) return;
+    const WalkCell<double, 2> *wc = (const WalkCell<double, 2> *)d.wc;
+    const uint32_t skip = wc[k].skip, start = wc[k].start;
+    const uint32_t end = wc[k].end < d.nf ? wc[k].end : d.nf;
+    const double w = d.cb[(size_t)k * 4 + 2] - d.cb[(size_t)k * 4];
+    D3Body b;
+    b.open = d3_open(w, d.theta, d.theta2);
+    b.count = end > start ? end - start : 0u;
+    b.skip = skip;
+    b.start = start;
+    b.nid = 0;
+    if (skip == k + 1) {
+        b.comx = start < d.nf ? d.sx[start] : 0.0;
+        b.comy = start < d.nf ? d.sy[start] : 0.0;
+        uint32_t a = 0xFFFFFFFFu;
+        double ax = 0.0, ay = 0.0;
+        for (uint32_t q = start; q < end; q++) {
+            const uint32_t i = d.order[q];
+            if (i < a) {
+                a = i;
+                ax = d.sx[q];
+                ay = d.sy[q];
+            }
+        }
+        uint32_t p = 0xFFFFFFFFu;
+        for (uint32_t q = start; q < end; q++) {
+            const uint32_t i = d.order[q];
+            if (!(d.sx[q] == ax && d.sy[q] == ay) && i < p) p = i;
+        }
+        d.amin[k] = a;
+        d.bmin[k] = p;
+    } else {
+        b.comx = 0.0;
+        b.comy = 0.0;
+        atomicAdd(&d.diff[k + 1], 1u);
+        atomicAdd(&d.diff[skip <= d.C ? skip : d.C], 0xFFFFFFFFu);  // -1 modulo 2^32
+    }
+    d.body[k] = b;
+}
+
+// depth[k] = the inclusive prefix of diff (the scan left the exclusive one); internal cells
+// are counted per depth
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_depth(const D3Dev d) {
+    const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k >= d.C) return;
+    const uint32_t dep = d.depth[k] + d.diff[k];
+    d.depth[k] = dep;
+    if (d.body[k].skip != k + 1) {
+        atomicAdd(&d.hist[dep < d.C ? dep : d.C], 1u);
+        atomicMax(d.maxd, dep);
+    }
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_bucket(const D3Dev d) {
+    const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k >= d.C || d.body[k].skip == k + 1) return;
+    const uint32_t dep = d.depth[k];
+    const uint32_t at = atomicAdd(&d.hist[dep < d.C ? dep : d.C], 1u);
+    if (at < d.C) d.bucket[at] = k;
+}
+
+// charge/threshold.rs:54-68 (the centre), and the smallest and pushing points, for the
+// internal cells of one depth; their children are final (deeper, or leaves).
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_level(const D3Dev d, uint32_t off,
+                                                       uint32_t cnt) {
+    const uint32_t x = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (x >= cnt) return;
+    const uint32_t k = d.bucket[off + x];
+    const uint32_t skip = d.body[k].skip < d.C ? d.body[k].skip : d.C;
+    double m = 0.0, cx = 0.0, cy = 0.0;
+    uint32_t ca[4], cp[4];
+    int nk = 0;
+    uint32_t c = k + 1;
+    // at most four children, every skip points forward: bounded on any arena
+    for (int q = 0; q < 4 && c < skip; q++) {
+        const D3Body ch = d.body[c];
+        const double cm = (double)ch.count;
+        m += cm;
+        cx += cm * ch.comx;
+        cy += cm * ch.comy;
+        ca[nk] = d.amin[c];
+        cp[nk] = d.bmin[c];
+        nk++;
+        c = ch.skip > c ? ch.skip : skip;
+    }
+    uint32_t a = 0xFFFFFFFFu;
+    int first = 0;
+    for (int q = 0; q < nk; q++)
+        if (ca[q] < a) {
+            a = ca[q];
+            first = q;
+        }
+    uint32_t p = nk ? cp[first] : 0xFFFFFFFFu;
+    for (int q = 0; q < nk; q++)
+        if (q != first && ca[q] < p) p = ca[q];
+    d.body[k].comx = cx / m;
+    d.body[k].comy = cy / m;
+    d.amin[k] = a;
+    d.bmin[k] = p;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_keys(const D3Dev d, uint32_t cbits) {
+    const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (k >= d.C) return;
+    const uint32_t t = d.body[k].skip == k + 1 ? d.amin[k] : d.bmin[k];
+    d.keys[k] = ((uint64_t)t << cbits) | k;
+    d.vals[k] = k;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_ids(const D3Dev d, const uint32_t *sorted) {
+    const uint32_t r = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (r >= d.C) return;
+    const uint32_t k = sorted[r];
+    if (k >= d.C) return;
+    d.body[k].nid = r;
+    if (d.nid_out) d.nid_out[k] = r;
+}
+
+// charge.rs:262-294: false past distanceMax, else the gap with an exactly zero axis
+// jiggled (keyed on (i, key)) and l raised to distanceMin
+__device__ __forceinline__ bool d3_settle(const D3Dev &d, uint32_t i, uint32_t key, double &dx,
+                                          double &dy, double &l) {
+    if (l >= d.dmax2) return false;
+    if (dx == 0.0) {
+        dx = d_jiggle(d.seed, d.tick, 2, i, key);
+        l += dx * dx;
+    }
+    if (dy == 0.0) {
+        dy = d_jiggle(d.seed, d.tick, 3, i, key);
+        l += dy * dy;
+    }
+    if (l < d.dmin2) l = sqrt(d.dmin2 * l);
+    if (l == 0.0) l = d.dmin2 > 2.2250738585072014e-308 ? d.dmin2 : 2.2250738585072014e-308;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t d3_wave_min(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t o = __shfl_xor(v, m, GR_WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// charge.rs:196-240 as one wave-uniform preorder scan per 64 consecutive tree positions:
+// the wave visits the smallest of its lanes' next cells, so each lane still meets its own
+// cells in preorder and adds its terms in the reference's order; a body is read once per
+// wave (a scalar load).
+constexpr uint32_t D3_WALK = 256;
+
+__global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
+    const uint32_t nb = (d.n + D3_WALK - 1) / D3_WALK;
+    const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
+    if (lb >= nb) return;
+    const uint32_t p = lb * D3_WALK + threadIdx.x;
+    if (p < d.n && p >= d.nf) {
+        const uint32_t i = d.order[p];
+        d.out[2 * (size_t)i] = NAN;
+        d.out[2 * (size_t)i + 1] = NAN;
+    }
+    if (__ballot(p < d.nf) == 0) return;
+    const bool live = p < d.nf;
+    const uint32_t pc = live ? p : d.nf - 1;
+    const double xi = d.sx[pc], yi = d.sy[pc];
+    const uint32_t i = d.order[pc];
+    const uint32_t C = d.C;
+    const D3Body *__restrict__ body = d.body;
+    uint32_t next = live ? 0u : 0xFFFFFFFFu;
+    double ox = 0.0, oy = 0.0;
+    uint32_t k = gr_uniform(d3_wave_min(next));
+    while (k < C) {
+        const D3Body bd = body[k];
+        if (next == k) {
+            double dx = bd.comx - xi, dy = bd.comy - yi;
+            double l = dx * dx + dy * dy;
+            double ddx = 0.0, ddy = 0.0;
+            bool add = true;
+            uint32_t nx = k + 1;
+            if (bd.open >= l) {
+                if (bd.skip != k + 1) {
+                    add = false;
+                } else {
+                    const uint32_t *chain = d.order + bd.start;
+                    if (d3_settle(d, i, chain[0], dx, dy, l)) {
+                        const double f = d.charge * d.alpha / l;
+                        for (uint32_t q = 0; q < bd.count; q++) {
+                            if (chain[q] == i) continue;
+                            ddx += dx * f;
+                            ddy += dy * f;
+                        }
+                    }
+                }
+            } else {
+                if (d3_settle(d, i, bd.nid, dx, dy, l)) {
+                    const double m = (double)bd.count;
+                    const double f = m * d.charge * d.alpha / l;
+                    ddx = dx * f;
+                    ddy = dy * f;
+                }
+                nx = bd.skip > k ? bd.skip : k + 1;
+            }
+            if (add) {
+                ox += ddx;
+                oy += ddy;
+            }
+            next = nx;
+        }
+        // most steps descend: k + 1 when any lane waits for it, else the smallest wait
+        const uint32_t k1 = k + 1;
+        k = __ballot(next == k1) != 0 ? k1 : gr_uniform(d3_wave_min(next));
+    }
+    if (live) {
+        d.out[2 * (size_t)i] = ox;
+        d.out[2 * (size_t)i + 1] = oy;
+    }
+}
+
+size_t d3_tmp_bytes(uint32_t C) {
+    const uint32_t c = C ? C : 1;
+    const size_t a = scan_tmp_bytes(c + 1), b = radix_tmp_bytes(c);
+    return a > b ? a : b;
+}
+
+void d3_launch_cells(const D3Dev &d) {
+    (void)hipMemsetAsync(d.diff, 0, ((size_t)d.C + 1) * 4, d.stream);
+    (void)hipMemsetAsync(d.hist, 0, ((size_t)d.C + 1) * 4, d.stream);
+    (void)hipMemsetAsync(d.maxd, 0, 4, d.stream);
+    hipLaunchKernelGGL(k_d3_cells, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
+                       d);
+}
+
+void d3_launch_depth(const D3Dev &d) {
+    scan_exclusive(d.diff, d.depth, d.C, nullptr, nullptr, d.tmp, d.stream);
+    hipLaunchKernelGGL(k_d3_depth, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
+                       d);
+}
+
+void d3_launch_bucket(const D3Dev &d) {
+    hipLaunchKernelGGL(k_d3_bucket, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       d.stream, d);
+}
+
+void d3_launch_level(const D3Dev &d, uint32_t off, uint32_t cnt) {
+    hipLaunchKernelGGL(k_d3_level, dim3(gr_blocks(cnt, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
+                       d, off, cnt);
+}
+
+void d3_launch_ids(const D3Dev &d) {
+    // the key: pushing point (below n) above the preorder index (below C)
+    uint32_t cbits = 0, nbits = 0;
+    while (cbits < 32 && (1ull << cbits) < (uint64_t)d.C) cbits++;
+    while (nbits < 32 && (1ull << nbits) < (uint64_t)d.n) nbits++;
+    hipLaunchKernelGGL(k_d3_keys, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
+                       d, cbits);
+    const int end_bit = (int)(cbits + nbits) > 0 ? (int)(cbits + nbits) : 1;
+    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, d.C, end_bit, d.tmp, d.stream);
+    const uint32_t *sorted = (radix_digits(end_bit) & 1) ? d.vals_alt : d.vals;
+    hipLaunchKernelGGL(k_d3_ids, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream, d,
+                       sorted);
+}
+
+void d3_launch_walk(const D3Dev &d) {
+    const uint32_t nb = gr_blocks(d.n, D3_WALK);
+    hipLaunchKernelGGL(k_d3_walk, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, d.stream, d);
+}
+
+}  // namespace gr

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/graph_render_fa2.h"
+#include "d3.h"
 #include "fa2_kernels.h"
 #include "sort_scan.h"
 
@@ -1505,6 +1506,134 @@ gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charg
             gr::pm_frame_read(fr.data(), frame_out);
         }
     }
+    gr_fa2_session_release(s);
+    return r ? r : ok();
+}
+
+// d3's many-body pass over the quadtree (barnes_hut/charge.rs) on an FP64 2D session that
+// holds only the positions: the FA2 tree build, then the d3 bodies, node ids and walk
+// (d3.hip). Writes the deltas when `out` is given and the node ids when `node_id` is given
+// and cap >= the cell count (*ncells_out); with neither it is a size query.
+static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double alpha,
+                         double dmin, double dmax, uint32_t seed, uint32_t tick, double *out,
+                         uint32_t cap, uint32_t *ncells_out, uint32_t *node_id) {
+    gr::FaDev &d = s->d;
+    GrState st;
+    gr_status r;
+    for (int tries = 0; tries < 8; tries++) {
+        gr::launch_tree(d);
+        HIP_TRY(hipGetLastError());
+        if ((r = s->read_state(st))) return r;
+        if (!st.halt) break;
+        if ((r = s->recover(st))) return r;
+    }
+    if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
+    const uint32_t C = st.n_cells, n = d.n;
+    if (ncells_out) *ncells_out = C;
+    const bool ids = node_id && cap >= C;
+    if (node_id && !ids) return fail(GR_ERR_CAPACITY, "cap", "cap >= *ncells_out node ids");
+    if (!out && !ids) return GR_OK;
+    gr::D3Dev g;
+    g.n = n;
+    g.nf = st.n_finite;
+    g.C = C;
+    g.wc = d.wc;
+    g.order = d.order;
+    g.sx = (const double *)d.sp[0];
+    g.sy = (const double *)d.sp[1];
+    g.cb = d.cbounds;
+    g.theta = theta;
+    g.theta2 = theta * theta;
+    g.dmin2 = dmin * dmin;
+    g.dmax2 = dmax * dmax;
+    g.charge = charge;
+    g.alpha = alpha;
+    g.seed = seed;
+    g.tick = tick;
+    g.stream = d.stream;
+    const size_t c = C ? C : 1;
+    if ((r = s->alloc(&g.body, c * sizeof(gr::D3Body))) || (r = s->alloc(&g.diff, (c + 1) * 4)) ||
+        (r = s->alloc(&g.depth, c * 4)) || (r = s->alloc(&g.amin, c * 4)) ||
+        (r = s->alloc(&g.bmin, c * 4)) || (r = s->alloc(&g.hist, (c + 1) * 4)) ||
+        (r = s->alloc(&g.maxd, 4)) || (r = s->alloc(&g.bucket, c * 4)) ||
+        (r = s->alloc(&g.keys, c * 8)) || (r = s->alloc(&g.keys_alt, c * 8)) ||
+        (r = s->alloc(&g.vals, c * 4)) || (r = s->alloc(&g.vals_alt, c * 4)) ||
+        (r = s->alloc(&g.tmp, gr::d3_tmp_bytes(C))) ||
+        (r = s->alloc(&g.out, (size_t)(n ? n : 1) * 16)) ||
+        (ids && (r = s->alloc(&g.nid_out, c * 4))))
+        return r;
+    std::vector<uint32_t> cnt, off;  // read by copies still queued on the stream
+    if (C > 0) {
+        gr::d3_launch_cells(g);
+        gr::d3_launch_depth(g);
+        HIP_TRY(hipGetLastError());
+        // the internal cells bucketed by depth, then summed deepest first
+        uint32_t maxd = 0;
+        HIP_TRY(hipMemcpyAsync(&maxd, g.maxd, 4, hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        if (maxd >= C) return fail(GR_ERR_DEVICE, "tree", "a cell deeper than the arena is long");
+        cnt.assign(maxd + 1, 0);
+        off.assign(maxd + 1, 0);
+        HIP_TRY(hipMemcpy(cnt.data(), g.hist, cnt.size() * 4, hipMemcpyDeviceToHost));
+        uint32_t acc = 0;
+        for (uint32_t l = 0; l <= maxd; l++) {
+            off[l] = acc;
+            acc += cnt[l];
+        }
+        if (acc > C) return fail(GR_ERR_DEVICE, "tree", "more internal cells than cells");
+        HIP_TRY(hipMemcpyAsync(g.hist, off.data(), off.size() * 4, hipMemcpyHostToDevice,
+                               d.stream));
+        gr::d3_launch_bucket(g);
+        for (uint32_t l = maxd + 1; l-- > 0;)
+            if (cnt[l]) gr::d3_launch_level(g, off[l], cnt[l]);
+        gr::d3_launch_ids(g);
+    }
+    if (out) gr::d3_launch_walk(g);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    if (out) HIP_TRY(hipMemcpy(out, g.out, (size_t)n * 16, hipMemcpyDeviceToHost));
+    if (ids && C) HIP_TRY(hipMemcpy(node_id, g.nid_out, (size_t)C * 4, hipMemcpyDeviceToHost));
+    return GR_OK;
+}
+
+static gr_fa2_session *d3_session(uint32_t n, const double *xy) {
+    gr_fa2_params p;
+    gr_fa2_params_default(&p);
+    gr_fa2_ext x;
+    gr_fa2_ext_default(&x);
+    x.repulsion = GR_REP_TREE;
+    x.precision = GR_PREC_FP64;
+    x.start = xy;
+    // the bounds of every cell are kept: a body's opening threshold is its cell's width
+    x.flags = GR_EXT_DIRECT_LAUNCH | GR_EXT_BLOCK_SUMS | GR_EXT_EXPORT_TREE;
+    return gr_fa2_session_create(n, nullptr, nullptr, 0, &p, &x);
+}
+
+gr_status gr_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge,
+                          double alpha, double distance_min, double distance_max, uint32_t seed,
+                          uint32_t tick, double *out) {
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (!std::isfinite(charge) || !std::isfinite(alpha) || !(distance_min >= 0.0) ||
+        !(distance_max >= 0.0) || std::isnan(distance_max))
+        return fail(GR_ERR_PARAM, "charge", "finite charge and alpha, distances >= 0");
+    if (!(theta >= 0.0)) return fail(GR_ERR_PARAM, "theta", "an opening angle >= 0");
+    if (n == 0) return ok();
+    gr_fa2_session *s = d3_session(n, xy);
+    if (!s) return g_err.code;
+    gr_status r = d3_pass(s, theta, charge, alpha, distance_min, distance_max, seed, tick, out, 0,
+                          nullptr, nullptr);
+    gr_fa2_session_release(s);
+    return r ? r : ok();
+}
+
+gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint32_t *ncells_out,
+                               uint32_t *node_id) {
+    if (ncells_out) *ncells_out = 0;
+    if (n > 0 && !xy) return fail(GR_ERR_PARAM, "xy", "n*2 positions");
+    if (n == 0) return ok();
+    gr_fa2_session *s = d3_session(n, xy);
+    if (!s) return g_err.code;
+    gr_status r = d3_pass(s, 0.9, -90.0, 1.0, 1.0, 520.0, 0, 0, nullptr, cap, ncells_out, node_id);
     gr_fa2_session_release(s);
     return r ? r : ok();
 }

@@ -275,6 +275,32 @@ gr_status gr_pm_charge_noalloc(uint32_t n, const double *xy, uint32_t side, doub
                                double alpha, double distance_min, double distance_max,
                                double *out, double *frame_out);
 
+/* ---- d3's many-body force over the quadtree (layout.force's Barnes-Hut arm) ---- */
+
+/* One charge pass of the reference's Barnes-Hut force simulation (barnes_hut/charge.rs:41-53,
+ * 120-294: prepare_with + node_delta over every node) on the device: the PR quadtree of the
+ * positions (quadtree.rs:46-196), d3 bodies in reverse preorder (a leaf at its chain head,
+ * an internal cell at the count-weighted mean of its children's centres, opening threshold
+ * w^2/theta^2 per charge/threshold.rs), then every node's stackless walk: open when
+ * threshold >= l, a leaf's chain summed pair by pair, a cell taken whole otherwise; gaps past
+ * distance_max contribute nothing, an exactly zero axis is jiggled keyed on (node, the
+ * cell's insertion node id or the chain head), l is raised to sqrt(dmin^2 l) below dmin^2.
+ * out: n*2 f64, the velocity each node gains (NaN for a non-finite node); xy: n*2 row-major
+ * f64. seed and tick key the jiggle as the simulation's seed and tick number do. theta >= 0
+ * (0 is the exact pair sum); charge and alpha finite; distances >= 0, distance_max may be
+ * infinite. All FP64, the reference's operations in its order: the same bits. */
+gr_status gr_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge,
+                          double alpha, double distance_min, double distance_max, uint32_t seed,
+                          uint32_t tick, double *out);
+
+/* The node id of every cell of the quadtree of xy, in preorder (Quadtree::node_id,
+ * quadtree.rs:64-88 via preorder.rs's open): the shape slot the reference's insertion
+ * (quadtree/build.rs:25-115) pushed the cell as, derived here from the arena a bulk build
+ * makes. *ncells_out receives the cell count; node_id (cap entries) is written when
+ * cap >= that count, a size query when node_id is NULL. */
+gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint32_t *ncells_out,
+                               uint32_t *node_id);
+
 /* Last failure of the calling thread: code, parameter/column name, rule text. */
 gr_status gr_last_error(const char **name, const char **rule);
 

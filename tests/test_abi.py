@@ -146,6 +146,27 @@ def test_noalloc_charge_pass_refuses_sizes_past_its_array():
     assert e.value.name == "side"
 
 
+@pytest.mark.parametrize("kw, name", [
+    (dict(charge=float("nan")), "charge"),
+    (dict(alpha=float("inf")), "charge"),
+    (dict(distance_min=-1.0), "charge"),
+    (dict(distance_max=float("nan")), "charge"),
+    (dict(theta=-0.5), "theta"),
+    (dict(theta=float("nan")), "theta"),
+])
+def test_d3_charge_pass_refuses_bad_parameters_before_the_device(kw, name):
+    from graph_render_amd import barnes_hut
+    with pytest.raises(G.ParamError) as e:
+        barnes_hut.charge_pass(np.zeros((4, 2)), **kw)
+    assert e.value.name == name
+
+
+def test_d3_charge_pass_of_no_points_is_empty():
+    from graph_render_amd import barnes_hut
+    assert barnes_hut.charge_pass(np.zeros((0, 2))).shape == (0, 2)
+    assert barnes_hut.node_ids(np.zeros((0, 2))).shape == (0,)
+
+
 # ---- the Rust binding (crates/graph-render-fa2-sys) declares what the header declares; no
 # Rust toolchain runs here, so its extern block is checked as text
 

@@ -453,6 +453,67 @@ def bench_autotune(args):
         "rows": rows}), flush=True)
 
 
+def golden_spiral(n: int) -> np.ndarray:
+    """barnes_hut/seed.rs: node i at 12 sqrt(i + 1) along the golden-angle spiral (numpy's
+    cos/sin; the device and the oracle read the same array)."""
+    i = np.arange(n, dtype=np.float64)
+    r = 12.0 * np.sqrt(i + 1.0)
+    a = i * 2.399963229728653
+    return np.stack([np.cos(a) * r, np.sin(a) * r], 1)
+
+
+def bench_d3(args):
+    """d3's many-body pass over the quadtree (layout.force's Barnes-Hut charge pass): 1M
+    nodes at the simulation's golden-spiral seed positions, theta 0.9, chargeStrength -90,
+    distanceMin 1, chargeDistanceMax 520, alpha 1. Passes/s with the positions resident on the
+    device (tree build, bodies, node ids, walk), the walk's roofline position, bit-equality of
+    the device deltas with the oracle's at the full size, and the oracle's pass timed on one
+    core."""
+    from graph_render_amd import barnes_hut
+    n = args.nodes or 1_000_000
+    xy = golden_spiral(n)
+    kw = dict(theta=0.9, charge=-90.0, alpha=1.0, distance_min=1.0, distance_max=520.0,
+              seed=0, tick=0)
+    got, ms_pass, ms_walk, ncells = barnes_hut.time_pass(xy, reps=max(args.steps, 1), **kw)
+    print(f"d3 BH n={n}: pass {ms_pass:.3f} ms, walk {ms_walk:.3f} ms, {ncells} cells",
+          file=sys.stderr, flush=True)
+    # algorithmic bytes of one walk launch: per node its position (16 B), point index (4 B)
+    # and delta (16 B); per cell one read of its 40-byte body
+    wb = 36 * n + 40 * ncells
+    achieved = wb / (ms_walk * 1e-3) / 1e9
+    line = {
+        "metric": "d3 Barnes-Hut charge passes/s @1M nodes (tree build + bodies + walk), FP64",
+        "value": round(1e3 / ms_pass, 3), "unit": "passes/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(ms_pass, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"d3 many-body pass, {n} nodes at the golden-spiral seed "
+                               "positions, theta 0.9, charge -90, distanceMin 1, "
+                               "chargeDistanceMax 520, alpha 1", "nodes": n, "cells": ncells},
+        "walk_ms": round(ms_walk, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_d3_walk",
+                     "algorithmic_bytes": wb},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        info = cpu_info()
+        core = pin_one_core()
+        t0 = time.perf_counter()
+        want = O.d3_charge_bh(xy, **kw)
+        el = time.perf_counter() - t0
+        line["bit_exact_vs_oracle"] = bool(np.array_equal(got.view(np.uint64),
+                                                          want.view(np.uint64)))
+        line["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "passes/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"one whole d3 pass over the same {n} positions "
+                                          "(oracle/fa2_oracle.c: insertion quadtree, bodies, "
+                                          f"walk; gcc -O2, one thread pinned to core {core})",
+                                **info}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,7 +526,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
-                                           "autotune"],
+                                           "autotune", "d3"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -484,6 +545,9 @@ def main():
         return
     if args.workload == "autotune":
         bench_autotune(args)
+        return
+    if args.workload == "d3":
+        bench_d3(args)
         return
 
     import graph_render_amd as G

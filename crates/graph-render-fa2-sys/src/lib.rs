@@ -157,6 +157,9 @@ extern "C" {
     pub fn gr_d3_charge_bh(n: u32, xy: *const f64, theta: f64, charge: f64, alpha: f64,
                            distance_min: f64, distance_max: f64, seed: u32, tick: u32,
                            out: *mut f64) -> c_int;
+    pub fn gr_d3_charge_bh_time(n: u32, xy: *const f64, theta: f64, charge: f64, alpha: f64,
+                                distance_min: f64, distance_max: f64, seed: u32, tick: u32,
+                                reps: u32, out: *mut f64, ms_out: *mut f64) -> c_int;
     pub fn gr_quadtree_node_ids(n: u32, xy: *const f64, cap: u32, ncells_out: *mut u32,
                                 node_id: *mut u32) -> c_int;
     pub fn gr_last_error(name: *mut *const c_char, rule: *mut *const c_char) -> c_int;

@@ -93,6 +93,9 @@ SIGNATURES = {
                                        C.c_void_p]),
     "gr_d3_charge_bh": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gr_d3_charge_bh_time": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "gr_quadtree_node_ids": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32,
                                        C.POINTER(C.c_uint32), C.c_void_p]),
     "gr_snapshot_encode": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,

@@ -27,6 +27,23 @@ def charge_pass(xy, theta: float = 0.9, charge: float = -90.0, alpha: float = 1.
     return out[: 2 * n].reshape(n, 2)
 
 
+def time_pass(xy, reps: int = 5, theta: float = 0.9, charge: float = -90.0,
+              alpha: float = 1.0, distance_min: float = 1.0, distance_max: float = 520.0,
+              seed: int = 0, tick: int = 0):
+    """charge_pass, then the pass timed on the device-resident positions: (deltas, whole pass
+    ms, walk ms, cells); medians of three timings of `reps` runs."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    ms = np.zeros(3)
+    _check(A.lib().gr_d3_charge_bh_time(n, xy.ctypes.data if n else None, float(theta),
+                                        float(charge), float(alpha), float(distance_min),
+                                        float(distance_max), int(seed) & 0xFFFFFFFF,
+                                        int(tick) & 0xFFFFFFFF, int(reps), out.ctypes.data,
+                                        ms.ctypes.data))
+    return out[: 2 * n].reshape(n, 2), float(ms[0]), float(ms[1]), int(ms[2])
+
+
 def node_ids(xy) -> np.ndarray:
     """Quadtree::node_id of every cell in preorder (quadtree.rs:64-88): the shape slot the
     reference's insertion pushed it as."""

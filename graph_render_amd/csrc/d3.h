@@ -32,11 +32,9 @@ struct D3Dev {
     uint32_t *depth = nullptr;   // C: prefix of diff, then each cell's depth
     uint32_t *amin = nullptr;    // C: the smallest point index of the cell's subtree
     uint32_t *bmin = nullptr;    // C: the insertion that created the cell (see d3.hip)
-    uint32_t *hist = nullptr;    // C + 1: internal cells per depth, then the bucket fills
-    uint32_t *maxd = nullptr;    // 1: the deepest internal cell's depth
-    uint32_t *bucket = nullptr;  // C: internal cells by depth
-    uint64_t *keys = nullptr, *keys_alt = nullptr;
-    uint32_t *vals = nullptr, *vals_alt = nullptr;
+    uint64_t *keys = nullptr, *keys_alt = nullptr;  // C: sort keys (depth, then pusher)
+    uint32_t *vals = nullptr, *vals_alt = nullptr;  // C: cells in sort order
+    uint32_t *lvl = nullptr;     // C + 3: deepest internal depth, internal count, depth starts
     void *tmp = nullptr;         // scan and sort scratch
     double *out = nullptr;       // n * 2: the velocity each node gains
     uint32_t *nid_out = nullptr; // C, or null
@@ -47,15 +45,14 @@ struct D3Dev {
 };
 
 size_t d3_tmp_bytes(uint32_t C);
-// Leaf bodies, the structure of every body, the leaves' creating points and the depth
-// difference array (d.diff cleared here).
+// Leaf bodies, the structure of every body, the leaves' smallest and pushing points and the
+// depth difference array (d.diff cleared here).
 void d3_launch_cells(const D3Dev &d);
-// Depths, and per-depth counts of the internal cells in d.hist with the deepest in d.maxd.
-void d3_launch_depth(const D3Dev &d);
-// d.hist holds each depth's first bucket slot: buckets the internal cells by depth.
-void d3_launch_bucket(const D3Dev &d);
-// The internal cells of one depth, [off, off + cnt) of the buckets, from their children.
-void d3_launch_level(const D3Dev &d, uint32_t off, uint32_t cnt);
+// Depths, then the cells sorted by depth with the level table in d.lvl (deepest internal
+// depth, internal cell count, the first sorted slot of each depth); returns the sorted cells.
+const uint32_t *d3_launch_depth(const D3Dev &d);
+// The internal cells of one depth, [off, off + cnt) of the sorted cells, from their children.
+void d3_launch_level(const D3Dev &d, const uint32_t *bucket, uint32_t off, uint32_t cnt);
 // Node ids: each cell's rank by (the point whose insertion pushed it, preorder index).
 void d3_launch_ids(const D3Dev &d);
 // The walk: every node's delta into d.out (NaN for a non-finite node).

@@ -81,34 +81,52 @@ __global__ __launch_bounds__(GR_BLOCK) void k_d3_cells(const D3Dev d) {
     d.body[k] = b;
 }
 
-// depth[k] = the inclusive prefix of diff (the scan left the exclusive one); internal cells
-// are counted per depth
+// Internal cells sort by depth (16-bit keys; leaves carry the largest key and sort last):
+// every depth from 0 to the deepest internal cell's holds one, since an internal cell's
+// ancestors are internal. Device-scope atomics into a few shared counters serialise across
+// the XCDs, so the counting is the radix sort's own.
+constexpr uint32_t D3_LEAF_KEY = 0xFFFFu;
+
+// depth[k] = the inclusive prefix of diff (the scan left the exclusive one); the sort keys
 __global__ __launch_bounds__(GR_BLOCK) void k_d3_depth(const D3Dev d) {
     const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (k >= d.C) return;
     const uint32_t dep = d.depth[k] + d.diff[k];
     d.depth[k] = dep;
-    if (d.body[k].skip != k + 1) {
-        atomicAdd(&d.hist[dep < d.C ? dep : d.C], 1u);
-        atomicMax(d.maxd, dep);
-    }
+    const bool inner = d.body[k].skip != k + 1;
+    d.keys[k] = inner ? (dep < D3_LEAF_KEY ? dep : D3_LEAF_KEY - 1u) : D3_LEAF_KEY;
+    d.vals[k] = k;
 }
 
-__global__ __launch_bounds__(GR_BLOCK) void k_d3_bucket(const D3Dev d) {
-    const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (k >= d.C || d.body[k].skip == k + 1) return;
-    const uint32_t dep = d.depth[k];
-    const uint32_t at = atomicAdd(&d.hist[dep < d.C ? dep : d.C], 1u);
-    if (at < d.C) d.bucket[at] = k;
+// From the sorted keys: lvl[0] the deepest internal depth, lvl[1] the internal cell count,
+// lvl[2 + l] the first sorted slot of depth l (lvl cleared before).
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_bounds(const D3Dev d, const uint64_t *sk) {
+    const uint32_t r = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (r >= d.C) return;
+    const uint32_t kr = (uint32_t)(sk[r] & 0xFFFFu);
+    const uint32_t kp = r ? (uint32_t)(sk[r - 1] & 0xFFFFu) : 0x10000u;
+    if (kr != kp) {
+        if (kr != D3_LEAF_KEY) {
+            if (kr < d.C) d.lvl[2 + kr] = r;
+        } else if (r > 0) {
+            d.lvl[1] = r;
+            d.lvl[0] = kp;
+        }
+    }
+    if (r == d.C - 1 && kr != D3_LEAF_KEY) {
+        d.lvl[1] = d.C;
+        d.lvl[0] = kr;
+    }
 }
 
 // charge/threshold.rs:54-68 (the centre), and the smallest and pushing points, for the
 // internal cells of one depth; their children are final (deeper, or leaves).
-__global__ __launch_bounds__(GR_BLOCK) void k_d3_level(const D3Dev d, uint32_t off,
-                                                       uint32_t cnt) {
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_level(const D3Dev d, const uint32_t *bucket,
+                                                       uint32_t off, uint32_t cnt) {
     const uint32_t x = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (x >= cnt) return;
-    const uint32_t k = d.bucket[off + x];
+    const uint32_t k = bucket[off + x];
+    if (k >= d.C) return;
     const uint32_t skip = d.body[k].skip < d.C ? d.body[k].skip : d.C;
     double m = 0.0, cx = 0.0, cy = 0.0;
     uint32_t ca[4], cp[4];
@@ -142,11 +160,12 @@ __global__ __launch_bounds__(GR_BLOCK) void k_d3_level(const D3Dev d, uint32_t o
     d.bmin[k] = p;
 }
 
-__global__ __launch_bounds__(GR_BLOCK) void k_d3_keys(const D3Dev d, uint32_t cbits) {
+// Node ids: a stable sort of the cells (in preorder) by the point whose insertion pushed
+// them ranks them by (pushing point, preorder index).
+__global__ __launch_bounds__(GR_BLOCK) void k_d3_id_keys(const D3Dev d) {
     const uint32_t k = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (k >= d.C) return;
-    const uint32_t t = d.body[k].skip == k + 1 ? d.amin[k] : d.bmin[k];
-    d.keys[k] = ((uint64_t)t << cbits) | k;
+    d.keys[k] = d.body[k].skip == k + 1 ? d.amin[k] : d.bmin[k];
     d.vals[k] = k;
 }
 
@@ -214,14 +233,15 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
     uint32_t k = gr_uniform(d3_wave_min(next));
     while (k < C) {
         const D3Body bd = body[k];
+        const uint32_t k1 = k + 1;
         if (next == k) {
             double dx = bd.comx - xi, dy = bd.comy - yi;
             double l = dx * dx + dy * dy;
             double ddx = 0.0, ddy = 0.0;
             bool add = true;
-            uint32_t nx = k + 1;
+            uint32_t nx = k1;
             if (bd.open >= l) {
-                if (bd.skip != k + 1) {
+                if (bd.skip != k1) {
                     add = false;
                 } else {
                     const uint32_t *chain = d.order + bd.start;
@@ -241,7 +261,7 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
                     ddx = dx * f;
                     ddy = dy * f;
                 }
-                nx = bd.skip > k ? bd.skip : k + 1;
+                nx = bd.skip > k ? bd.skip : k1;
             }
             if (add) {
                 ox += ddx;
@@ -250,7 +270,6 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
             next = nx;
         }
         // most steps descend: k + 1 when any lane waits for it, else the smallest wait
-        const uint32_t k1 = k + 1;
         k = __ballot(next == k1) != 0 ? k1 : gr_uniform(d3_wave_min(next));
     }
     if (live) {
@@ -267,38 +286,34 @@ size_t d3_tmp_bytes(uint32_t C) {
 
 void d3_launch_cells(const D3Dev &d) {
     (void)hipMemsetAsync(d.diff, 0, ((size_t)d.C + 1) * 4, d.stream);
-    (void)hipMemsetAsync(d.hist, 0, ((size_t)d.C + 1) * 4, d.stream);
-    (void)hipMemsetAsync(d.maxd, 0, 4, d.stream);
     hipLaunchKernelGGL(k_d3_cells, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
                        d);
 }
 
-void d3_launch_depth(const D3Dev &d) {
+const uint32_t *d3_launch_depth(const D3Dev &d) {
     scan_exclusive(d.diff, d.depth, d.C, nullptr, nullptr, d.tmp, d.stream);
     hipLaunchKernelGGL(k_d3_depth, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
                        d);
+    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, d.C, 16, d.tmp, d.stream);
+    const bool odd = radix_digits(16) & 1;
+    (void)hipMemsetAsync(d.lvl, 0, ((size_t)d.C + 3) * 4, d.stream);
+    hipLaunchKernelGGL(k_d3_bounds, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
+                       d, odd ? d.keys_alt : d.keys);
+    return odd ? d.vals_alt : d.vals;
 }
 
-void d3_launch_bucket(const D3Dev &d) {
-    hipLaunchKernelGGL(k_d3_bucket, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0,
-                       d.stream, d);
-}
-
-void d3_launch_level(const D3Dev &d, uint32_t off, uint32_t cnt) {
+void d3_launch_level(const D3Dev &d, const uint32_t *bucket, uint32_t off, uint32_t cnt) {
     hipLaunchKernelGGL(k_d3_level, dim3(gr_blocks(cnt, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
-                       d, off, cnt);
+                       d, bucket, off, cnt);
 }
 
 void d3_launch_ids(const D3Dev &d) {
-    // the key: pushing point (below n) above the preorder index (below C)
-    uint32_t cbits = 0, nbits = 0;
-    while (cbits < 32 && (1ull << cbits) < (uint64_t)d.C) cbits++;
+    uint32_t nbits = 1;
     while (nbits < 32 && (1ull << nbits) < (uint64_t)d.n) nbits++;
-    hipLaunchKernelGGL(k_d3_keys, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream,
-                       d, cbits);
-    const int end_bit = (int)(cbits + nbits) > 0 ? (int)(cbits + nbits) : 1;
-    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, d.C, end_bit, d.tmp, d.stream);
-    const uint32_t *sorted = (radix_digits(end_bit) & 1) ? d.vals_alt : d.vals;
+    hipLaunchKernelGGL(k_d3_id_keys, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       d.stream, d);
+    radix_sort_pairs(d.keys, d.vals, d.keys_alt, d.vals_alt, d.C, (int)nbits, d.tmp, d.stream);
+    const uint32_t *sorted = (radix_digits((int)nbits) & 1) ? d.vals_alt : d.vals;
     hipLaunchKernelGGL(k_d3_ids, dim3(gr_blocks(d.C, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream, d,
                        sorted);
 }

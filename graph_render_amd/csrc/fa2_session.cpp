@@ -1512,36 +1512,116 @@ gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charg
 
 // d3's many-body pass over the quadtree (barnes_hut/charge.rs) on an FP64 2D session that
 // holds only the positions: the FA2 tree build, then the d3 bodies, node ids and walk
-// (d3.hip). Writes the deltas when `out` is given and the node ids when `node_id` is given
-// and cap >= the cell count (*ncells_out); with neither it is a size query.
+// (d3.hip).
+struct D3Pass {
+    gr_fa2_session *s;
+    gr::D3Dev g;
+    std::vector<uint32_t> lvl;  // the level table read back
+
+    // The tree of the session's positions; grows the arena when it runs out.
+    gr_status tree() {
+        gr::FaDev &d = s->d;
+        GrState st;
+        gr_status r;
+        for (int tries = 0; tries < 8; tries++) {
+            gr::launch_tree(d);
+            HIP_TRY(hipGetLastError());
+            if ((r = s->read_state(st))) return r;
+            if (!st.halt) break;
+            if ((r = s->recover(st))) return r;
+        }
+        if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
+        g.n = d.n;
+        g.nf = st.n_finite;
+        g.C = st.n_cells;
+        g.wc = d.wc;
+        g.order = d.order;
+        g.sx = (const double *)d.sp[0];
+        g.sy = (const double *)d.sp[1];
+        g.cb = d.cbounds;
+        g.stream = d.stream;
+        return GR_OK;
+    }
+
+    // The pass's buffers for the current tree's cell count.
+    gr_status alloc(bool ids) {
+        const size_t c = g.C ? g.C : 1;
+        gr_status r;
+        if ((r = s->alloc(&g.body, c * sizeof(gr::D3Body))) ||
+            (r = s->alloc(&g.diff, (c + 1) * 4)) || (r = s->alloc(&g.depth, c * 4)) ||
+            (r = s->alloc(&g.amin, c * 4)) || (r = s->alloc(&g.bmin, c * 4)) ||
+            (r = s->alloc(&g.keys, c * 8)) || (r = s->alloc(&g.keys_alt, c * 8)) ||
+            (r = s->alloc(&g.vals, c * 4)) || (r = s->alloc(&g.vals_alt, c * 4)) ||
+            (r = s->alloc(&g.lvl, (c + 3) * 4)) || (r = s->alloc(&g.tmp, gr::d3_tmp_bytes(g.C))) ||
+            (r = s->alloc(&g.out, (size_t)(g.n ? g.n : 1) * 16)) ||
+            (ids && (r = s->alloc(&g.nid_out, c * 4))))
+            return r;
+        return GR_OK;
+    }
+
+    // Bodies and node ids: leaves, depths, then the internal cells deepest level first (one
+    // launch per depth; the level table is read back once).
+    gr_status bodies() {
+        const uint32_t C = g.C;
+        if (C == 0) return GR_OK;
+        gr::d3_launch_cells(g);
+        const uint32_t *sorted = gr::d3_launch_depth(g);
+        HIP_TRY(hipGetLastError());
+        // the table's head and the first 64 depths in one read; deeper trees read the rest
+        const size_t first = std::min<size_t>((size_t)C + 3, 66);
+        lvl.assign(first, 0);
+        HIP_TRY(hipMemcpyAsync(lvl.data(), g.lvl, first * 4, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        const uint32_t maxd = lvl[0], nint = lvl[1];
+        if (nint > C || (nint > 0 && maxd >= C))
+            return fail(GR_ERR_DEVICE, "tree", "a level table outside the arena");
+        if (nint > 0 && (size_t)maxd + 3 > first) {
+            lvl.resize((size_t)maxd + 3);
+            HIP_TRY(hipMemcpy(lvl.data() + first, g.lvl + first, (lvl.size() - first) * 4,
+                              hipMemcpyDeviceToHost));
+        }
+        if (nint > 0)
+            for (uint32_t l = maxd + 1; l-- > 0;) {
+                const uint32_t a = lvl[2 + l], b = l == maxd ? nint : lvl[3 + l];
+                if (a > b || b > nint)
+                    return fail(GR_ERR_DEVICE, "tree", "a level table out of order");
+                if (b > a) gr::d3_launch_level(g, sorted, a, b - a);
+            }
+        gr::d3_launch_ids(g);
+        HIP_TRY(hipGetLastError());
+        return GR_OK;
+    }
+
+    // The whole pass again on the resident positions: tree, bodies, walk.
+    gr_status again() {
+        const uint32_t C = g.C;
+        gr_status r;
+        if ((r = tree())) return r;
+        if (g.C != C) return fail(GR_ERR_DEVICE, "tree", "the same positions gave another tree");
+        if ((r = bodies())) return r;
+        gr::d3_launch_walk(g);
+        HIP_TRY(hipGetLastError());
+        return GR_OK;
+    }
+};
+
+// Writes the deltas when `out` is given and the node ids when `node_id` is given and
+// cap >= the cell count (*ncells_out); with neither it is a size query. reps > 0 then times
+// the whole pass and the walk alone on the resident positions (ms_out[0], ms_out[1]: medians
+// of three timings of `reps` runs) and gives the cell count (ms_out[2]).
 static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double alpha,
                          double dmin, double dmax, uint32_t seed, uint32_t tick, double *out,
-                         uint32_t cap, uint32_t *ncells_out, uint32_t *node_id) {
-    gr::FaDev &d = s->d;
-    GrState st;
+                         uint32_t cap, uint32_t *ncells_out, uint32_t *node_id, uint32_t reps,
+                         double *ms_out) {
+    D3Pass p{s, {}, {}};
     gr_status r;
-    for (int tries = 0; tries < 8; tries++) {
-        gr::launch_tree(d);
-        HIP_TRY(hipGetLastError());
-        if ((r = s->read_state(st))) return r;
-        if (!st.halt) break;
-        if ((r = s->recover(st))) return r;
-    }
-    if (st.halt) return fail(GR_ERR_CAPACITY, "tree", "cell capacity");
-    const uint32_t C = st.n_cells, n = d.n;
+    if ((r = p.tree())) return r;
+    gr::D3Dev &g = p.g;
+    const uint32_t C = g.C, n = g.n;
     if (ncells_out) *ncells_out = C;
     const bool ids = node_id && cap >= C;
     if (node_id && !ids) return fail(GR_ERR_CAPACITY, "cap", "cap >= *ncells_out node ids");
     if (!out && !ids) return GR_OK;
-    gr::D3Dev g;
-    g.n = n;
-    g.nf = st.n_finite;
-    g.C = C;
-    g.wc = d.wc;
-    g.order = d.order;
-    g.sx = (const double *)d.sp[0];
-    g.sy = (const double *)d.sp[1];
-    g.cb = d.cbounds;
     g.theta = theta;
     g.theta2 = theta * theta;
     g.dmin2 = dmin * dmin;
@@ -1550,49 +1630,25 @@ static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double 
     g.alpha = alpha;
     g.seed = seed;
     g.tick = tick;
-    g.stream = d.stream;
-    const size_t c = C ? C : 1;
-    if ((r = s->alloc(&g.body, c * sizeof(gr::D3Body))) || (r = s->alloc(&g.diff, (c + 1) * 4)) ||
-        (r = s->alloc(&g.depth, c * 4)) || (r = s->alloc(&g.amin, c * 4)) ||
-        (r = s->alloc(&g.bmin, c * 4)) || (r = s->alloc(&g.hist, (c + 1) * 4)) ||
-        (r = s->alloc(&g.maxd, 4)) || (r = s->alloc(&g.bucket, c * 4)) ||
-        (r = s->alloc(&g.keys, c * 8)) || (r = s->alloc(&g.keys_alt, c * 8)) ||
-        (r = s->alloc(&g.vals, c * 4)) || (r = s->alloc(&g.vals_alt, c * 4)) ||
-        (r = s->alloc(&g.tmp, gr::d3_tmp_bytes(C))) ||
-        (r = s->alloc(&g.out, (size_t)(n ? n : 1) * 16)) ||
-        (ids && (r = s->alloc(&g.nid_out, c * 4))))
-        return r;
-    std::vector<uint32_t> cnt, off;  // read by copies still queued on the stream
-    if (C > 0) {
-        gr::d3_launch_cells(g);
-        gr::d3_launch_depth(g);
-        HIP_TRY(hipGetLastError());
-        // the internal cells bucketed by depth, then summed deepest first
-        uint32_t maxd = 0;
-        HIP_TRY(hipMemcpyAsync(&maxd, g.maxd, 4, hipMemcpyDeviceToHost, d.stream));
-        HIP_TRY(hipStreamSynchronize(d.stream));
-        if (maxd >= C) return fail(GR_ERR_DEVICE, "tree", "a cell deeper than the arena is long");
-        cnt.assign(maxd + 1, 0);
-        off.assign(maxd + 1, 0);
-        HIP_TRY(hipMemcpy(cnt.data(), g.hist, cnt.size() * 4, hipMemcpyDeviceToHost));
-        uint32_t acc = 0;
-        for (uint32_t l = 0; l <= maxd; l++) {
-            off[l] = acc;
-            acc += cnt[l];
-        }
-        if (acc > C) return fail(GR_ERR_DEVICE, "tree", "more internal cells than cells");
-        HIP_TRY(hipMemcpyAsync(g.hist, off.data(), off.size() * 4, hipMemcpyHostToDevice,
-                               d.stream));
-        gr::d3_launch_bucket(g);
-        for (uint32_t l = maxd + 1; l-- > 0;)
-            if (cnt[l]) gr::d3_launch_level(g, off[l], cnt[l]);
-        gr::d3_launch_ids(g);
-    }
+    if ((r = p.alloc(ids)) || (r = p.bodies())) return r;
     if (out) gr::d3_launch_walk(g);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(d.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     if (out) HIP_TRY(hipMemcpy(out, g.out, (size_t)n * 16, hipMemcpyDeviceToHost));
     if (ids && C) HIP_TRY(hipMemcpy(node_id, g.nid_out, (size_t)C * 4, hipMemcpyDeviceToHost));
+    if (reps && ms_out) {
+        gr_status e = GR_OK;
+        if ((r = time_median(s, reps, [&] { if (!e) e = p.again(); }, &ms_out[0]))) return r;
+        if (e) return e;
+        if ((r = time_median(s, reps, [&] { gr::d3_launch_walk(g); }, &ms_out[1]))) return r;
+        ms_out[2] = (double)C;
+        // the timed runs recomputed the same deltas: they must carry the same bits
+        std::vector<double> again((size_t)n * 2);
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        HIP_TRY(hipMemcpy(again.data(), g.out, (size_t)n * 16, hipMemcpyDeviceToHost));
+        if (out && std::memcmp(again.data(), out, (size_t)n * 16) != 0)
+            return fail(GR_ERR_DEVICE, "d3", "a timed run gave other deltas");
+    }
     return GR_OK;
 }
 
@@ -1621,7 +1677,27 @@ gr_status gr_d3_charge_bh(uint32_t n, const double *xy, double theta, double cha
     gr_fa2_session *s = d3_session(n, xy);
     if (!s) return g_err.code;
     gr_status r = d3_pass(s, theta, charge, alpha, distance_min, distance_max, seed, tick, out, 0,
-                          nullptr, nullptr);
+                          nullptr, nullptr, 0, nullptr);
+    gr_fa2_session_release(s);
+    return r ? r : ok();
+}
+
+gr_status gr_d3_charge_bh_time(uint32_t n, const double *xy, double theta, double charge,
+                               double alpha, double distance_min, double distance_max,
+                               uint32_t seed, uint32_t tick, uint32_t reps, double *out,
+                               double *ms_out) {
+    if (!ms_out || reps == 0) return fail(GR_ERR_PARAM, "reps", "reps >= 1 and ms_out[3]");
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (!std::isfinite(charge) || !std::isfinite(alpha) || !(distance_min >= 0.0) ||
+        !(distance_max >= 0.0) || std::isnan(distance_max))
+        return fail(GR_ERR_PARAM, "charge", "finite charge and alpha, distances >= 0");
+    if (!(theta >= 0.0)) return fail(GR_ERR_PARAM, "theta", "an opening angle >= 0");
+    for (int k = 0; k < 3; k++) ms_out[k] = 0.0;
+    if (n == 0) return ok();
+    gr_fa2_session *s = d3_session(n, xy);
+    if (!s) return g_err.code;
+    gr_status r = d3_pass(s, theta, charge, alpha, distance_min, distance_max, seed, tick, out, 0,
+                          nullptr, nullptr, reps, ms_out);
     gr_fa2_session_release(s);
     return r ? r : ok();
 }
@@ -1633,7 +1709,8 @@ gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint3
     if (n == 0) return ok();
     gr_fa2_session *s = d3_session(n, xy);
     if (!s) return g_err.code;
-    gr_status r = d3_pass(s, 0.9, -90.0, 1.0, 1.0, 520.0, 0, 0, nullptr, cap, ncells_out, node_id);
+    gr_status r = d3_pass(s, 0.9, -90.0, 1.0, 1.0, 520.0, 0, 0, nullptr, cap, ncells_out, node_id,
+                          0, nullptr);
     gr_fa2_session_release(s);
     return r ? r : ok();
 }

@@ -293,6 +293,17 @@ gr_status gr_d3_charge_bh(uint32_t n, const double *xy, double theta, double cha
                           double alpha, double distance_min, double distance_max, uint32_t seed,
                           uint32_t tick, double *out);
 
+/* gr_d3_charge_bh, then the pass timed on the device-resident positions (no reference
+ * counterpart; the timing face gr_fa2_session_time gives the FA2 stages): ms_out[0] the whole
+ * pass (tree build, bodies, node ids, walk), ms_out[1] the walk alone, each the median of
+ * three timings of `reps` runs with HIP events on the pass's stream, and ms_out[2] the cell
+ * count. Every timed run recomputes the deltas in `out`; a run that gives other bits is
+ * reported as a device failure. */
+gr_status gr_d3_charge_bh_time(uint32_t n, const double *xy, double theta, double charge,
+                               double alpha, double distance_min, double distance_max,
+                               uint32_t seed, uint32_t tick, uint32_t reps, double *out,
+                               double *ms_out);
+
 /* The node id of every cell of the quadtree of xy, in preorder (Quadtree::node_id,
  * quadtree.rs:64-88 via preorder.rs's open): the shape slot the reference's insertion
  * (quadtree/build.rs:25-115) pushed the cell as, derived here from the arena a bulk build

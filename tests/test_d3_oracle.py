@@ -105,8 +105,8 @@ def insertion_ids(pts):
     return out
 
 
-def rank_ids(tree):
-    """Node ids as the rank of (creating insertion, depth, leaf flag), from the arena alone."""
+def pushers(tree):
+    """Per cell: the point whose insertion pushed it, its depth and whether it is a leaf."""
     C = tree.ncells
     skip, start, end, order = tree.skip, tree.start, tree.end, tree.order
     depth = np.zeros(C, np.int64)
@@ -134,7 +134,13 @@ def rank_ids(tree):
             first = next(c for c in kids if a[c] == a[k])
             b[k] = min([b[first]] + [a[c] for c in kids if c != first])
     leaf = np.array([skip[k] == k + 1 for k in range(C)])
-    t = np.where(leaf, a, b)
+    return np.where(leaf, a, b), depth, leaf
+
+
+def rank_ids(tree):
+    """Node ids as the rank of (creating insertion, depth, leaf flag), from the arena alone."""
+    C = tree.ncells
+    t, depth, leaf = pushers(tree)
     key = sorted(range(C), key=lambda k: (t[k], depth[k], leaf[k]))
     ids = np.zeros(C, np.int64)
     for r, k in enumerate(key):
@@ -156,12 +162,41 @@ def test_node_ids_replay_the_insertion(oracle):
     assert sorted(ids.tolist()) == list(range(t.ncells)) and ids.max() >= 4
 
 
+def count_ids(tree, n):
+    """The same ids as the device forms them, without a sort: cells pushed by earlier points,
+    plus the rank inside the point's own group (its internal cells sit at consecutive depths;
+    its leaf, which later splits may have moved down, comes last)."""
+    t, depth, leaf = pushers(tree)
+    cnt = np.bincount(t, minlength=n)
+    base = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    low = np.full(n, 1 << 40)
+    np.minimum.at(low, t, depth)
+    return np.where(leaf, base[t] + cnt[t] - 1, base[t] + depth - low[t])
+
+
 @pytest.mark.parametrize("seed", range(8))
 def test_node_ids_are_the_creation_rank(oracle, seed):
     pts = tricky_points(seed, 150 + 31 * seed, 2)
     t = oracle.Tree(pts)
     t.pos = pts
     assert np.array_equal(t.node_ids(), rank_ids(t))
+    assert np.array_equal(t.node_ids(), count_ids(t, len(pts)))
+
+
+def test_a_moved_leaf_keeps_the_last_rank_of_its_group(oracle):
+    # a leaf pushed below its point's internal cells and then moved down by later splits:
+    # its depth is no longer next to theirs, so a rank by depth offset would skip ids
+    pts = tricky_points(0, 60, 2)
+    t = oracle.Tree(pts)
+    t.pos = pts
+    tt, depth, leaf = pushers(t)
+    moved = [k for k in range(t.ncells) if leaf[k] and
+             any(tt[j] == tt[k] and not leaf[j] and depth[k] - depth[j] > 1
+                 and depth[j] == max(depth[q] for q in range(t.ncells)
+                                     if tt[q] == tt[k] and not leaf[q])
+                 for j in range(t.ncells))]
+    assert moved
+    assert np.array_equal(t.node_ids(), count_ids(t, len(pts)))
 
 
 def d3_direct(xy, charge, alpha, dmin, dmax, oracle, seed=0, tick=0):

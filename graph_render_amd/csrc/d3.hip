@@ -184,11 +184,11 @@ __device__ __forceinline__ bool d3_settle(const D3Dev &d, uint32_t i, uint32_t k
                                           double &dy, double &l) {
     if (l >= d.dmax2) return false;
     if (dx == 0.0) {
-        dx = d_jiggle(d.seed, d.tick, 2, i, key);
+        dx = d_jiggle_inline(d.seed, d.tick, 2, i, key);
         l += dx * dx;
     }
     if (dy == 0.0) {
-        dy = d_jiggle(d.seed, d.tick, 3, i, key);
+        dy = d_jiggle_inline(d.seed, d.tick, 3, i, key);
         l += dy * dy;
     }
     if (l < d.dmin2) l = sqrt(d.dmin2 * l);
@@ -211,29 +211,37 @@ __device__ __forceinline__ uint32_t d3_wave_min(uint32_t v) {
 // wave (a scalar load).
 constexpr uint32_t D3_WALK = 256;
 
-__global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
+// The arrays are kernel arguments of their own, marked __restrict__: with the stores to
+// `out` known not to touch the bodies, the uniform body reads become scalar loads.
+__global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d,
+                                                     const D3Body *__restrict__ body,
+                                                     const uint32_t *__restrict__ order,
+                                                     const double *__restrict__ sx,
+                                                     const double *__restrict__ sy,
+                                                     double *__restrict__ out) {
     const uint32_t nb = (d.n + D3_WALK - 1) / D3_WALK;
     const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
     if (lb >= nb) return;
     const uint32_t p = lb * D3_WALK + threadIdx.x;
     if (p < d.n && p >= d.nf) {
-        const uint32_t i = d.order[p];
-        d.out[2 * (size_t)i] = NAN;
-        d.out[2 * (size_t)i + 1] = NAN;
+        const uint32_t i = order[p];
+        out[2 * (size_t)i] = NAN;
+        out[2 * (size_t)i + 1] = NAN;
     }
     if (__ballot(p < d.nf) == 0) return;
     const bool live = p < d.nf;
     const uint32_t pc = live ? p : d.nf - 1;
-    const double xi = d.sx[pc], yi = d.sy[pc];
-    const uint32_t i = d.order[pc];
+    const double xi = sx[pc], yi = sy[pc];
+    const uint32_t i = order[pc];
     const uint32_t C = d.C;
-    const D3Body *__restrict__ body = d.body;
     uint32_t next = live ? 0u : 0xFFFFFFFFu;
     double ox = 0.0, oy = 0.0;
     uint32_t k = gr_uniform(d3_wave_min(next));
+    D3Body bd = body[k < C ? k : 0u];
     while (k < C) {
-        const D3Body bd = body[k];
         const uint32_t k1 = k + 1;
+        // most steps descend: the next body is read while this one is worked on
+        const D3Body b1 = body[k1 < C ? k1 : k];
         if (next == k) {
             double dx = bd.comx - xi, dy = bd.comy - yi;
             double l = dx * dx + dy * dy;
@@ -244,7 +252,7 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
                 if (bd.skip != k1) {
                     add = false;
                 } else {
-                    const uint32_t *chain = d.order + bd.start;
+                    const uint32_t *chain = order + bd.start;
                     if (d3_settle(d, i, chain[0], dx, dy, l)) {
                         const double f = d.charge * d.alpha / l;
                         for (uint32_t q = 0; q < bd.count; q++) {
@@ -269,12 +277,17 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d) {
             }
             next = nx;
         }
-        // most steps descend: k + 1 when any lane waits for it, else the smallest wait
-        k = __ballot(next == k1) != 0 ? k1 : gr_uniform(d3_wave_min(next));
+        // k + 1 when any lane waits for it, else the smallest wait
+        const uint32_t kn = __ballot(next == k1) != 0 ? k1 : gr_uniform(d3_wave_min(next));
+        if (kn == k1)
+            bd = b1;
+        else if (kn < C)
+            bd = body[kn];
+        k = kn;
     }
     if (live) {
-        d.out[2 * (size_t)i] = ox;
-        d.out[2 * (size_t)i + 1] = oy;
+        out[2 * (size_t)i] = ox;
+        out[2 * (size_t)i + 1] = oy;
     }
 }
 
@@ -320,7 +333,8 @@ void d3_launch_ids(const D3Dev &d) {
 
 void d3_launch_walk(const D3Dev &d) {
     const uint32_t nb = gr_blocks(d.n, D3_WALK);
-    hipLaunchKernelGGL(k_d3_walk, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, d.stream, d);
+    hipLaunchKernelGGL(k_d3_walk, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, d.stream, d,
+                       (const D3Body *)d.body, d.order, d.sx, d.sy, d.out);
 }
 
 }  // namespace gr

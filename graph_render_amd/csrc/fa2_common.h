@@ -60,8 +60,8 @@ __device__ __forceinline__ uint64_t d_fmix64(uint64_t k) {
     return k;
 }
 
-static __device__ __noinline__ double d_jiggle(uint32_t seed, uint32_t tick, uint32_t pass,
-                                               uint32_t i, uint32_t j) {
+__device__ __forceinline__ double d_jiggle_inline(uint32_t seed, uint32_t tick, uint32_t pass,
+                                                 uint32_t i, uint32_t j) {
     uint32_t lo = i <= j ? i : j, hi = i <= j ? j : i;
     uint64_t a = d_fmix64((uint64_t)seed ^ ((uint64_t)tick << 32));
     uint64_t b = d_fmix64(a ^ ((uint64_t)pass << 32) ^ (uint64_t)lo);
@@ -69,6 +69,12 @@ static __device__ __noinline__ double d_jiggle(uint32_t seed, uint32_t tick, uin
     if (w == (1ull << 52)) w = (1ull << 52) + 1;
     double u = (double)w * (1.0 / (double)(1ull << 53));
     return (u - 0.5) * 1e-6;
+}
+
+// The same, kept out of line: the FA2 walks take it on rare branches only.
+static __device__ __noinline__ double d_jiggle(uint32_t seed, uint32_t tick, uint32_t pass,
+                                               uint32_t i, uint32_t j) {
+    return d_jiggle_inline(seed, tick, pass, i, j);
 }
 
 // Iteration kernels stop once the run has exited early or a capacity check failed; a

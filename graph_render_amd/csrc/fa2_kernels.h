@@ -158,6 +158,8 @@ void launch_iteration(FaDev &d);
 void launch_forces(FaDev &d, uint32_t parts);
 // The tree build alone (after launch_bounds): keys, sort, topology, centre of mass.
 void launch_tree(FaDev &d);
+// The tree's arena alone (cells, bounds, order), without the centres of mass.
+void launch_tree_arena(FaDev &d);
 // The walk alone over the current tree, for timing.
 void launch_walk(FaDev &d);
 // The force finish alone (attraction, gravity and the swing/traction terms over the last

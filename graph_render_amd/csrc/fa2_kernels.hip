@@ -2298,7 +2298,7 @@ static void head_impl(FaDev &d) {
 }
 
 template <typename T, int D>
-static void tree_impl(FaDev &d) {
+static void tree_impl(FaDev &d, bool com = true) {
     const uint32_t n = d.n;
     const uint32_t nbl = gr_blocks(n, GR_BLOCK);
     hipStream_t s = d.stream;
@@ -2320,6 +2320,7 @@ static void tree_impl(FaDev &d) {
     hipLaunchKernelGGL((k_group_info<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
     scan_exclusive(d.gcells, d.gbase, n, &d.st->n_groups, &d.st->need_cells, d.scan_tmp, s);
     hipLaunchKernelGGL((k_emit<T, D>), dim3(nbl), dim3(GR_BLOCK), 0, s, d);
+    if (!com) return;  // the arena alone
     const uint32_t ntiles = gr_blocks(d.cap, COM_TILE);
     hipLaunchKernelGGL((k_com_tiles<T, D>), dim3(ntiles), dim3(COM_THREADS), 0, s, d);
     const uint32_t span_blocks = gr_blocks((uint64_t)ntiles * SPAN_SLOTS, GR_BLOCK);
@@ -2437,8 +2438,8 @@ static void bounds_impl(FaDev &d) {
 }
 
 template <typename T, int D, bool KAHAN>
-static void tree_only_impl(FaDev &d) {
-    tree_impl<T, D>(d);
+static void tree_only_impl(FaDev &d, bool com) {
+    tree_impl<T, D>(d, com);
 }
 
 void launch_bounds(FaDev &d) { GR_DISPATCH(bounds_impl, d); }
@@ -2451,7 +2452,12 @@ void launch_forces(FaDev &dd, uint32_t parts) {
 void launch_tree(FaDev &dd) {
     FaDev d = dd;
     d.ignore_done = 1;
-    GR_DISPATCH(tree_only_impl, d);
+    GR_DISPATCH(tree_only_impl, d, true);
+}
+void launch_tree_arena(FaDev &dd) {
+    FaDev d = dd;
+    d.ignore_done = 1;
+    GR_DISPATCH(tree_only_impl, d, false);
 }
 void launch_walk(FaDev &dd) {
     FaDev d = dd;

@@ -1524,7 +1524,7 @@ struct D3Pass {
         GrState st;
         gr_status r;
         for (int tries = 0; tries < 8; tries++) {
-            gr::launch_tree(d);
+            gr::launch_tree_arena(d);  // d3 forms its own bodies
             HIP_TRY(hipGetLastError());
             if ((r = s->read_state(st))) return r;
             if (!st.halt) break;

@@ -212,10 +212,10 @@ pub enum Refusal {
     Device { code: c_int },
 }
 
-const PARAM_NAMES: [&str; 13] = ["max_iter", "jitter_tolerance", "scaling_ratio", "gravity",
+const PARAM_NAMES: [&str; 16] = ["max_iter", "jitter_tolerance", "scaling_ratio", "gravity",
                                  "seed", "dim", "params", "theta", "precision", "repulsion",
-                                 "pm_grid", "edges", "edge_weights"];
-const CAPACITY_WHAT: [&str; 5] = ["nodes", "edges", "tree", "n", "side"];
+                                 "pm_grid", "edges", "edge_weights", "charge", "xy", "reps"];
+const CAPACITY_WHAT: [&str; 6] = ["nodes", "edges", "tree", "n", "side", "cap"];
 const SNAPSHOT_FIELDS: [&str; 6] = ["node.id", "edge.id", "note", "note.code", "note.index",
                                     "snapshot"];
 const RULE_OUT_OF_RANGE: &str = "inside the range the schema publishes, never clamped";
@@ -285,6 +285,38 @@ pub fn layout_run(layout_id: &str, n: u32, source: &[u32], target: &[u32], wire:
     };
     match refusal(code) {
         None => Ok(Columns { dim, x, y, z }),
+        Some(r) => Err(r),
+    }
+}
+
+/// The many-body constants one d3 charge pass reads (barnes_hut/charge.rs `Ctx`).
+#[derive(Clone, Copy, Debug, PartialEq)]
+pub struct D3Charge {
+    pub theta: f64,
+    pub charge: f64,
+    pub alpha: f64,
+    pub distance_min: f64,
+    pub distance_max: f64,
+    pub seed: u32,
+    pub tick: u32,
+}
+
+/// One d3 many-body pass over the quadtree of `(x, y)` on the device (`gr_d3_charge_bh`):
+/// the velocity each node gains, in node order.
+pub fn d3_charge_bh(x: &[f64], y: &[f64], c: &D3Charge) -> Result<Vec<(f64, f64)>, Refusal> {
+    if x.len() != y.len() {
+        return Err(Refusal::Param { name: "xy", rule: "equal x and y columns" });
+    }
+    let n = u32::try_from(x.len()).map_err(|_| Refusal::Capacity { what: "nodes" })?;
+    let xy: Vec<f64> = x.iter().zip(y).flat_map(|(&a, &b)| [a, b]).collect();
+    let mut out = vec![0f64; xy.len()];
+    // SAFETY: xy and out hold 2n values each; the library borrows them for the call.
+    let code = unsafe {
+        gr_d3_charge_bh(n, xy.as_ptr(), c.theta, c.charge, c.alpha, c.distance_min,
+                        c.distance_max, c.seed, c.tick, out.as_mut_ptr())
+    };
+    match refusal(code) {
+        None => Ok(out.chunks_exact(2).map(|d| (d[0], d[1])).collect()),
         Some(r) => Err(r),
     }
 }

@@ -73,6 +73,24 @@ def lib():
         L.gro_tree_order.argtypes = [C.c_void_p, u32p]
         L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
         L.gro_tree_node_ids.argtypes = [C.c_void_p, u32p]
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.gro_pcg64_state.argtypes = [C.c_uint64, u64p]
+        L.gro_pcg64_raw.argtypes = [C.c_uint64, C.c_uint32, u64p]
+        L.gro_pcg64_doubles.argtypes = [C.c_uint64, C.c_uint32, f64p]
+        L.gro_pairwise_f32.restype = C.c_float
+        L.gro_pairwise_f32.argtypes = [f32p, C.c_size_t]
+        L.gro_pairwise_f64.restype = C.c_double
+        L.gro_pairwise_f64.argtypes = [f64p, C.c_size_t]
+        L.gro_dot3_f32.restype = C.c_float
+        L.gro_dot3_f32.argtypes = [f32p, f32p]
+        L.gro_cbrt_cr.restype = C.c_double
+        L.gro_cbrt_cr.argtypes = [C.c_double]
+        L.gro_forcesim_constants.argtypes = [C.c_uint32, f32p, C.c_double, C.c_double, f64p]
+        L.gro_forcesim_run.restype = C.c_int
+        L.gro_forcesim_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_uint64, f32p, f64p]
         L.gro_d3_charge_bh.argtypes = [C.c_uint32, f64p, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_uint32, C.c_uint32, f64p]
         L.gro_pm_h.restype = C.c_double
@@ -293,3 +311,69 @@ def d3_charge_bh(xy: np.ndarray, theta: float = 0.9, charge: float = -30.0,
     lib().gro_d3_charge_bh(n, xy.reshape(-1) if n else np.zeros(2), theta, charge, alpha,
                            distance_min, distance_max, seed, tick, out)
     return out[: 2 * n].reshape(n, 2)
+
+
+# ---- layout.forceatlas2.forcesim (oracle/forcesim_oracle.c) ----
+
+FORCESIM_SEED = 1_767_573_729
+
+
+def pcg64_state(seed: int):
+    """numpy's default_rng(seed) after seeding: (state, inc) as Python ints."""
+    out = np.zeros(4, np.uint64)
+    lib().gro_pcg64_state(seed, out)
+    return (int(out[0]) << 64) | int(out[1]), (int(out[2]) << 64) | int(out[3])
+
+
+def pcg64_raw(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(max(count, 1), np.uint64)
+    lib().gro_pcg64_raw(seed, count, out)
+    return out[:count]
+
+
+def pcg64_doubles(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(max(count, 1))
+    lib().gro_pcg64_doubles(seed, count, out)
+    return out[:count]
+
+
+def pairwise_f32(v) -> np.float32:
+    v = np.ascontiguousarray(v, np.float32)
+    return np.float32(lib().gro_pairwise_f32(v if v.size else np.zeros(1, np.float32), v.size))
+
+
+def pairwise_f64(v) -> float:
+    v = np.ascontiguousarray(v, np.float64)
+    return lib().gro_pairwise_f64(v if v.size else np.zeros(1), v.size)
+
+
+def dot3_f32(a, b) -> np.float32:
+    return np.float32(lib().gro_dot3_f32(np.ascontiguousarray(a, np.float32),
+                                         np.ascontiguousarray(b, np.float32)))
+
+
+def cbrt_cr(x: float) -> float:
+    return lib().gro_cbrt_cr(float(x))
+
+
+def forcesim_constants(mass, scaling_ratio=2.0, gravity=1.0):
+    mass = np.ascontiguousarray(mass, np.float32)
+    out = np.zeros(3)
+    lib().gro_forcesim_constants(mass.size, mass if mass.size else np.zeros(1, np.float32),
+                                 scaling_ratio, gravity, out)
+    return tuple(out)
+
+
+def forcesim(n: int, source, target, iterations: int = 50, scale: float = 5.0,
+             scaling_ratio: float = 2.0, gravity: float = 1.0, jitter_tolerance: float = 1.0,
+             seed: int = FORCESIM_SEED):
+    """layout.forceatlas2.forcesim's answer: (xyz (n, 3) f32, status, (k, repulsion, gravity)).
+    status 0, or 2 for NonFinite{node.x} (the columns are then zero)."""
+    s = np.ascontiguousarray(source, np.uint32)
+    t = np.ascontiguousarray(target, np.uint32)
+    out = np.zeros(max(n, 1) * 3, np.float32)
+    cst = np.zeros(3)
+    st = lib().gro_forcesim_run(n, s.ctypes.data if s.size else None,
+                                t.ctypes.data if t.size else None, s.size, iterations, scale,
+                                scaling_ratio, gravity, jitter_tolerance, seed, out, cst)
+    return out[: 3 * n].reshape(n, 3), st, tuple(cst)

@@ -89,6 +89,18 @@ pub struct gr_fa2_schedule {
     pub finish_split: u32,
 }
 
+/// `Fa2ForceSimParams` (layout/force/forcesim.rs:104-153), field for field.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, PartialEq)]
+pub struct gr_forcesim_params {
+    pub iterations: u32,
+    pub scale: f64,
+    pub scaling_ratio: f64,
+    pub gravity: f64,
+    pub jitter_tolerance: f64,
+    pub seed: u64,
+}
+
 /// An opaque device session (`struct gr_fa2_session`).
 #[repr(C)]
 pub struct gr_fa2_session {
@@ -162,6 +174,19 @@ extern "C" {
                                 reps: u32, out: *mut f64, ms_out: *mut f64) -> c_int;
     pub fn gr_quadtree_node_ids(n: u32, xy: *const f64, cap: u32, ncells_out: *mut u32,
                                 node_id: *mut u32) -> c_int;
+    pub fn gr_forcesim_params_default(out: *mut gr_forcesim_params);
+    pub fn gr_forcesim_run(n: u32, source: *const u32, target: *const u32, m: usize,
+                           params: *const gr_forcesim_params, out_x: *mut f32, out_y: *mut f32,
+                           out_z: *mut f32) -> c_int;
+    pub fn gr_forcesim_batch(ngraphs: u32, node_counts: *const u32, edge_counts: *const usize,
+                             source: *const u32, target: *const u32,
+                             params: *const gr_forcesim_params, out_x: *mut f32,
+                             out_y: *mut f32, out_z: *mut f32, status: *mut i32) -> c_int;
+    pub fn gr_forcesim_batch_time(ngraphs: u32, node_counts: *const u32,
+                                  edge_counts: *const usize, source: *const u32,
+                                  target: *const u32, params: *const gr_forcesim_params,
+                                  out_x: *mut f32, out_y: *mut f32, out_z: *mut f32,
+                                  status: *mut i32, reps: u32, ms_out: *mut f64) -> c_int;
     pub fn gr_last_error(name: *mut *const c_char, rule: *mut *const c_char) -> c_int;
     pub fn gr_version() -> *const c_char;
 }

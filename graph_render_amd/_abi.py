@@ -36,6 +36,13 @@ class GrFa2Params(C.Structure):
                 ("seed", C.c_uint32), ("dim", C.c_uint32)]
 
 
+class GrForceSimParams(C.Structure):
+    """gr_forcesim_params: Fa2ForceSimParams (layout/force/forcesim.rs:104-153)."""
+    _fields_ = [("iterations", C.c_uint32), ("scale", C.c_double),
+                ("scaling_ratio", C.c_double), ("gravity", C.c_double),
+                ("jitter_tolerance", C.c_double), ("seed", C.c_uint64)]
+
+
 class GrFa2Schedule(C.Structure):
     _fields_ = [("walk_block", C.c_uint32), ("walk_xcd", C.c_uint32),
                 ("sort_items", C.c_uint32), ("finish_split", C.c_uint32)]
@@ -98,6 +105,16 @@ SIGNATURES = {
                                        C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "gr_quadtree_node_ids": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32,
                                        C.POINTER(C.c_uint32), C.c_void_p]),
+    "gr_forcesim_params_default": (None, [C.c_void_p]),
+    "gr_forcesim_run": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gr_forcesim_batch": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "gr_forcesim_batch_time": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_uint32,
+                                         C.POINTER(C.c_double)]),
     "gr_snapshot_encode": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,

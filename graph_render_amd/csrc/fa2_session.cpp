@@ -421,6 +421,17 @@ static gr_status time_median(gr_fa2_session *s, uint32_t reps, F launch, double 
     return GR_OK;
 }
 
+// The simple graph (layout/force/mod.rs:70-180) for another translation unit of this
+// library: self-loops and repeated unordered pairs dropped, (lo, hi) in first-occurrence order.
+__attribute__((visibility("hidden"))) void gr_internal_simple_edges(
+    uint32_t n, const uint32_t *src, const uint32_t *dst, size_t m, std::vector<uint32_t> &lo,
+    std::vector<uint32_t> &hi) {
+    SimpleGraph g;
+    build_simple(n, src, dst, m, nullptr, g);
+    lo.swap(g.lo);
+    hi.swap(g.hi);
+}
+
 extern "C" {
 
 // Sets the calling thread's last error from another translation unit of this library.
@@ -1732,6 +1743,20 @@ gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *sourc
         return fail(GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id");
     }
     std::string id(layout_id);
+    if (id == "layout.forceatlas2.forcesim") {
+        // registered with no published parameters (registry/forceatlas2_forcesim.rs:67-72):
+        // the empty buffer is the defaults, any byte is malformed; no gr_fa2_ext applies
+        if (params_len != 0) {
+            zero();
+            return fail(GR_ERR_PARAMS_MALFORMED, "params",
+                        "one little-endian f64 per published parameter, and no other bytes");
+        }
+        if (ext) {
+            zero();
+            return fail(GR_ERR_UNSUPPORTED, "ext", "no extension applies to this layout");
+        }
+        return gr_forcesim_run(n, source, target, m, nullptr, out_x, out_y, out_z);
+    }
     uint32_t dim;
     int rep;
     if (id == "layout.forceatlas2") {

@@ -269,6 +269,19 @@ LAYOUTS: Dict[str, Capability] = {
 }
 
 
+def _forcesim_run(topology: Topology, params=None, ext=None) -> Geometry:
+    from .forcesim import ForceAtlas2ForceSim
+    if ext is not None:
+        raise StageError(A.GR_ERR_UNSUPPORTED, "ext", "no extension applies to this layout")
+    return ForceAtlas2ForceSim.run(topology, params)
+
+
+# registry/forceatlas2_forcesim.rs:67-72: no published parameters, ceiling 2000
+LAYOUTS["layout.forceatlas2.forcesim"] = Capability(
+    "layout.forceatlas2.forcesim", _forcesim_run, (), 2_000,
+    "SciGraphs' own ForceSim (model FA2, direct repulsion)")
+
+
 def find(layout_id: str) -> Capability:
     if layout_id not in LAYOUTS:
         raise StageError(A.GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id")
@@ -289,7 +302,7 @@ def run_params(layout_id: str, topology: Topology, wire: bytes,
     """registry/params.rs:144-160: the layout at the values a wire buffer carries (empty =
     defaults), refused (never clamped) when malformed or out of range."""
     find(layout_id)
-    dim = 3 if layout_id == ForceAtlas2_3d.ID else 2
+    dim = 3 if layout_id in (ForceAtlas2_3d.ID, "layout.forceatlas2.forcesim") else 2
     return _run_id(layout_id, topology, wire, ext, dim)
 
 

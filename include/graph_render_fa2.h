@@ -312,6 +312,51 @@ gr_status gr_d3_charge_bh_time(uint32_t n, const double *xy, double theta, doubl
 gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint32_t *ncells_out,
                                uint32_t *node_id);
 
+/* ---- layout.forceatlas2.forcesim (SciGraphs' ForceSim as the reference ports it) ---- */
+
+/* Fa2ForceSimParams (layout/force/forcesim.rs:104-153) and its Default: iterations 50 (at
+ * least one step always runs), scale 5 (the rescale's width only), scaling_ratio 2, gravity 1
+ * (times 0.1 inside), jitter_tolerance 1, seed 1767573729 (numpy default_rng's seed). */
+typedef struct gr_forcesim_params {
+    uint32_t iterations;
+    double scale;
+    double scaling_ratio;
+    double gravity;
+    double jitter_tolerance;
+    uint64_t seed;
+} gr_forcesim_params;
+
+void gr_forcesim_params_default(gr_forcesim_params *out);
+
+/* ForceAtlas2ForceSim::run (forcesim.rs:155-163) on the device: the simple graph of the raw
+ * (source, target) columns, f32 start positions from numpy's default_rng(seed), the FA2
+ * force field with direct repulsion (f64 coefficient rows), attraction and gravity, the
+ * adaptive speed and capped move, recentring, then the f64 rescale to `scale`; x, y, z as
+ * f32, bit for bit the reference's arithmetic and reduction orders. params NULL = defaults.
+ * Refuses a non-finite coordinate as GR_ERR_NON_FINITE (column node.x), outputs zeroed. */
+gr_status gr_forcesim_run(uint32_t n, const uint32_t *source, const uint32_t *target, size_t m,
+                          const gr_forcesim_params *params, float *out_x, float *out_y,
+                          float *out_z);
+
+/* Many graphs in one launch (one workgroup lays out one graph; no reference counterpart, the
+ * reference runs one graph per call): graph g has node_counts[g] nodes and edge_counts[g]
+ * raw edges (graph-local endpoints, concatenated in graph order); params holds ngraphs
+ * entries (NULL = defaults for all). Outputs concatenated in graph order; status[g] is 0, or
+ * 2 where the graph's coordinates are not finite (its columns then hold them unrefused). */
+gr_status gr_forcesim_batch(uint32_t ngraphs, const uint32_t *node_counts,
+                            const size_t *edge_counts, const uint32_t *source,
+                            const uint32_t *target, const gr_forcesim_params *params,
+                            float *out_x, float *out_y, float *out_z, int32_t *status);
+
+/* gr_forcesim_batch, then the whole batch timed from its device-resident inputs: *ms_out is
+ * the median of three timings of `reps` layouts (HIP events; each run copies the start
+ * positions in and clears the previous field). */
+gr_status gr_forcesim_batch_time(uint32_t ngraphs, const uint32_t *node_counts,
+                                 const size_t *edge_counts, const uint32_t *source,
+                                 const uint32_t *target, const gr_forcesim_params *params,
+                                 float *out_x, float *out_y, float *out_z, int32_t *status,
+                                 uint32_t reps, double *ms_out);
+
 /* Last failure of the calling thread: code, parameter/column name, rule text. */
 gr_status gr_last_error(const char **name, const char **rule);
 

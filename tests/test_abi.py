@@ -88,10 +88,31 @@ def test_unknown_layout_id():
     assert err.value.code == A.GR_ERR_UNKNOWN_LAYOUT
 
 
-def test_registry_lists_the_three_ids():
+def test_registry_lists_the_four_ids():
     assert set(G.LAYOUTS) == {"layout.forceatlas2", "layout.forceatlas2.barnes_hut",
-                              "layout.forceatlas2.3d"}
+                              "layout.forceatlas2.3d", "layout.forceatlas2.forcesim"}
     assert G.find("layout.forceatlas2.barnes_hut").scale_ceiling == 250_000
+    assert G.find("layout.forceatlas2.forcesim").scale_ceiling == 2_000
+    assert G.find("layout.forceatlas2.forcesim").params == ()
+
+
+def test_forcesim_publishes_no_parameters():
+    # registry/forceatlas2_forcesim.rs:67-72 registers it with LayoutParams::NONE: any byte of
+    # a wire buffer is malformed, refused before the device is touched
+    topo = G.Topology(4, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    with pytest.raises(G.StageError) as e:
+        G.run_params("layout.forceatlas2.forcesim", topo, b"\0" * 8)
+    assert e.value.code == 21
+
+
+def test_forcesim_defaults_are_the_references():
+    from graph_render_amd import forcesim as FS
+    p = A.GrForceSimParams()
+    A.lib().gr_forcesim_params_default(C.addressof(p))
+    assert (p.iterations, p.scale, p.scaling_ratio, p.gravity, p.jitter_tolerance, p.seed) == \
+        (50, 5.0, 2.0, 1.0, 1.0, 1_767_573_729)
+    d = FS.Fa2ForceSimParams()
+    assert (d.iterations, d.scale, d.seed) == (50, 5.0, 1_767_573_729)
 
 
 def test_a_bad_dim_is_refused_before_the_device():
@@ -199,7 +220,8 @@ def test_rust_binding_declares_every_header_function_with_its_arity():
 def test_rust_structs_follow_the_header_field_order():
     text = open(os.path.join(ROOT, "crates", "graph-render-fa2-sys", "src", "lib.rs")).read()
     for name, fields in (("gr_fa2_params", A.GrFa2Params._fields_),
-                         ("gr_fa2_ext", A.GrFa2Ext._fields_)):
+                         ("gr_fa2_ext", A.GrFa2Ext._fields_),
+                         ("gr_forcesim_params", A.GrForceSimParams._fields_)):
         body = text[text.index(f"pub struct {name} {{"):]
         body = body[:body.index("}")]
         got = re.findall(r"pub (\w+):", body)

@@ -514,6 +514,68 @@ def bench_d3(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_forcesim(args):
+    """layout.forceatlas2.forcesim over a batch of small graphs, one launch (a workgroup per
+    graph): graphs with n uniform in [10, 400] (the reference's DIRECT_MAX) and random edges
+    at mean degree 4, the stage's defaults (50 iterations) with a seed per graph. Layouts/s
+    with the inputs resident; every 2nd graph is checked bit for bit against the oracle and
+    timed there on one core, the CPU rate scaled by the batch's pair work (sum of n^2)."""
+    import graph_render_amd as G
+    from graph_render_amd import forcesim as FS
+    ng = args.graphs if args.graphs != 4096 else 1024
+    rng = np.random.default_rng(5)
+    topos, params = [], []
+    for b in range(ng):
+        n = int(rng.integers(10, 401))
+        m = 2 * n
+        topos.append(G.Topology(n, rng.integers(0, n, m).astype(np.uint32),
+                                rng.integers(0, n, m).astype(np.uint32)))
+        params.append(FS.Fa2ForceSimParams(seed=1_767_573_729 + b))
+    res, ms = FS.time_batch(topos, params, reps=max(args.steps, 1))
+    nodes = np.array([t.n for t in topos], np.float64)
+    pairs = float((nodes ** 2).sum()) * 50
+    # FP64 work per pair and iteration: the coefficient (2 mul, 1 div), three products and
+    # three sums, one pairwise add
+    flops = 10.0 * pairs
+    achieved = flops / (ms * 1e-3) / 1e12
+    print(f"forcesim batch {ng} graphs: {ms:.3f} ms", file=sys.stderr, flush=True)
+    line = {
+        "metric": "layout.forceatlas2.forcesim layouts/s, batch of small graphs (50 iterations)",
+        "value": round(ng / (ms * 1e-3), 2), "unit": "layouts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{ng} random graphs, n uniform 10..400, mean degree 4, "
+                               "forcesim defaults (50 iterations, seed per graph)",
+                   "graphs": ng, "nodes": int(nodes.sum()), "pairs_x_iterations": pairs},
+        "roofline": {"bound": "fp64", "achieved": round(achieved, 3), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": round(achieved / 78.6, 5), "traffic": None,
+                     "kernel": "k_fs_layout", "algorithmic_flops": flops},
+        "nonfinite_graphs": int(sum(st != 0 for _, st in res)),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        info = cpu_info()
+        core = pin_one_core()
+        sample = list(range(0, ng, 2))  # about 5 s of CPU work at the default 1024 graphs
+        same = True
+        t0 = time.perf_counter()
+        for b in sample:
+            t = topos[b]
+            want, _, _ = O.forcesim(t.n, t.source, t.target, seed=params[b].seed)
+            same &= bool(np.array_equal(want.view(np.uint32), res[b][0].view(np.uint32)))
+        el = time.perf_counter() - t0
+        sw = float((nodes[sample] ** 2).sum())
+        line["bit_exact_sample"] = same
+        line["cpu_baseline"] = {
+            "value": round(len(sample) / el * (sw / len(sample)) / ((nodes ** 2).sum() / ng), 3),
+            "unit": "layouts/s", "cores": 1, "kind": "port",
+            "sample": f"every 2nd graph ({len(sample)} graphs, {el:.1f}s) through "
+                      "oracle/forcesim_oracle.c on one core (core "
+                      f"{core}), scaled to the batch's mean pair work", **info}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -526,7 +588,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
-                                           "autotune", "d3"],
+                                           "autotune", "d3", "forcesim"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -548,6 +610,9 @@ def main():
         return
     if args.workload == "d3":
         bench_d3(args)
+        return
+    if args.workload == "forcesim":
+        bench_forcesim(args)
         return
 
     import graph_render_amd as G

@@ -198,6 +198,7 @@ struct FsDev {
     const uint32_t *aoff, *adj, *elo, *ehi, *tok;
     float *out;
     int32_t *status;
+    uint32_t lds_n;  // node records the LDS variant holds (the launch's largest graph)
 };
 
 constexpr uint32_t FS_BLOCK = 256;
@@ -270,7 +271,16 @@ __device__ __forceinline__ void fs_column_means(const float *p, uint32_t n, floa
     }
 }
 
+// LDS: every graph of the launch has at most FS_LDS_NODES nodes, and each iteration stages
+// its positions with their own dots (x, y, z, |p|^2) and the masses in LDS, where the rows
+// read them as broadcasts; otherwise the rows read global memory.
+constexpr uint32_t FS_LDS_NODES = 2048;
+
+template <bool LDS>
 __global__ __launch_bounds__(FS_BLOCK) void k_fs_layout(const FsDev d) {
+    extern __shared__ float4 fs_lds[];
+    float4 *Q = fs_lds;                          // LDS: n records (x, y, z, s2)
+    float *MS = (float *)(fs_lds + d.lds_n);     // LDS: n masses
     const FsGraph G = d.g[blockIdx.x];
     const uint32_t n = G.n, T = FS_BLOCK, tid = threadIdx.x;
     float *P = d.pos + 3 * (size_t)G.node0;
@@ -288,13 +298,18 @@ __global__ __launch_bounds__(FS_BLOCK) void k_fs_layout(const FsDev d) {
     __shared__ double mean64[3];
     __shared__ double spanw[FS_BLOCK / GR_WAVE];
     if (tid == 0) speed_s = 1.0;
+    if constexpr (LDS)
+        for (uint32_t j = tid; j < n; j += T) MS[j] = M[j];
     const uint32_t steps = G.iters > 0 ? G.iters : 1;
     for (uint32_t it = 0; it < steps && n >= 2; it++) {
         // center_was (before the field), and every row's own dot
         fs_column_means(P, n, cw);
-        for (uint32_t j = tid; j < n; j += T)
-            S2[j] = fs_dot3(P[3 * j], P[3 * j + 1], P[3 * j + 2], P[3 * j], P[3 * j + 1],
-                            P[3 * j + 2]);
+        for (uint32_t j = tid; j < n; j += T) {
+            const float x = P[3 * j], y = P[3 * j + 1], z = P[3 * j + 2];
+            const float s2 = fs_dot3(x, y, z, x, y, z);
+            S2[j] = s2;
+            if constexpr (LDS) Q[j] = make_float4(x, y, z, s2);
+        }
         __syncthreads();
         // the field, a row per thread: repulsion, then attraction, then gravity
         for (uint32_t i = tid; i < n; i += T) {
@@ -303,12 +318,26 @@ __global__ __launch_bounds__(FS_BLOCK) void k_fs_layout(const FsDev d) {
             const double mi = (double)M[i];
             double px = 0.0, py = 0.0, pz = 0.0;
             auto coef = [&](uint32_t j) -> double {
-                const float xj = P[3 * j], yj = P[3 * j + 1], zj = P[3 * j + 2];
-                float d2 = own + S2[j];
+                float xj, yj, zj, sj, mj;
+                if constexpr (LDS) {
+                    const float4 q = Q[j];
+                    xj = q.x;
+                    yj = q.y;
+                    zj = q.z;
+                    sj = q.w;
+                    mj = MS[j];
+                } else {
+                    xj = P[3 * j];
+                    yj = P[3 * j + 1];
+                    zj = P[3 * j + 2];
+                    sj = S2[j];
+                    mj = M[j];
+                }
+                float d2 = own + sj;
                 d2 = d2 - 2.0f * fs_dot3(xi, yi, zi, xj, yj, zj);
                 if (d2 < 0.0f) d2 = 0.0f;
                 d2 = d2 + G.soften;
-                double c = G.rep * mi * (double)M[j] / (double)d2;
+                double c = G.rep * mi * (double)mj / (double)d2;
                 if (j == i) c = 0.0;  // the diagonal, zeroed after the division
                 px += c * (double)xj;
                 py += c * (double)yj;
@@ -609,9 +638,17 @@ void fs_launch(const FsBatch &b, FsDevice &dv) {
     (void)hipMemcpyAsync(dv.d.pos, dv.start, N * 12, hipMemcpyDeviceToDevice, dv.stream);
     (void)hipMemsetAsync(dv.d.prev, 0, N * 12, dv.stream);
     (void)hipMemsetAsync(dv.d.status, 0, b.graphs.size() * 4, dv.stream);
-    if (!b.graphs.empty())
-        hipLaunchKernelGGL(k_fs_layout, dim3((uint32_t)b.graphs.size()), dim3(FS_BLOCK), 0,
-                           dv.stream, dv.d);
+    if (b.graphs.empty()) return;
+    uint32_t biggest = 0;
+    for (const FsGraph &g : b.graphs) biggest = std::max(biggest, g.n);
+    if (biggest <= FS_LDS_NODES) {
+        FsDev d = dv.d;
+        d.lds_n = std::max(biggest, 1u);
+        hipLaunchKernelGGL(k_fs_layout<true>, dim3((uint32_t)b.graphs.size()), dim3(FS_BLOCK),
+                           d.lds_n * (sizeof(float4) + sizeof(float)), dv.stream, d);
+    } else
+        hipLaunchKernelGGL(k_fs_layout<false>, dim3((uint32_t)b.graphs.size()), dim3(FS_BLOCK),
+                           0, dv.stream, dv.d);
 }
 
 gr_status fs_run(uint32_t ngraphs, const uint32_t *node_counts, const size_t *edge_counts,

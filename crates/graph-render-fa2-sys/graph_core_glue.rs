@@ -87,3 +87,43 @@ pub fn run_3d_device(topology: &Topology, params: &Fa2Params) -> Result<Geometry
     let p = Fa2Params { dim: 3, ..*params };
     run_id("layout.forceatlas2.3d", topology, &p)
 }
+
+/// `layout.forceatlas2.forcesim` on the device (forcesim.rs:155-163): the same `Fa2ForceSimParams`
+/// field for field; the registry row runs it with the defaults.
+pub struct ForceAtlas2ForceSimDevice;
+
+impl Stage for ForceAtlas2ForceSimDevice {
+    type Params = crate::layout::force::Fa2ForceSimParams;
+    const ID: &'static str = "layout.forceatlas2.forcesim";
+
+    fn run(topology: &Topology, p: &Self::Params) -> Result<Geometry, StageError> {
+        let edges = topology.edges();
+        let n = topology.node_count();
+        let c = sys::gr_forcesim_params {
+            iterations: p.iterations,
+            scale: p.scale,
+            scaling_ratio: p.scaling_ratio,
+            gravity: p.gravity,
+            jitter_tolerance: p.jitter_tolerance,
+            seed: p.seed,
+        };
+        let (mut x, mut y, mut z) = (vec![0f32; n as usize], vec![0f32; n as usize],
+                                     vec![0f32; n as usize]);
+        // SAFETY: the columns hold n values each; the edge columns are borrowed for the call.
+        let code = unsafe {
+            sys::gr_forcesim_run(n, edges.source.as_ptr(), edges.target.as_ptr(),
+                                 edges.source.len(), &c, x.as_mut_ptr(), y.as_mut_ptr(),
+                                 z.as_mut_ptr())
+        };
+        if let Some(r) = sys::refusal(code) {
+            return Err(stage_error(r));
+        }
+        Ok(Geometry::in_space(
+            graph_contract::geometry::NodeGeometry::Point { x, y },
+            graph_contract::geometry::EdgeGeometry::Line,
+            Vec::new(),
+            z,
+        ))
+    }
+}
+

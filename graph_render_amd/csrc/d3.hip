@@ -223,14 +223,10 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d,
     const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
     if (lb >= nb) return;
     const uint32_t p = lb * D3_WALK + threadIdx.x;
-    if (p < d.n && p >= d.nf) {
-        const uint32_t i = order[p];
-        out[2 * (size_t)i] = NAN;
-        out[2 * (size_t)i + 1] = NAN;
-    }
-    if (__ballot(p < d.nf) == 0) return;
-    const bool live = p < d.nf;
-    const uint32_t pc = live ? p : d.nf - 1;
+    // every node walks, the non-finite ones (tree positions nf..n, in no leaf) too
+    if (__ballot(p < d.n) == 0) return;
+    const bool live = p < d.n;
+    const uint32_t pc = live ? p : d.n - 1;
     const double xi = sx[pc], yi = sy[pc];
     const uint32_t i = order[pc];
     const uint32_t C = d.C;

@@ -70,6 +70,13 @@ double gro_jiggle(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t i, uint3
  * edges; an edge's strength is drawn after its endpoints and only when it is kept. The
  * seed remix (stage/topology.rs:37-46) redraws kinds only, so endpoints are seed-free. */
 size_t gro_gate_edges(uint32_t count, uint32_t *source, uint32_t *target, size_t cap) {
+    return gro_gate_model(count, source, target, NULL, cap);
+}
+
+/* synthetic.rs:92-170 with each kept edge's strength (0.4 + 0.4 * a draw, drawn after the
+ * endpoints; a self-edge draws none) */
+size_t gro_gate_model(uint32_t count, uint32_t *source, uint32_t *target, double *strength,
+                      size_t cap) {
     uint32_t st = 0x051042u;
     for (uint32_t i = 0; i < count; i++) {
         double roll = gro_mulberry32_next(&st);
@@ -85,8 +92,9 @@ size_t gro_gate_edges(uint32_t count, uint32_t *source, uint32_t *target, size_t
         if (a_ != b_ && m < cap) {                                                          \
             source[m] = a_;                                                                 \
             target[m] = b_;                                                                 \
+            const double s_ = 0.4 + gro_mulberry32_next(&st) * 0.4;                         \
+            if (strength) strength[m] = s_;                                                 \
             m++;                                                                            \
-            (void)gro_mulberry32_next(&st);                                                 \
         }                                                                                   \
     } while (0)
     for (uint32_t i = 1; i < count; i++) {
@@ -558,7 +566,12 @@ static void d3_node_delta(const d3ctx_t *c, uint32_t i, double *ox, double *oy) 
 /* The velocity each node gains from d3's many-body force over the quadtree of its
  * positions (barnes_hut/charge.rs:41-53 + 120-140): bodies in reverse preorder (a leaf at
  * its chain head's position, an internal cell at the count-weighted mean of its children's
- * centres in slot order), then every node's walk. xy: n*2 row-major; out: n*2. */
+ * centres in slot order), then every node's walk, non-finite nodes included. xy: n*2
+ * row-major; out: n*2. */
+static void d3_charge_core(uint32_t n, const double *xs, const double *ys, double theta,
+                           double charge, double alpha, double dmin, double dmax, uint32_t seed,
+                           uint32_t tick, double *out);
+
 void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge, double alpha,
                       double dmin, double dmax, uint32_t seed, uint32_t tick, double *out) {
     double *xs = (double *)malloc((n ? n : 1) * sizeof(double));
@@ -567,6 +580,15 @@ void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge,
         xs[i] = xy[2 * (size_t)i];
         ys[i] = xy[2 * (size_t)i + 1];
     }
+    d3_charge_core(n, xs, ys, theta, charge, alpha, dmin, dmax, seed, tick, out);
+    free(xs);
+    free(ys);
+}
+
+/* out: n*2, the delta of every node */
+static void d3_charge_core(uint32_t n, const double *xs, const double *ys, double theta,
+                           double charge, double alpha, double dmin, double dmax, uint32_t seed,
+                           uint32_t tick, double *out) {
     gro_tree *t = gro_tree_build(n, 2, xs, ys, NULL);
     const size_t nc = t->ncells;
     d3body_t *bodies = (d3body_t *)calloc(nc ? nc : 1, sizeof(d3body_t));
@@ -601,18 +623,226 @@ void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge,
         b->start = t->start[kk];
     }
     d3ctx_t c = {t, bodies, xs, ys, dmin * dmin, dmax * dmax, charge, alpha, seed, tick};
-    for (uint32_t i = 0; i < n; i++) {
-        if (!isfinite(xs[i]) || !isfinite(ys[i])) {
-            out[2 * (size_t)i] = NAN;
-            out[2 * (size_t)i + 1] = NAN;
-            continue;
-        }
+    /* every node walks, a non-finite one too (it is in no leaf, so its walk reads every
+     * cell as it would any far query: NaN gaps approximate, infinite ones fall past
+     * distanceMax) */
+    for (uint32_t i = 0; i < n; i++)
         d3_node_delta(&c, i, &out[2 * (size_t)i], &out[2 * (size_t)i + 1]);
-    }
     free(bodies);
     gro_tree_free(t);
-    free(xs);
-    free(ys);
+}
+
+/* ------------------------------------------------- d3's Barnes-Hut force simulation */
+
+/* collide.rs: the node's walk over the quadtree of the projected positions; a leaf's
+ * points are resolved pair by pair, an internal cell is skipped when its square lies
+ * beyond the reach on some axis */
+static void d3_collide_delta(const gro_tree *t, uint32_t i, const double *px, const double *py,
+                             double d2, uint32_t seed, uint32_t tick, double *ox, double *oy) {
+    const double xi = px[i], yi = py[i], reach = sqrt(d2);
+    double outx = 0.0, outy = 0.0;
+    uint32_t k = 0;
+    while (k < t->ncells) {
+        if (t->skip[k] == k + 1) {
+            for (uint32_t q = t->start[k]; q < t->end[k]; q++) {
+                const uint32_t p = t->order[q];
+                if (p == i) continue;
+                double dx = xi - px[p], dy = yi - py[p];
+                double l = dx * dx + dy * dy;
+                if (l >= d2) continue;
+                if (dx == 0.0) {
+                    dx = gro_jiggle(seed, tick, 4, i, p);
+                    l += dx * dx;
+                }
+                if (dy == 0.0) {
+                    dy = gro_jiggle(seed, tick, 5, i, p);
+                    l += dy * dy;
+                }
+                const double dist = sqrt(l);
+                const double push = (reach - dist) / dist * 0.5;
+                outx += dx * push;
+                outy += dy * push;
+            }
+            k = k + 1;
+        } else {
+            const box_t *b = &t->cbox[k];
+            const int outside = b->lo[0] > xi + reach || b->hi[0] < xi - reach ||
+                                b->lo[1] > yi + reach || b->hi[1] < yi - reach;
+            k = outside ? t->skip[k] : k + 1;
+        }
+    }
+    *ox = outx;
+    *oy = outy;
+}
+
+/* step.rs:200-214 (merge): every delta added to its own node's velocity, as d + 0 */
+static void d3_merge(uint32_t n, double *vx, double *vy, const double *d) {
+    for (uint32_t i = 0; i < n; i++) {
+        vx[i] += d[2 * (size_t)i] + 0.0;
+        vy[i] += d[2 * (size_t)i + 1] + 0.0;
+    }
+}
+
+/* The frozen layout.force.barnes_hut (barnes_hut.rs run_under + session.rs step_under +
+ * sim.rs tick): the simple graph with strengths, link geometry (distance / max(0.4, s),
+ * min(0.7, scale * s), degree bias), golden-spiral start, then `ticks` ticks of alpha decay,
+ * link, charge, center, collide, integrate. p: charge, theta, distance_min, distance_max,
+ * link_distance, link_strength_scale, collide_radius, center_strength, velocity_decay,
+ * alpha_decay, initial_alpha. x_out, y_out: n each. */
+void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
+                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
+                   double *y_out) {
+    const double charge = p[0], theta = p[1], dmin = p[2], dmax = p[3], ldist = p[4];
+    const double lscale = p[5], radius = p[6], cstrength = p[7], decay = p[8];
+    const double adecay = p[9];
+    double alpha = p[10];
+    /* simple graph (layout/force/mod.rs:101-120): first occurrence of a pair keeps its
+     * strength */
+    uint32_t *lo = (uint32_t *)malloc((m_raw ? m_raw : 1) * sizeof(uint32_t));
+    uint32_t *hi = (uint32_t *)malloc((m_raw ? m_raw : 1) * sizeof(uint32_t));
+    double *str = (double *)malloc((m_raw ? m_raw : 1) * sizeof(double));
+    uint32_t m = 0;
+    for (size_t e = 0; e < m_raw; e++) {
+        uint32_t a = src[e], b = dst[e];
+        if (a == b) continue;
+        const uint32_t l = a < b ? a : b, h = a < b ? b : a;
+        int dup = 0;
+        for (uint32_t f = 0; f < m && !dup; f++) dup = lo[f] == l && hi[f] == h;
+        if (dup) continue;
+        lo[m] = l;
+        hi[m] = h;
+        str[m] = strength ? strength[e] : 1.0;
+        m++;
+    }
+    const size_t nn = n ? n : 1;
+    uint32_t *deg = (uint32_t *)calloc(nn, sizeof(uint32_t));
+    for (uint32_t e = 0; e < m; e++) {
+        deg[lo[e]]++;
+        deg[hi[e]]++;
+    }
+    /* rows: each node's simple edges in ascending edge index */
+    uint32_t *roff = (uint32_t *)calloc(nn + 1, sizeof(uint32_t));
+    for (uint32_t v = 0; v < n; v++) roff[v + 1] = roff[v] + deg[v];
+    uint32_t *rows = (uint32_t *)malloc((size_t)(m ? 2 * m : 1) * sizeof(uint32_t));
+    uint32_t *fill = (uint32_t *)malloc(nn * sizeof(uint32_t));
+    for (uint32_t v = 0; v < n; v++) fill[v] = roff[v];
+    for (uint32_t e = 0; e < m; e++) {
+        rows[fill[lo[e]]++] = e;
+        rows[fill[hi[e]]++] = e;
+    }
+    /* link.rs geometry */
+    double *ld = (double *)malloc((m ? m : 1) * sizeof(double));
+    double *ls = (double *)malloc((m ? m : 1) * sizeof(double));
+    double *lb = (double *)malloc((m ? m : 1) * sizeof(double));
+    for (uint32_t e = 0; e < m; e++) {
+        const double sv = str[e], dlo = (double)deg[lo[e]], dhi = (double)deg[hi[e]];
+        ld[e] = ldist / fmax(0.4, sv);
+        ls[e] = fmin(0.7, lscale * sv);
+        lb[e] = dlo / (dlo + dhi);
+    }
+    /* seed.rs: node i at 12 sqrt(i + 1) along the golden-angle spiral */
+    double *x = (double *)malloc(nn * sizeof(double)), *y = (double *)malloc(nn * sizeof(double));
+    double *vx = (double *)calloc(nn, sizeof(double)), *vy = (double *)calloc(nn, sizeof(double));
+    double *px = (double *)malloc(nn * sizeof(double)), *py = (double *)malloc(nn * sizeof(double));
+    double *d = (double *)malloc(nn * 2 * sizeof(double));
+    double *lf = (double *)malloc((size_t)(m ? m : 1) * 2 * sizeof(double));
+    for (uint32_t i = 0; i < n; i++) {
+        const double r = 12.0 * sqrt((double)i + 1.0);
+        const double a = (double)i * 2.399963229728653;
+        x[i] = cos(a) * r;
+        y[i] = sin(a) * r;
+    }
+    const double reach2 = (2.0 * radius) * (2.0 * radius);
+    for (uint32_t tick = 0; tick < ticks; tick++) {
+        alpha += (0.0 - alpha) * adecay;
+        /* link (link.rs force + step.rs LinkPass): per edge, then each node's share */
+        for (uint32_t e = 0; e < m; e++) {
+            const uint32_t a = lo[e], b = hi[e];
+            double dx = (x[b] + vx[b]) - (x[a] + vx[a]);
+            double dy = (y[b] + vy[b]) - (y[a] + vy[a]);
+            if (dx == 0.0) dx = gro_jiggle(seed, tick, 0, a, b);
+            if (dy == 0.0) dy = gro_jiggle(seed, tick, 1, a, b);
+            const double l = sqrt(dx * dx + dy * dy);
+            if (l == 0.0) {
+                lf[2 * e] = 0.0;
+                lf[2 * e + 1] = 0.0;
+                continue;
+            }
+            const double factor = (l - ld[e]) / l * alpha * ls[e];
+            lf[2 * e] = dx * factor;
+            lf[2 * e + 1] = dy * factor;
+        }
+        for (uint32_t v = 0; v < n; v++) {
+            double sx = 0.0, sy = 0.0;
+            for (uint32_t q = roff[v]; q < roff[v + 1]; q++) {
+                const uint32_t e = rows[q];
+                const double b = lb[e];
+                if (hi[e] == v) {
+                    sx += -lf[2 * e] * b;
+                    sy += -lf[2 * e + 1] * b;
+                } else {
+                    sx += lf[2 * e] * (1.0 - b);
+                    sy += lf[2 * e + 1] * (1.0 - b);
+                }
+            }
+            d[2 * v] = sx;
+            d[2 * v + 1] = sy;
+        }
+        d3_merge(n, vx, vy, d);
+        /* charge */
+        d3_charge_core(n, x, y, theta, charge, alpha, dmin, dmax, seed, tick, d);
+        d3_merge(n, vx, vy, d);
+        /* center (sim.rs center_shift): the mean in node order, times the strength */
+        if (n > 0) {
+            double sx = 0.0, sy = 0.0;
+            for (uint32_t i = 0; i < n; i++) {
+                sx = sx + x[i];
+                sy = sy + y[i];
+            }
+            const double shx = (sx / (double)n) * cstrength, shy = (sy / (double)n) * cstrength;
+            for (uint32_t i = 0; i < n; i++) {
+                x[i] -= shx;
+                y[i] -= shy;
+            }
+        }
+        /* collide over the projected positions */
+        for (uint32_t i = 0; i < n; i++) {
+            px[i] = x[i] + vx[i];
+            py[i] = y[i] + vy[i];
+        }
+        gro_tree *ct = gro_tree_build(n, 2, px, py, NULL);
+        for (uint32_t i = 0; i < n; i++)
+            d3_collide_delta(ct, i, px, py, reach2, seed, tick, &d[2 * i], &d[2 * i + 1]);
+        gro_tree_free(ct);
+        d3_merge(n, vx, vy, d);
+        /* integrate (sim.rs): velocity decays, position moves */
+        for (uint32_t i = 0; i < n; i++) {
+            vx[i] *= decay;
+            x[i] += vx[i];
+            vy[i] *= decay;
+            y[i] += vy[i];
+        }
+    }
+    memcpy(x_out, x, (size_t)n * sizeof(double));
+    memcpy(y_out, y, (size_t)n * sizeof(double));
+    free(lo);
+    free(hi);
+    free(str);
+    free(deg);
+    free(roff);
+    free(rows);
+    free(fill);
+    free(ld);
+    free(ls);
+    free(lb);
+    free(x);
+    free(y);
+    free(vx);
+    free(vy);
+    free(px);
+    free(py);
+    free(d);
+    free(lf);
 }
 
 /* barnes_hut.rs:62-110: reverse preorder; a leaf sums its chain, an internal cell its

@@ -41,6 +41,8 @@ double gro_jiggle(uint32_t seed, uint32_t tick, uint32_t pass, uint32_t i, uint3
  * Writes the raw edge list of seeded_model(*, count, *) in record order; returns its
  * length. `cap` must be >= 2*count + count/20 + 1. */
 size_t gro_gate_edges(uint32_t count, uint32_t *source, uint32_t *target, size_t cap);
+size_t gro_gate_model(uint32_t count, uint32_t *source, uint32_t *target, double *strength,
+                      size_t cap);
 
 /* ---- FA2 state (layout/forceatlas2/state.rs, state/schedule.rs, state/barnes_hut.rs) ---- */
 typedef struct gro_fa2 gro_fa2;
@@ -85,6 +87,12 @@ void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, co
 /* ---- d3's many-body force over the quadtree (layout/force/barnes_hut/charge.rs) ----
  * parity: theta 0 equals the direct d3 pair sum (closed form); node ids are the insertion's
  * own shape slots */
+/* ---- the frozen layout.force.barnes_hut (barnes_hut.rs, sim.rs tick) ----
+ * p: charge, theta, distance_min, distance_max, link_distance, link_strength_scale,
+ * collide_radius, center_strength, velocity_decay, alpha_decay, initial_alpha */
+void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
+                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
+                   double *y_out);
 void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge, double alpha,
                       double dmin, double dmax, uint32_t seed, uint32_t tick, double *out);
 

@@ -73,6 +73,10 @@ def lib():
         L.gro_tree_order.argtypes = [C.c_void_p, u32p]
         L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
         L.gro_tree_node_ids.argtypes = [C.c_void_p, u32p]
+        L.gro_gate_model.restype = C.c_size_t
+        L.gro_gate_model.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_size_t]
+        L.gro_d3sim_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                    C.c_uint32, f64p, C.c_uint32, f64p, f64p]
         u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
         f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
         L.gro_pcg64_state.argtypes = [C.c_uint64, u64p]
@@ -377,3 +381,40 @@ def forcesim(n: int, source, target, iterations: int = 50, scale: float = 5.0,
                                 t.ctypes.data if t.size else None, s.size, iterations, scale,
                                 scaling_ratio, gravity, jitter_tolerance, seed, out, cst)
     return out[: 3 * n].reshape(n, 3), st, tuple(cst)
+
+
+# ---- layout.force.barnes_hut, the frozen d3 simulation ----
+
+D3_DEFAULTS = dict(charge=-90.0, theta=0.9, distance_min=1.0, distance_max=520.0,
+                   link_distance=60.0, link_strength_scale=0.15, collide_radius=16.0,
+                   center_strength=1.0, velocity_decay=1.0 - 0.42, alpha_decay=0.06,
+                   initial_alpha=1.0)
+D3_TICKS = 112
+
+
+def gate_model(count: int):
+    """synthetic.rs's model: (source, target, strength) of its raw edges."""
+    cap = 2 * count + 8
+    s = np.zeros(cap, np.uint32)
+    t = np.zeros(cap, np.uint32)
+    w = np.zeros(cap)
+    m = lib().gro_gate_model(count, s, t, w, cap)
+    return s[:m], t[:m], w[:m]
+
+
+def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0, **params):
+    """The frozen layout.force.barnes_hut's f64 positions after `ticks` ticks: (x, y)."""
+    p = dict(D3_DEFAULTS)
+    p.update(params)
+    vec = np.array([p[k] for k in ("charge", "theta", "distance_min", "distance_max",
+                                   "link_distance", "link_strength_scale", "collide_radius",
+                                   "center_strength", "velocity_decay", "alpha_decay",
+                                   "initial_alpha")], np.float64)
+    s = np.ascontiguousarray(source, np.uint32)
+    t = np.ascontiguousarray(target, np.uint32)
+    w = np.ascontiguousarray(strength, np.float64)
+    x = np.zeros(max(n, 1))
+    y = np.zeros(max(n, 1))
+    lib().gro_d3sim_run(n, s.ctypes.data if s.size else None, t.ctypes.data if t.size else None,
+                        w.ctypes.data if w.size else None, s.size, ticks, vec, seed, x, y)
+    return x[:n], y[:n]

@@ -101,6 +101,30 @@ pub struct gr_forcesim_params {
     pub seed: u64,
 }
 
+/// `ForceParams` (layout/force/params.rs:26-77), field for field.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, PartialEq)]
+pub struct gr_d3_params {
+    pub charge_strength: f64,
+    pub theta: f64,
+    pub distance_min: f64,
+    pub distance_max: f64,
+    pub link_distance: f64,
+    pub link_strength_scale: f64,
+    pub collide_radius: f64,
+    pub center_strength: f64,
+    pub velocity_decay: f64,
+    pub alpha_decay: f64,
+    pub alpha_min: f64,
+    pub initial_alpha: f64,
+}
+
+/// An opaque resident d3 simulation (`struct gr_d3_session`).
+#[repr(C)]
+pub struct gr_d3_session {
+    _private: [u8; 0],
+}
+
 /// An opaque device session (`struct gr_fa2_session`).
 #[repr(C)]
 pub struct gr_fa2_session {
@@ -174,6 +198,23 @@ extern "C" {
                                 reps: u32, out: *mut f64, ms_out: *mut f64) -> c_int;
     pub fn gr_quadtree_node_ids(n: u32, xy: *const f64, cap: u32, ncells_out: *mut u32,
                                 node_id: *mut u32) -> c_int;
+    pub fn gr_d3_collide(n: u32, xy: *const f64, radius: f64, seed: u32, tick: u32,
+                         out: *mut f64) -> c_int;
+    pub fn gr_d3_params_default(out: *mut gr_d3_params);
+    pub fn gr_d3_session_create(n: u32, source: *const u32, target: *const u32,
+                                strength: *const f64, m: usize, params: *const gr_d3_params)
+                                -> *mut gr_d3_session;
+    pub fn gr_d3_session_step(s: *mut gr_d3_session, ticks: u32, alpha_out: *mut f64,
+                              settled_out: *mut i32) -> c_int;
+    pub fn gr_d3_session_state(s: *mut gr_d3_session, x: *mut f64, y: *mut f64, vx: *mut f64,
+                               vy: *mut f64) -> c_int;
+    pub fn gr_d3_session_alpha(s: *const gr_d3_session) -> f64;
+    pub fn gr_d3_session_reheat(s: *mut gr_d3_session, alpha: f64) -> c_int;
+    pub fn gr_d3_session_set_alpha_target(s: *mut gr_d3_session, target: f64) -> c_int;
+    pub fn gr_d3_session_release(s: *mut gr_d3_session);
+    pub fn gr_d3_layout_run(n: u32, source: *const u32, target: *const u32, strength: *const f64,
+                            m: usize, params: *const gr_d3_params, ticks: u32, out_x: *mut f32,
+                            out_y: *mut f32, ms_ticks_out: *mut f64) -> c_int;
     pub fn gr_forcesim_params_default(out: *mut gr_forcesim_params);
     pub fn gr_forcesim_run(n: u32, source: *const u32, target: *const u32, m: usize,
                            params: *const gr_forcesim_params, out_x: *mut f32, out_y: *mut f32,

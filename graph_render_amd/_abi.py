@@ -43,6 +43,14 @@ class GrForceSimParams(C.Structure):
                 ("jitter_tolerance", C.c_double), ("seed", C.c_uint64)]
 
 
+class GrD3Params(C.Structure):
+    """gr_d3_params: ForceParams (layout/force/params.rs:26-77)."""
+    _fields_ = [(f, C.c_double) for f in (
+        "charge_strength", "theta", "distance_min", "distance_max", "link_distance",
+        "link_strength_scale", "collide_radius", "center_strength", "velocity_decay",
+        "alpha_decay", "alpha_min", "initial_alpha")]
+
+
 class GrFa2Schedule(C.Structure):
     _fields_ = [("walk_block", C.c_uint32), ("walk_xcd", C.c_uint32),
                 ("sort_items", C.c_uint32), ("finish_split", C.c_uint32)]
@@ -105,6 +113,22 @@ SIGNATURES = {
                                        C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "gr_quadtree_node_ids": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32,
                                        C.POINTER(C.c_uint32), C.c_void_p]),
+    "gr_d3_collide": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_uint32, C.c_uint32,
+                                C.c_void_p]),
+    "gr_d3_params_default": (None, [C.c_void_p]),
+    "gr_d3_session_create": (C.c_void_p, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_size_t, C.c_void_p]),
+    "gr_d3_session_step": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int32)]),
+    "gr_d3_session_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "gr_d3_session_alpha": (C.c_double, [C.c_void_p]),
+    "gr_d3_session_reheat": (C.c_int, [C.c_void_p, C.c_double]),
+    "gr_d3_session_set_alpha_target": (C.c_int, [C.c_void_p, C.c_double]),
+    "gr_d3_session_release": (None, [C.c_void_p]),
+    "gr_d3_layout_run": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                   C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_double)]),
     "gr_forcesim_params_default": (None, [C.c_void_p]),
     "gr_forcesim_run": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -55,7 +55,47 @@ const uint32_t *d3_launch_depth(const D3Dev &d);
 void d3_launch_level(const D3Dev &d, const uint32_t *bucket, uint32_t off, uint32_t cnt);
 // Node ids: each cell's rank by (the point whose insertion pushed it, preorder index).
 void d3_launch_ids(const D3Dev &d);
-// The walk: every node's delta into d.out (NaN for a non-finite node).
+// The walk: every node's delta into d.out.
 void d3_launch_walk(const D3Dev &d);
+
+// ---- the rest of the Barnes-Hut force simulation's tick (barnes_hut/sim.rs) ----
+
+struct D3SimDev {
+    uint32_t n = 0, m = 0;  // nodes, simple edges
+    double *x = nullptr, *y = nullptr;    // positions (the charge tree session's columns)
+    double *vx = nullptr, *vy = nullptr;  // velocities
+    double *px = nullptr, *py = nullptr;  // projected positions (the collide tree's columns)
+    const uint32_t *lo = nullptr, *hi = nullptr;      // simple edges, lo < hi
+    const uint32_t *roff = nullptr, *rows = nullptr;  // node -> its edges, ascending
+    const double *ld = nullptr, *ls = nullptr, *lb = nullptr;  // link distance, strength, bias
+    double *lf = nullptr;  // 2m: each edge's force this tick
+    double *dl = nullptr;  // 2n: deltas of the collide pass
+    uint32_t seed = 0;
+    double decay = 0.58, center_strength = 1.0, reach2 = 1024.0;
+    hipStream_t stream = nullptr;
+};
+
+// The collide tree as the walk reads it (an FP64 2D arena with exported bounds).
+struct D3CollideTree {
+    uint32_t C = 0, n = 0;
+    const void *wc = nullptr;
+    const double *cb = nullptr;
+    const uint32_t *order = nullptr;
+    const double *sx = nullptr, *sy = nullptr;  // projected positions in tree order
+};
+
+// link.rs: every edge's force from the displaced positions, then every node's share added
+// to its velocity (as share + 0).
+void d3sim_link(const D3SimDev &s, double alpha, uint32_t tick);
+// step.rs merge: deltas (2n, node order) added to the velocities as d + 0.
+void d3sim_merge(const D3SimDev &s, const double *deltas);
+// sim.rs center: the mean position (summed in node order) times the strength, subtracted.
+void d3sim_center(const D3SimDev &s);
+// collide.rs prepare: px = x + vx, py = y + vy.
+void d3sim_project(const D3SimDev &s);
+// collide.rs: every node's push from the projected positions within the reach, into s.dl.
+void d3sim_collide(const D3SimDev &s, const D3CollideTree &t, uint32_t tick);
+// sim.rs integrate: v *= decay; position += v.
+void d3sim_integrate(const D3SimDev &s);
 
 }  // namespace gr

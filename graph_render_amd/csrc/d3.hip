@@ -333,4 +333,206 @@ void d3_launch_walk(const D3Dev &d) {
                        (const D3Body *)d.body, d.order, d.sx, d.sy, d.out);
 }
 
+// ---------------------------------------------------------------- the rest of the tick
+
+// link.rs force: the spring of one edge on the displaced positions (x + vx), an exactly
+// zero axis jiggled (passes 0 and 1, keyed on the edge's ends), nothing at zero length.
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_link_force(const D3SimDev s, double alpha,
+                                                             uint32_t tick) {
+    const uint32_t e = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (e >= s.m) return;
+    const uint32_t a = s.lo[e], b = s.hi[e];
+    double dx = (s.x[b] + s.vx[b]) - (s.x[a] + s.vx[a]);
+    double dy = (s.y[b] + s.vy[b]) - (s.y[a] + s.vy[a]);
+    if (dx == 0.0) dx = d_jiggle_inline(s.seed, tick, 0, a, b);
+    if (dy == 0.0) dy = d_jiggle_inline(s.seed, tick, 1, a, b);
+    const double l = sqrt(dx * dx + dy * dy);
+    double fx = 0.0, fy = 0.0;
+    if (l != 0.0) {
+        const double factor = (l - s.ld[e]) / l * alpha * s.ls[e];
+        fx = dx * factor;
+        fy = dy * factor;
+    }
+    s.lf[2 * (size_t)e] = fx;
+    s.lf[2 * (size_t)e + 1] = fy;
+}
+
+// step.rs LinkPass: the node's shares over its edges in ascending order (the higher end
+// takes -f * bias, the lower f * (1 - bias)), then the merge.
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_link_gather(const D3SimDev s) {
+    const uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (v >= s.n) return;
+    double sx = 0.0, sy = 0.0;
+    for (uint32_t q = s.roff[v]; q < s.roff[v + 1]; q++) {
+        const uint32_t e = s.rows[q];
+        const double b = s.lb[e];
+        const double fx = s.lf[2 * (size_t)e], fy = s.lf[2 * (size_t)e + 1];
+        if (s.hi[e] == v) {
+            sx += -fx * b;
+            sy += -fy * b;
+        } else {
+            sx += fx * (1.0 - b);
+            sy += fy * (1.0 - b);
+        }
+    }
+    s.vx[v] += sx + 0.0;
+    s.vy[v] += sy + 0.0;
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_merge(const D3SimDev s, const double *dl) {
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= s.n) return;
+    s.vx[i] += dl[2 * (size_t)i] + 0.0;
+    s.vy[i] += dl[2 * (size_t)i + 1] + 0.0;
+}
+
+// sim.rs center_shift: the two column sums in node order (one lane of one wave each, the
+// other waves staging the next tile in LDS), (sum / n) * strength, then every position moves.
+constexpr uint32_t SIM_TILE = 2048;
+
+__global__ __launch_bounds__(1024) void k_sim_center(const D3SimDev s) {
+    __shared__ double tile[2][2][SIM_TILE];
+    __shared__ double shift[2];
+    const uint32_t n = s.n, tid = threadIdx.x;
+    double acc = 0.0;  // lane 0 of wave 0 sums x, lane 0 of wave 1 sums y
+    const uint32_t ntiles = (n + SIM_TILE - 1) / SIM_TILE;
+    for (uint32_t q = tid; q < SIM_TILE && q < n; q += blockDim.x) {
+        tile[0][0][q] = s.x[q];
+        tile[0][1][q] = s.y[q];
+    }
+    __syncthreads();
+    for (uint32_t t = 0; t < ntiles; t++) {
+        const uint32_t cur = t & 1, nxt = cur ^ 1;
+        const uint32_t base = t * SIM_TILE, len = min(SIM_TILE, n - base);
+        if (tid == 0 || tid == GR_WAVE) {
+            const double *v = tile[cur][tid == 0 ? 0 : 1];
+            for (uint32_t q = 0; q < len; q++) acc = acc + v[q];
+        } else if (tid >= 2 * GR_WAVE && t + 1 < ntiles) {
+            const uint32_t nb = base + SIM_TILE, nl = min(SIM_TILE, n - nb);
+            for (uint32_t q = tid - 2 * GR_WAVE; q < nl; q += blockDim.x - 2 * GR_WAVE) {
+                tile[nxt][0][q] = s.x[nb + q];
+                tile[nxt][1][q] = s.y[nb + q];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) shift[0] = (acc / (double)n) * s.center_strength;
+    if (tid == GR_WAVE) shift[1] = (acc / (double)n) * s.center_strength;
+    __syncthreads();
+    const double hx = shift[0], hy = shift[1];
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+        s.x[i] -= hx;
+        s.y[i] -= hy;
+    }
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_project(const D3SimDev s) {
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= s.n) return;
+    s.px[i] = s.x[i] + s.vx[i];
+    s.py[i] = s.y[i] + s.vy[i];
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_integrate(const D3SimDev s) {
+    const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
+    if (i >= s.n) return;
+    const double vx = s.vx[i] * s.decay, vy = s.vy[i] * s.decay;
+    s.vx[i] = vx;
+    s.x[i] += vx;
+    s.vy[i] = vy;
+    s.y[i] += vy;
+}
+
+// collide.rs node_delta as one wave-uniform preorder scan per 64 consecutive tree
+// positions (as the charge walk): a leaf's points are resolved pair by pair in the leaf's
+// order, an internal cell is skipped by the lanes for which its square lies beyond the
+// reach on some axis. Every node walks, non-finite ones too.
+__global__ __launch_bounds__(D3_WALK) void k_sim_collide(const D3SimDev s, const D3CollideTree t,
+                                                         uint32_t tick) {
+    const uint32_t nb = (t.n + D3_WALK - 1) / D3_WALK;
+    const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
+    if (lb >= nb) return;
+    const uint32_t p = lb * D3_WALK + threadIdx.x;
+    if (__ballot(p < t.n) == 0) return;
+    const bool live = p < t.n;
+    const uint32_t pc = live ? p : t.n - 1;
+    const double xi = t.sx[pc], yi = t.sy[pc];
+    const uint32_t i = t.order[pc];
+    const double d2 = s.reach2, reach = sqrt(d2);
+    const WalkCell<double, 2> *wc = (const WalkCell<double, 2> *)t.wc;
+    uint32_t next = live ? 0u : 0xFFFFFFFFu;
+    double ox = 0.0, oy = 0.0;
+    uint32_t k = gr_uniform(d3_wave_min(next));
+    while (k < t.C) {
+        const uint32_t skip = wc[k].skip, k1 = k + 1;
+        if (next == k) {
+            if (skip == k1) {
+                for (uint32_t q = wc[k].start; q < wc[k].end; q++) {
+                    const uint32_t pj = t.order[q];
+                    if (pj == i) continue;
+                    double dx = xi - t.sx[q], dy = yi - t.sy[q];
+                    double l = dx * dx + dy * dy;
+                    if (l >= d2) continue;
+                    if (dx == 0.0) {
+                        dx = d_jiggle_inline(s.seed, tick, 4, i, pj);
+                        l += dx * dx;
+                    }
+                    if (dy == 0.0) {
+                        dy = d_jiggle_inline(s.seed, tick, 5, i, pj);
+                        l += dy * dy;
+                    }
+                    const double dist = sqrt(l);
+                    const double push = (reach - dist) / dist * 0.5;
+                    ox += dx * push;
+                    oy += dy * push;
+                }
+                next = k1;
+            } else {
+                const double *b = t.cb + (size_t)k * 4;  // x0, y0, x1, y1
+                const bool outside = b[0] > xi + reach || b[2] < xi - reach ||
+                                     b[1] > yi + reach || b[3] < yi - reach;
+                next = outside ? (skip > k ? skip : k1) : k1;
+            }
+        }
+        k = __ballot(next == k1) != 0 ? k1 : gr_uniform(d3_wave_min(next));
+    }
+    if (live) {
+        s.dl[2 * (size_t)i] = ox;
+        s.dl[2 * (size_t)i + 1] = oy;
+    }
+}
+
+void d3sim_link(const D3SimDev &s, double alpha, uint32_t tick) {
+    if (s.m)
+        hipLaunchKernelGGL(k_sim_link_force, dim3(gr_blocks(s.m, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                           s.stream, s, alpha, tick);
+    hipLaunchKernelGGL(k_sim_link_gather, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       s.stream, s);
+}
+
+void d3sim_merge(const D3SimDev &s, const double *deltas) {
+    hipLaunchKernelGGL(k_sim_merge, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0, s.stream,
+                       s, deltas);
+}
+
+void d3sim_center(const D3SimDev &s) {
+    if (s.n) hipLaunchKernelGGL(k_sim_center, dim3(1), dim3(1024), 0, s.stream, s);
+}
+
+void d3sim_project(const D3SimDev &s) {
+    hipLaunchKernelGGL(k_sim_project, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       s.stream, s);
+}
+
+void d3sim_collide(const D3SimDev &s, const D3CollideTree &t, uint32_t tick) {
+    const uint32_t nb = gr_blocks(t.n, D3_WALK);
+    hipLaunchKernelGGL(k_sim_collide, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, s.stream, s, t,
+                       tick);
+}
+
+void d3sim_integrate(const D3SimDev &s) {
+    hipLaunchKernelGGL(k_sim_integrate, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+                       s.stream, s);
+}
+
 }  // namespace gr

@@ -14,6 +14,7 @@
 
 #include "../../include/graph_render_fa2.h"
 #include "d3.h"
+#include "libm_trig.h"
 #include "fa2_kernels.h"
 #include "sort_scan.h"
 
@@ -1528,6 +1529,8 @@ struct D3Pass {
     gr_fa2_session *s;
     gr::D3Dev g;
     std::vector<uint32_t> lvl;  // the level table read back
+    uint32_t cap = 0;           // cells the buffers hold (a tick's tree may be larger)
+    bool have_ids = false;
 
     // The tree of the session's positions; grows the arena when it runs out.
     gr_status tree() {
@@ -1554,19 +1557,23 @@ struct D3Pass {
         return GR_OK;
     }
 
-    // The pass's buffers for the current tree's cell count.
+    // The pass's buffers for the current tree's cell count, kept while later trees fit.
     gr_status alloc(bool ids) {
-        const size_t c = g.C ? g.C : 1;
+        if (g.C <= cap && (have_ids || !ids) && cap > 0) return GR_OK;
+        const size_t c = std::max<size_t>(g.C, (size_t)cap + cap / 4) + 64;
         gr_status r;
         if ((r = s->alloc(&g.body, c * sizeof(gr::D3Body))) ||
             (r = s->alloc(&g.diff, (c + 1) * 4)) || (r = s->alloc(&g.depth, c * 4)) ||
             (r = s->alloc(&g.amin, c * 4)) || (r = s->alloc(&g.bmin, c * 4)) ||
             (r = s->alloc(&g.keys, c * 8)) || (r = s->alloc(&g.keys_alt, c * 8)) ||
             (r = s->alloc(&g.vals, c * 4)) || (r = s->alloc(&g.vals_alt, c * 4)) ||
-            (r = s->alloc(&g.lvl, (c + 3) * 4)) || (r = s->alloc(&g.tmp, gr::d3_tmp_bytes(g.C))) ||
+            (r = s->alloc(&g.lvl, (c + 3) * 4)) ||
+            (r = s->alloc(&g.tmp, gr::d3_tmp_bytes((uint32_t)c))) ||
             (r = s->alloc(&g.out, (size_t)(g.n ? g.n : 1) * 16)) ||
             (ids && (r = s->alloc(&g.nid_out, c * 4))))
             return r;
+        cap = (uint32_t)c;
+        have_ids = have_ids || ids;
         return GR_OK;
     }
 
@@ -1624,7 +1631,7 @@ static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double 
                          double dmin, double dmax, uint32_t seed, uint32_t tick, double *out,
                          uint32_t cap, uint32_t *ncells_out, uint32_t *node_id, uint32_t reps,
                          double *ms_out) {
-    D3Pass p{s, {}, {}};
+    D3Pass p{s, {}, {}, 0, false};
     gr_status r;
     if ((r = p.tree())) return r;
     gr::D3Dev &g = p.g;
@@ -1713,6 +1720,43 @@ gr_status gr_d3_charge_bh_time(uint32_t n, const double *xy, double theta, doubl
     return r ? r : ok();
 }
 
+gr_status gr_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed,
+                        uint32_t tick, double *out) {
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (!std::isfinite(radius) || radius < 0.0)
+        return fail(GR_ERR_PARAM, "radius", "a finite radius >= 0");
+    if (n == 0) return ok();
+    gr_fa2_session *s = d3_session(n, xy);
+    if (!s) return g_err.code;
+    gr_status r = GR_OK;
+    {
+        D3Pass p{s, {}, {}, 0, false};
+        gr::D3SimDev sd;
+        double *dl = nullptr;
+        if (!(r = p.tree()) && !(r = s->alloc(&dl, (size_t)n * 16))) {
+            sd.n = n;
+            sd.dl = dl;
+            sd.seed = seed;
+            sd.reach2 = (2.0 * radius) * (2.0 * radius);
+            sd.stream = s->d.stream;
+            gr::D3CollideTree t;
+            t.C = p.g.C;
+            t.n = n;
+            t.wc = p.g.wc;
+            t.cb = p.g.cb;
+            t.order = p.g.order;
+            t.sx = p.g.sx;
+            t.sy = p.g.sy;
+            gr::d3sim_collide(sd, t, tick);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(sd.stream) != hipSuccess ||
+                hipMemcpy(out, dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess)
+                r = fail(GR_ERR_DEVICE, "hip", "the collide pass failed on the device");
+        }
+    }
+    gr_fa2_session_release(s);
+    return r ? r : ok();
+}
+
 gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint32_t *ncells_out,
                                uint32_t *node_id) {
     if (ncells_out) *ncells_out = 0;
@@ -1724,6 +1768,336 @@ gr_status gr_quadtree_node_ids(uint32_t n, const double *xy, uint32_t cap, uint3
                           0, nullptr);
     gr_fa2_session_release(s);
     return r ? r : ok();
+}
+
+// ---- the frozen layout.force.barnes_hut (barnes_hut.rs run_under, sim.rs tick) ----
+
+// Two FP64 tree sessions over the nodes, sharing the charge session's stream: the charge
+// tree over the positions (its position columns are the simulation's x, y) and the collide
+// tree over the projected positions (its columns are px, py); the rest of the tick is d3.hip.
+struct D3Sim {
+    gr_fa2_session *cs = nullptr, *ks = nullptr;
+    hipStream_t ks_stream = nullptr;  // the collide session's own stream, restored for release
+    D3Pass charge{nullptr, {}, {}, 0, false};
+    D3Pass collide{nullptr, {}, {}, 0, false};  // its tree only
+    gr::D3SimDev sd;
+    ~D3Sim() {
+        if (cs) (void)hipStreamSynchronize(cs->d.stream);
+        if (ks) {
+            if (ks_stream) ks->d.stream = ks_stream;
+            gr_fa2_session_release(ks);
+        }
+        if (cs) gr_fa2_session_release(cs);
+    }
+};
+
+static gr_status d3sim_check(const gr_d3_params &p) {
+    const double v[12] = {p.charge_strength, p.theta, p.distance_min, p.distance_max,
+                          p.link_distance, p.link_strength_scale, p.collide_radius,
+                          p.center_strength, p.velocity_decay, p.alpha_decay, p.alpha_min,
+                          p.initial_alpha};
+    static const char *names[12] = {"charge_strength", "theta", "distance_min", "distance_max",
+                                    "link_distance", "link_strength_scale", "collide_radius",
+                                    "center_strength", "velocity_decay", "alpha_decay",
+                                    "alpha_min", "initial_alpha"};
+    for (int k = 0; k < 12; k++)
+        if (!std::isfinite(v[k])) return fail(GR_ERR_PARAM, names[k], "a finite value");
+    if (p.theta < 0.0 || p.distance_min < 0.0 || p.distance_max < 0.0)
+        return fail(GR_ERR_PARAM, "theta", "theta and distances >= 0");
+    return GR_OK;
+}
+
+static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const uint32_t *dst,
+                             const double *strength, size_t m_raw, const gr_d3_params &p) {
+    // the simple graph (layout/force/mod.rs:101-120): a pair keeps its first raw edge's
+    // strength; each node's row lists its edges in ascending index
+    SimpleGraph g;
+    build_simple(n, src, dst, m_raw, nullptr, g);
+    const uint32_t m = (uint32_t)g.lo.size();
+    std::vector<uint32_t> deg(n, 0), roff((size_t)n + 1, 0), rows((size_t)2 * m);
+    for (uint32_t e = 0; e < m; e++) {
+        deg[g.lo[e]]++;
+        deg[g.hi[e]]++;
+    }
+    for (uint32_t v = 0; v < n; v++) roff[v + 1] = roff[v] + deg[v];
+    std::vector<uint32_t> fillp(roff.begin(), roff.end() - 1);
+    for (uint32_t e = 0; e < m; e++) {
+        rows[fillp[g.lo[e]]++] = e;
+        rows[fillp[g.hi[e]]++] = e;
+    }
+    // link.rs geometry
+    std::vector<double> ld(m), ls(m), lb(m);
+    for (uint32_t e = 0; e < m; e++) {
+        const double sv = strength ? strength[g.raw[e]] : 1.0;
+        const double dlo = (double)deg[g.lo[e]], dhi = (double)deg[g.hi[e]];
+        ld[e] = p.link_distance / std::fmax(0.4, sv);
+        ls[e] = std::fmin(0.7, p.link_strength_scale * sv);
+        lb[e] = dlo / (dlo + dhi);
+    }
+    // seed.rs: node i at 12 sqrt(i + 1) along the golden-angle spiral, libm's cos and sin
+    std::vector<double> xy((size_t)n * 2);
+    for (uint32_t i = 0; i < n; i++) {
+        const double r = 12.0 * std::sqrt((double)i + 1.0);
+        const double a = (double)i * 2.399963229728653;
+        xy[2 * (size_t)i] = gr_libm::cos(a) * r;
+        xy[2 * (size_t)i + 1] = gr_libm::sin(a) * r;
+    }
+    if (!(S.cs = d3_session(n, xy.data()))) return g_err.code;
+    if (!(S.ks = d3_session(n, xy.data()))) return g_err.code;
+    S.ks_stream = S.ks->d.stream;
+    S.ks->d.stream = S.cs->d.stream;
+    S.charge.s = S.cs;
+    S.collide.s = S.ks;
+    gr::D3SimDev &sd = S.sd;
+    gr_fa2_session *c = S.cs;
+    sd.n = n;
+    sd.m = m;
+    sd.x = (double *)c->d.pos[0];
+    sd.y = (double *)c->d.pos[1];
+    sd.px = (double *)S.ks->d.pos[0];
+    sd.py = (double *)S.ks->d.pos[1];
+    sd.stream = c->d.stream;
+    sd.seed = 0;  // session.rs SEED
+    sd.decay = p.velocity_decay;
+    sd.center_strength = p.center_strength;
+    sd.reach2 = (2.0 * p.collide_radius) * (2.0 * p.collide_radius);
+    const size_t nn = n ? n : 1, mm = m ? m : 1;
+    uint32_t *lo = nullptr, *hi = nullptr, *ro = nullptr, *rw = nullptr;
+    double *dld = nullptr, *dls = nullptr, *dlb = nullptr;
+    gr_status r;
+    if ((r = c->alloc(&sd.vx, nn * 8)) || (r = c->alloc(&sd.vy, nn * 8)) ||
+        (r = c->alloc(&sd.dl, nn * 16)) || (r = c->alloc(&sd.lf, mm * 16)) ||
+        (r = c->alloc(&lo, mm * 4)) || (r = c->alloc(&hi, mm * 4)) ||
+        (r = c->alloc(&ro, (nn + 1) * 4)) || (r = c->alloc(&rw, 2 * mm * 4)) ||
+        (r = c->alloc(&dld, mm * 8)) || (r = c->alloc(&dls, mm * 8)) || (r = c->alloc(&dlb, mm * 8)))
+        return r;
+    HIP_TRY(hipMemsetAsync(sd.vx, 0, nn * 8, sd.stream));
+    HIP_TRY(hipMemsetAsync(sd.vy, 0, nn * 8, sd.stream));
+    HIP_TRY(hipStreamSynchronize(sd.stream));
+    if (m) {
+        HIP_TRY(hipMemcpy(lo, g.lo.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(hi, g.hi.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(rw, rows.data(), (size_t)2 * m * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dld, ld.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dls, ls.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dlb, lb.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(ro, roff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+    sd.lo = lo;
+    sd.hi = hi;
+    sd.roff = ro;
+    sd.rows = rw;
+    sd.ld = dld;
+    sd.ls = dls;
+    sd.lb = dlb;
+    gr::D3Dev &cg = S.charge.g;
+    cg.theta = p.theta;
+    cg.theta2 = p.theta * p.theta;
+    cg.dmin2 = p.distance_min * p.distance_min;
+    cg.dmax2 = p.distance_max * p.distance_max;
+    cg.charge = p.charge_strength;
+    cg.seed = 0;
+    return GR_OK;
+}
+
+// One tick (sim.rs tick): alpha decays, then link, many-body, center, collide, integrate.
+static gr_status d3sim_tick(D3Sim &S, double alpha, uint32_t tick) {
+    gr::D3SimDev &sd = S.sd;
+    gr::d3sim_link(sd, alpha, tick);
+    gr_status r;
+    // many-body over the quadtree of the positions (bounds of the moved positions first)
+    gr::launch_bounds(S.cs->d);
+    if ((r = S.charge.tree()) || (r = S.charge.alloc(false))) return r;
+    S.charge.g.alpha = alpha;
+    S.charge.g.tick = tick;
+    if ((r = S.charge.bodies())) return r;
+    gr::d3_launch_walk(S.charge.g);
+    gr::d3sim_merge(sd, S.charge.g.out);
+    gr::d3sim_center(sd);
+    // collide over the quadtree of the projected positions
+    gr::d3sim_project(sd);
+    gr::launch_bounds(S.ks->d);
+    if ((r = S.collide.tree())) return r;
+    gr::D3CollideTree t;
+    t.C = S.collide.g.C;
+    t.n = sd.n;
+    t.wc = S.collide.g.wc;
+    t.cb = S.collide.g.cb;
+    t.order = S.collide.g.order;
+    t.sx = S.collide.g.sx;
+    t.sy = S.collide.g.sy;
+    gr::d3sim_collide(sd, t, tick);
+    gr::d3sim_merge(sd, sd.dl);
+    gr::d3sim_integrate(sd);
+    HIP_TRY(hipGetLastError());
+    return GR_OK;
+}
+
+void gr_d3_params_default(gr_d3_params *out) {
+    if (!out) return;
+    out->charge_strength = -90.0;
+    out->theta = 0.9;
+    out->distance_min = 1.0;
+    out->distance_max = 520.0;
+    out->link_distance = 60.0;
+    out->link_strength_scale = 0.15;
+    out->collide_radius = 16.0;
+    out->center_strength = 1.0;
+    out->velocity_decay = 1.0 - 0.42;
+    out->alpha_decay = 0.06;
+    out->alpha_min = 0.001;
+    out->initial_alpha = 1.0;
+}
+
+// The resident simulation: ForceSession (layout/force/session.rs:118-270) over D3Sim.
+struct gr_d3_session {
+    D3Sim S;
+    gr_d3_params p;
+    double alpha = 1.0, alpha_target = 0.0;
+    uint32_t tick = 0;
+};
+
+static gr_status d3_edges_check(uint32_t n, const uint32_t *source, const uint32_t *target,
+                                size_t m) {
+    if (n >= 0xFFFFFFFFu) return fail(GR_ERR_CAPACITY, "nodes", "node count fits the u32 index space");
+    if (m >= 0xFFFFFFFFull) return fail(GR_ERR_CAPACITY, "edges", "edge count fits the u32 index space");
+    if (m > 0 && (!source || !target)) return fail(GR_ERR_PARAM, "edges", "source and target columns");
+    for (size_t e = 0; e < m; e++)
+        if (source[e] >= n || target[e] >= n)
+            return fail(GR_ERR_PARAM, "edges", "edge endpoints below the node count");
+    return GR_OK;
+}
+
+gr_d3_session *gr_d3_session_create(uint32_t n, const uint32_t *source, const uint32_t *target,
+                                    const double *strength, size_t m,
+                                    const gr_d3_params *params) {
+    if (d3_edges_check(n, source, target, m)) return nullptr;
+    gr_d3_params p;
+    gr_d3_params_default(&p);
+    if (params) p = *params;
+    if (d3sim_check(p)) return nullptr;
+    gr_d3_session *h = new (std::nothrow) gr_d3_session;
+    if (!h) {
+        fail(GR_ERR_DEVICE, "host", "out of host memory");
+        return nullptr;
+    }
+    h->p = p;
+    h->alpha = p.initial_alpha;
+    if (n > 0 && d3sim_setup(h->S, n, source, target, strength, m, p)) {
+        delete h;
+        return nullptr;
+    }
+    ok();
+    return h;
+}
+
+void gr_d3_session_release(gr_d3_session *h) { delete h; }
+
+gr_status gr_d3_session_step(gr_d3_session *h, uint32_t ticks, double *alpha_out,
+                             int32_t *settled_out) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    gr_status r;
+    for (uint32_t t = 0; t < ticks; t++) {
+        // sim.rs tick: alpha moves toward its target (simulation.js:45)
+        h->alpha += (h->alpha_target - h->alpha) * h->p.alpha_decay;
+        if (h->S.sd.n && (r = d3sim_tick(h->S, h->alpha, h->tick))) return r;
+        h->tick++;
+    }
+    if (h->S.sd.n) HIP_TRY(hipStreamSynchronize(h->S.sd.stream));
+    if (alpha_out) *alpha_out = h->alpha;
+    // session.rs report: settled when alpha is below alpha_min and no target holds it up
+    if (settled_out) *settled_out = h->alpha < h->p.alpha_min && h->alpha_target == 0.0;
+    return ok();
+}
+
+gr_status gr_d3_session_state(gr_d3_session *h, double *x, double *y, double *vx, double *vy) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    const size_t bytes = (size_t)h->S.sd.n * 8;
+    if (!bytes) return ok();
+    HIP_TRY(hipStreamSynchronize(h->S.sd.stream));
+    if (x) HIP_TRY(hipMemcpy(x, h->S.sd.x, bytes, hipMemcpyDeviceToHost));
+    if (y) HIP_TRY(hipMemcpy(y, h->S.sd.y, bytes, hipMemcpyDeviceToHost));
+    if (vx) HIP_TRY(hipMemcpy(vx, h->S.sd.vx, bytes, hipMemcpyDeviceToHost));
+    if (vy) HIP_TRY(hipMemcpy(vy, h->S.sd.vy, bytes, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+double gr_d3_session_alpha(const gr_d3_session *h) { return h ? h->alpha : NAN; }
+
+gr_status gr_d3_session_reheat(gr_d3_session *h, double alpha) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    // session.rs reheat: ALPHA's range, finite and 0..=1, taken exactly
+    if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(GR_ERR_PARAM, "alpha", "alpha: finite, 0..=1");
+    h->alpha = alpha;
+    return ok();
+}
+
+gr_status gr_d3_session_set_alpha_target(gr_d3_session *h, double target) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    // session.rs set_alpha_target: finite, 0 <= target < 1
+    if (!(target >= 0.0 && target < 1.0))
+        return fail(GR_ERR_PARAM, "alpha_target", "alpha_target: finite, 0..<1");
+    h->alpha_target = target;
+    return ok();
+}
+
+gr_status gr_d3_layout_run(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out) {
+    if (n >= 0xFFFFFFFFu) return fail(GR_ERR_CAPACITY, "nodes", "node count fits the u32 index space");
+    auto zero = [&]() {
+        if (out_x) std::memset(out_x, 0, (size_t)n * 4);
+        if (out_y) std::memset(out_y, 0, (size_t)n * 4);
+    };
+    if (ms_ticks_out) *ms_ticks_out = 0.0;
+    if (n > 0 && (!out_x || !out_y)) return fail(GR_ERR_PARAM, "out", "two output columns");
+    gr_status r = d3_edges_check(n, source, target, m);
+    gr_d3_params p;
+    gr_d3_params_default(&p);
+    if (params) p = *params;
+    if (!r) r = d3sim_check(p);
+    if (r) {
+        zero();
+        return r;
+    }
+    if (n == 0) return ok();
+    gr_d3_session *h = gr_d3_session_create(n, source, target, strength, m, &p);
+    if (!h) {
+        zero();
+        return g_err.code;
+    }
+    std::vector<double> x(n), y(n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms_ticks_out && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+                         hipEventRecord(e0, h->S.sd.stream) != hipSuccess))
+        r = fail(GR_ERR_DEVICE, "hipEvent", "event creation failed");
+    if (!r) r = gr_d3_session_step(h, ticks, nullptr, nullptr);
+    if (!r && ms_ticks_out) {
+        float ms = 0.f;
+        if (hipEventRecord(e1, h->S.sd.stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            r = fail(GR_ERR_DEVICE, "hipEvent", "event timing failed");
+        *ms_ticks_out = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!r) r = gr_d3_session_state(h, x.data(), y.data(), nullptr, nullptr);
+    gr_d3_session_release(h);
+    if (r) {
+        zero();
+        return r;
+    }
+    // planar_points (layout/force/mod.rs:47-59): refused if any coordinate is not finite
+    for (uint32_t i = 0; i < n; i++)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i])) {
+            zero();
+            return fail(GR_ERR_NON_FINITE, "node.x", "finite coordinates");
+        }
+    for (uint32_t i = 0; i < n; i++) {
+        out_x[i] = (float)x[i];
+        out_y[i] = (float)y[i];
+    }
+    return ok();
 }
 
 gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *source,
@@ -1743,6 +2117,31 @@ gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *sourc
         return fail(GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id");
     }
     std::string id(layout_id);
+    if (id == "layout.force.barnes_hut") {
+        // registered with no published parameters (registry/layouts.rs:92-97); the frozen
+        // ForceParams, 112 ticks; the edges' strength column arrives as ext->edge_weights
+        if (params_len != 0) {
+            zero();
+            return fail(GR_ERR_PARAMS_MALFORMED, "params",
+                        "one little-endian f64 per published parameter, and no other bytes");
+        }
+        if (ext && ext->struct_size != sizeof(gr_fa2_ext)) {
+            zero();
+            return fail(GR_ERR_PARAM, "ext", "struct_size must equal sizeof(gr_fa2_ext)");
+        }
+        // of the extensions, the strength column and the opening angle apply here
+        if (ext && (ext->start || ext->precision != GR_PREC_FP64 ||
+                    (ext->repulsion != GR_REP_DEFAULT && ext->repulsion != GR_REP_TREE))) {
+            zero();
+            return fail(GR_ERR_UNSUPPORTED, "ext",
+                        "layout.force.barnes_hut takes edge_weights and theta only");
+        }
+        gr_d3_params p;
+        gr_d3_params_default(&p);
+        if (ext && ext->theta >= 0.0) p.theta = ext->theta;
+        return gr_d3_layout_run(n, source, target, ext ? ext->edge_weights : nullptr, m, &p,
+                                112, out_x, out_y, nullptr);
+    }
     if (id == "layout.forceatlas2.forcesim") {
         // registered with no published parameters (registry/forceatlas2_forcesim.rs:67-72):
         // the empty buffer is the defaults, any byte is malformed; no gr_fa2_ext applies

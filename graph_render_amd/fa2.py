@@ -282,6 +282,20 @@ LAYOUTS["layout.forceatlas2.forcesim"] = Capability(
     "SciGraphs' own ForceSim (model FA2, direct repulsion)")
 
 
+def _d3_run(topology: Topology, params=None, ext=None) -> Geometry:
+    # the registry form: no published parameters; the strength column as ext.edge_weights
+    if params is not None:
+        raise StageError(A.GR_ERR_UNSUPPORTED, "params", "layout.force.barnes_hut publishes none")
+    return _run_id("layout.force.barnes_hut", topology, b"", ext, 2)
+
+
+# registry/layouts.rs:92-97: no published parameters; ceiling FORCE_CEILING 100 000
+# (registry/force.rs:31, :92-126)
+LAYOUTS["layout.force.barnes_hut"] = Capability(
+    "layout.force.barnes_hut", _d3_run, (), 100_000,
+    "d3-force@3.0.0 src/{manyBody,link,center,collide}.js at TICKS=112")
+
+
 def find(layout_id: str) -> Capability:
     if layout_id not in LAYOUTS:
         raise StageError(A.GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id")

@@ -104,7 +104,10 @@ void gr_fa2_ext_default(gr_fa2_ext *out);
  * schema's range, non-finite, or fractional for an Int is refused, never clamped. */
 gr_status gr_fa2_params_decode(const uint8_t *bytes, size_t len, gr_fa2_params *out);
 
-/* Stage::run for one of the three ids (stage.rs:23-31): dedups the raw edge list as
+/* Stage::run for one of the registered ids (stage.rs:23-31); the FA2 ids below, and
+ * "layout.forceatlas2.forcesim" and "layout.force.barnes_hut", which publish no parameters
+ * (params_len 0; for the latter ext->edge_weights carries the edges' strength column). For
+ * the three FA2 ids: dedups the raw edge list as
  * simple_graph does (layout/force/mod.rs:101-120), runs to max_iter or the 1e-10 early
  * exit, and writes f32 columns (x, y, and z for the 3D id) narrowed as settle() narrows
  * them (layout/forceatlas2.rs:82-91). On failure the output columns are zeroed. */
@@ -303,6 +306,74 @@ gr_status gr_d3_charge_bh_time(uint32_t n, const double *xy, double theta, doubl
                                double alpha, double distance_min, double distance_max,
                                uint32_t seed, uint32_t tick, uint32_t reps, double *out,
                                double *ms_out);
+
+/* d3's collision pass over the quadtree of the given positions (collide.rs:37-127, the
+ * projected positions px = x + vx of a tick): every node's push from the others within
+ * 2 * radius, each leaf's points in chain order, a cell skipped when its square lies beyond
+ * the reach (outside()), an exactly zero axis jiggled (passes 4 and 5 keyed on (node, other)).
+ * out: n*2 f64 deltas in node order (NaN where a node is NaN). FP64, bit for bit. */
+gr_status gr_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed,
+                        uint32_t tick, double *out);
+
+/* ---- layout.force.barnes_hut: d3's force simulation (the reference's Barnes-Hut layout) ---- */
+
+/* ForceParams (layout/force/params.rs:1-77) field for field; defaults: charge -90, theta 0.9,
+ * distance_min 1, distance_max 520, link_distance 60, link_strength_scale 0.15,
+ * collide_radius 16, center_strength 1, velocity_decay 0.58, alpha_decay 0.06,
+ * alpha_min 0.001, initial_alpha 1. */
+typedef struct gr_d3_params {
+    double charge_strength;
+    double theta;
+    double distance_min;
+    double distance_max;
+    double link_distance;
+    double link_strength_scale;
+    double collide_radius;
+    double center_strength;
+    double velocity_decay;
+    double alpha_decay;
+    double alpha_min;
+    double initial_alpha;
+} gr_d3_params;
+
+void gr_d3_params_default(gr_d3_params *out);
+
+/* BarnesHut::run (layout/force/barnes_hut.rs:139-200) on the device: the simple graph (a
+ * pair keeps its first raw edge's strength; strength NULL = 1 per edge), the golden-spiral
+ * start (seed.rs), then `ticks` ticks (the stage runs 112) of the tick of sim.rs: alpha decay,
+ * link springs (link.rs), many-body over the quadtree (as gr_d3_charge_bh), centering,
+ * collision over the quadtree of the projected positions (collide.rs), integration. FP64,
+ * the reference's operations and orders, jiggle seed 0; x, y narrowed to f32
+ * (planar_points, NonFinite{node.x} refused). params NULL = the frozen ForceParams.
+ * ms_ticks_out (may be NULL): the tick loop's time on the device (HIP events). */
+gr_status gr_d3_layout_run(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out);
+
+/* ---- the resident simulation: ForceSession (layout/force/session.rs:118-270) in HBM ---- */
+typedef struct gr_d3_session gr_d3_session;
+
+/* ForceSession::from_frozen (session.rs:132-139): the simple graph with its strengths
+ * (strength NULL = 1 per raw edge), the golden-spiral start, alpha = initial_alpha, target 0;
+ * parameters checked for finiteness (theta and distances also >= 0). NULL on failure. */
+gr_d3_session *gr_d3_session_create(uint32_t n, const uint32_t *source, const uint32_t *target,
+                                    const double *strength, size_t m,
+                                    const gr_d3_params *params);
+/* ForceSession::step (session.rs:165-196): `ticks` ticks; step(112), 112 x step(1) and
+ * 7 x step(16) give the same bytes. alpha_out / settled_out (may be NULL): StepReport's
+ * alpha and settled (alpha < alpha_min with target 0). */
+gr_status gr_d3_session_step(gr_d3_session *s, uint32_t ticks, double *alpha_out,
+                             int32_t *settled_out);
+/* xs(), ys() (session.rs:223-231) and, with no reference counterpart, the velocity columns:
+ * f64, node order; any pointer may be NULL. */
+gr_status gr_d3_session_state(gr_d3_session *s, double *x, double *y, double *vx, double *vy);
+/* alpha() (session.rs:238); NaN for a NULL handle. */
+double gr_d3_session_alpha(const gr_d3_session *s);
+/* reheat (session.rs:247-252): alpha taken exactly, refused outside 0..=1. */
+gr_status gr_d3_session_reheat(gr_d3_session *s, double alpha);
+/* set_alpha_target (session.rs:263-267): refused outside 0 <= target < 1. */
+gr_status gr_d3_session_set_alpha_target(gr_d3_session *s, double target);
+void gr_d3_session_release(gr_d3_session *s);
 
 /* The node id of every cell of the quadtree of xy, in preorder (Quadtree::node_id,
  * quadtree.rs:64-88 via preorder.rs's open): the shape slot the reference's insertion

@@ -675,6 +675,23 @@ static void d3_collide_delta(const gro_tree *t, uint32_t i, const double *px, co
     *oy = outy;
 }
 
+void gro_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed, uint32_t tick,
+                    double *out) {
+    const size_t nn = n ? n : 1;
+    double *px = (double *)calloc(nn, sizeof(double)), *py = (double *)calloc(nn, sizeof(double));
+    for (uint32_t i = 0; i < n; i++) {
+        px[i] = xy[2 * (size_t)i];
+        py[i] = xy[2 * (size_t)i + 1];
+    }
+    gro_tree *t = gro_tree_build(n, 2, px, py, NULL);
+    const double d2 = (2.0 * radius) * (2.0 * radius);
+    for (uint32_t i = 0; i < n; i++)
+        d3_collide_delta(t, i, px, py, d2, seed, tick, &out[2 * (size_t)i], &out[2 * (size_t)i + 1]);
+    gro_tree_free(t);
+    free(px);
+    free(py);
+}
+
 /* step.rs:200-214 (merge): every delta added to its own node's velocity, as d + 0 */
 static void d3_merge(uint32_t n, double *vx, double *vy, const double *d) {
     for (uint32_t i = 0; i < n; i++) {
@@ -691,7 +708,7 @@ static void d3_merge(uint32_t n, double *vx, double *vy, const double *d) {
  * alpha_decay, initial_alpha. x_out, y_out: n each. */
 void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
                    size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
-                   double *y_out) {
+                   double *y_out, double *vx_out, double *vy_out) {
     const double charge = p[0], theta = p[1], dmin = p[2], dmax = p[3], ldist = p[4];
     const double lscale = p[5], radius = p[6], cstrength = p[7], decay = p[8];
     const double adecay = p[9];
@@ -740,7 +757,7 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
         ls[e] = fmin(0.7, lscale * sv);
         lb[e] = dlo / (dlo + dhi);
     }
-    /* seed.rs: node i at 12 sqrt(i + 1) along the golden-angle spiral */
+    /* seed.rs: node i at 12 sqrt(i + 1) along the golden-angle spiral, libm's cos and sin */
     double *x = (double *)malloc(nn * sizeof(double)), *y = (double *)malloc(nn * sizeof(double));
     double *vx = (double *)calloc(nn, sizeof(double)), *vy = (double *)calloc(nn, sizeof(double));
     double *px = (double *)malloc(nn * sizeof(double)), *py = (double *)malloc(nn * sizeof(double));
@@ -749,8 +766,8 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
     for (uint32_t i = 0; i < n; i++) {
         const double r = 12.0 * sqrt((double)i + 1.0);
         const double a = (double)i * 2.399963229728653;
-        x[i] = cos(a) * r;
-        y[i] = sin(a) * r;
+        x[i] = gro_libm_cos(a) * r;
+        y[i] = gro_libm_sin(a) * r;
     }
     const double reach2 = (2.0 * radius) * (2.0 * radius);
     for (uint32_t tick = 0; tick < ticks; tick++) {
@@ -825,6 +842,8 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
     }
     memcpy(x_out, x, (size_t)n * sizeof(double));
     memcpy(y_out, y, (size_t)n * sizeof(double));
+    if (vx_out) memcpy(vx_out, vx, (size_t)n * sizeof(double));
+    if (vy_out) memcpy(vy_out, vy, (size_t)n * sizeof(double));
     free(lo);
     free(hi);
     free(str);

@@ -92,7 +92,13 @@ void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, co
  * collide_radius, center_strength, velocity_decay, alpha_decay, initial_alpha */
 void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
                    size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
-                   double *y_out);
+                   double *y_out, double *vx_out, double *vy_out);
+/* libm 0.2.16's cos and sin (libm_trig.c); NaN past |x| = 2^20 pi/2 */
+double gro_libm_cos(double x);
+double gro_libm_sin(double x);
+/* collide.rs node_delta of every node over the quadtree of (px, py): out n*2, node order */
+void gro_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed, uint32_t tick,
+                    double *out);
 void gro_d3_charge_bh(uint32_t n, const double *xy, double theta, double charge, double alpha,
                       double dmin, double dmax, uint32_t seed, uint32_t tick, double *out);
 

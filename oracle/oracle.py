@@ -76,7 +76,13 @@ def lib():
         L.gro_gate_model.restype = C.c_size_t
         L.gro_gate_model.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_size_t]
         L.gro_d3sim_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
-                                    C.c_uint32, f64p, C.c_uint32, f64p, f64p]
+                                    C.c_uint32, f64p, C.c_uint32, f64p, f64p, C.c_void_p,
+                                    C.c_void_p]
+        L.gro_libm_cos.restype = C.c_double
+        L.gro_libm_cos.argtypes = [C.c_double]
+        L.gro_libm_sin.restype = C.c_double
+        L.gro_libm_sin.argtypes = [C.c_double]
+        L.gro_d3_collide.argtypes = [C.c_uint32, f64p, C.c_double, C.c_uint32, C.c_uint32, f64p]
         u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
         f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
         L.gro_pcg64_state.argtypes = [C.c_uint64, u64p]
@@ -402,8 +408,10 @@ def gate_model(count: int):
     return s[:m], t[:m], w[:m]
 
 
-def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0, **params):
-    """The frozen layout.force.barnes_hut's f64 positions after `ticks` ticks: (x, y)."""
+def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0,
+          velocities: bool = False, **params):
+    """The frozen layout.force.barnes_hut's f64 positions after `ticks` ticks: (x, y), and
+    (x, y, vx, vy) with velocities=True."""
     p = dict(D3_DEFAULTS)
     p.update(params)
     vec = np.array([p[k] for k in ("charge", "theta", "distance_min", "distance_max",
@@ -415,6 +423,37 @@ def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0
     w = np.ascontiguousarray(strength, np.float64)
     x = np.zeros(max(n, 1))
     y = np.zeros(max(n, 1))
+    vx = np.zeros(max(n, 1))
+    vy = np.zeros(max(n, 1))
     lib().gro_d3sim_run(n, s.ctypes.data if s.size else None, t.ctypes.data if t.size else None,
-                        w.ctypes.data if w.size else None, s.size, ticks, vec, seed, x, y)
+                        w.ctypes.data if w.size else None, s.size, ticks, vec, seed, x, y,
+                        vx.ctypes.data, vy.ctypes.data)
+    if velocities:
+        return x[:n], y[:n], vx[:n], vy[:n]
     return x[:n], y[:n]
+
+
+def libm_cos(x: float) -> float:
+    """libm 0.2.16's cos (fdlibm via musl), medium reduction only (NaN past 2^20 pi/2)."""
+    return lib().gro_libm_cos(float(x))
+
+
+def libm_sin(x: float) -> float:
+    return lib().gro_libm_sin(float(x))
+
+
+def golden_spiral(n: int):
+    """seed.rs:1-17: (x, y) of node i at 12 sqrt(i + 1) along the golden angle."""
+    x = np.array([libm_cos(i * 2.399963229728653) * (12.0 * np.sqrt(i + 1.0)) for i in range(n)])
+    y = np.array([libm_sin(i * 2.399963229728653) * (12.0 * np.sqrt(i + 1.0)) for i in range(n)])
+    return x, y
+
+
+def d3_collide(xy, radius: float = 16.0, seed: int = 0, tick: int = 0) -> np.ndarray:
+    """collide.rs node_delta of every node over the quadtree of these positions: (n, 2)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    lib().gro_d3_collide(n, xy if n else np.zeros(2), float(radius), seed & 0xFFFFFFFF,
+                         tick & 0xFFFFFFFF, out)
+    return out[: 2 * n].reshape(n, 2)

@@ -1,9 +1,11 @@
 """Writes tests/golden/pm_d3_mesh1k.npz from the reference's particle-mesh fixtures
 fixtures/gpu/mesh-1k-{start,settled}.gmfx (graph-cli emit-gpu-fixtures; the layout is the
 reference's fixtures/gpu/README.md): for each state, the node positions, the frame the
-reference placed (step, h, origin, cells, reach) and the velocity its charge pass adds from
-rest at alpha 1 (delta_charge). Only inputs and expected outputs are kept; twiddles, the
-spectrum and the link/collide columns are left out. Run here, where the reference is mounted:
+reference placed (step, h, origin, cells, reach) and the velocity each pass adds from rest at
+alpha 1 (delta_charge of the mesh, delta_link, delta_collide of its hashed grid at tick 0);
+plus the collapsed simple edges with their strengths (shared by both states) and the
+forward twiddle table of libm's cos and sin. Only inputs and expected outputs are kept; the
+spectrum is left out. Run here, where the reference is mounted:
     python tests/golden/make_pm_d3_fixture.py /root/reference
 """
 import os
@@ -20,16 +22,27 @@ def load(path):
     h, ox, oy = struct.unpack_from("<ddd", b, 40)
     off = 64
     cells, reach = struct.unpack_from("<II", b, off)
-    off += 8 + 16 * m
-    px = np.frombuffer(b, "<f8", n, off)
-    py = np.frombuffer(b, "<f8", n, off + 8 * n)
-    off += 16 * n + 16 * side + 16 * side * side + 16 * n  # twiddles, spectrum, link deltas
-    cx = np.frombuffer(b, "<f8", n, off)
-    cy = np.frombuffer(b, "<f8", n, off + 8 * n)
-    off += 16 * n + 16 * n  # charge deltas, collide deltas
+    off += 8
+    lo = np.frombuffer(b, "<u4", m, off)
+    hi = np.frombuffer(b, "<u4", m, off + 4 * m)
+    strength = np.frombuffer(b, "<f8", m, off + 8 * m)
+    off += 16 * m
+
+    def pair(count):
+        nonlocal off
+        a = np.frombuffer(b, "<f8", count, off)
+        c = np.frombuffer(b, "<f8", count, off + 8 * count)
+        off += 16 * count
+        return np.stack([a, c], 1)
+
+    xy = pair(n)
+    twiddle = pair(side)
+    off += 16 * side * side  # the spectrum
+    link, charge, collide = pair(n), pair(n), pair(n)
     assert off == len(b)
-    return {"xy": np.stack([px, py], 1), "side": side, "step": step, "h": h, "origin": (ox, oy),
-            "cells": cells, "reach": reach, "delta_charge": np.stack([cx, cy], 1)}
+    return {"xy": xy, "side": side, "step": step, "h": h, "origin": (ox, oy),
+            "cells": cells, "reach": reach, "delta_charge": charge, "delta_link": link,
+            "delta_collide": collide, "twiddle": twiddle, "edges": (lo, hi, strength)}
 
 
 def main(ref_root):
@@ -38,6 +51,10 @@ def main(ref_root):
         f = load(os.path.join(ref_root, "fixtures", "gpu", f"mesh-1k-{state}.gmfx"))
         out[f"{state}_xy"] = f["xy"]
         out[f"{state}_delta_charge"] = f["delta_charge"]
+        out[f"{state}_delta_link"] = f["delta_link"]
+        out[f"{state}_delta_collide"] = f["delta_collide"]
+        out["twiddle"] = f["twiddle"]
+        out["edge_lo"], out["edge_hi"], out["edge_strength"] = f["edges"]
         out[f"{state}_frame"] = np.array([f["step"], f["h"], f["origin"][0], f["origin"][1],
                                           f["cells"], f["reach"], f["side"]], np.float64)
     here = os.path.dirname(os.path.abspath(__file__))

@@ -84,16 +84,62 @@ def test_malformed_buffers_are_refused(data):
 def test_unknown_layout_id():
     e = G.Topology(0, [], [])
     with pytest.raises(G.StageError) as err:
-        G.run_params("layout.force.barnes_hut", e, b"")
+        G.run_params("layout.force.spring", e, b"")
     assert err.value.code == A.GR_ERR_UNKNOWN_LAYOUT
 
 
-def test_registry_lists_the_four_ids():
+def test_registry_lists_the_five_ids():
     assert set(G.LAYOUTS) == {"layout.forceatlas2", "layout.forceatlas2.barnes_hut",
-                              "layout.forceatlas2.3d", "layout.forceatlas2.forcesim"}
+                              "layout.forceatlas2.3d", "layout.forceatlas2.forcesim",
+                              "layout.force.barnes_hut"}
     assert G.find("layout.forceatlas2.barnes_hut").scale_ceiling == 250_000
     assert G.find("layout.forceatlas2.forcesim").scale_ceiling == 2_000
     assert G.find("layout.forceatlas2.forcesim").params == ()
+    # registry/force.rs:31 FORCE_CEILING; registry/layouts.rs:92-97 LayoutParams::NONE
+    assert G.find("layout.force.barnes_hut").scale_ceiling == 100_000
+    assert G.find("layout.force.barnes_hut").params == ()
+
+
+def test_d3_layout_publishes_no_parameters():
+    topo = G.Topology(4, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    with pytest.raises(G.StageError) as e:
+        G.run_params("layout.force.barnes_hut", topo, b"\0" * 8)
+    assert e.value.code == 21
+    # of the extensions only the strength column and the opening angle apply
+    with pytest.raises(G.StageError) as e:
+        G.run_params("layout.force.barnes_hut", topo, b"",
+                     G.Fa2Ext(start=np.zeros((4, 2))))
+    assert e.value.code == A.GR_ERR_UNSUPPORTED
+
+
+def test_d3_defaults_are_the_references():
+    # layout/force/params.rs:59-76
+    from graph_render_amd import barnes_hut as BH
+    p = A.GrD3Params()
+    A.lib().gr_d3_params_default(C.addressof(p))
+    want = (-90.0, 0.9, 1.0, 520.0, 60.0, 0.15, 16.0, 1.0, 1.0 - 0.42, 0.06, 0.001, 1.0)
+    assert tuple(getattr(p, f) for f, _ in A.GrD3Params._fields_) == want
+    d = BH.ForceParams()
+    assert tuple(getattr(d, f) for f, _ in A.GrD3Params._fields_) == want
+    assert BH.TICKS == 112
+
+
+@pytest.mark.parametrize("change", [dict(theta=float("nan")), dict(theta=-1.0),
+                                    dict(alpha_decay=float("inf")),
+                                    dict(distance_max=-1.0)])
+def test_d3_layout_refuses_bad_parameters_before_the_device(change):
+    from graph_render_amd import barnes_hut as BH
+    topo = G.Topology(3, np.array([0], np.uint32), np.array([1], np.uint32))
+    with pytest.raises(G.StageError) as e:
+        BH.BarnesHut.run(topo, params=BH.ForceParams(**change))
+    assert e.value.code == A.GR_ERR_PARAM
+
+
+def test_d3_layout_refuses_a_short_strength_column():
+    from graph_render_amd import barnes_hut as BH
+    topo = G.Topology(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    with pytest.raises(G.ParamError):
+        BH.BarnesHut.run(topo, strength=[1.0])
 
 
 def test_forcesim_publishes_no_parameters():
@@ -221,7 +267,8 @@ def test_rust_structs_follow_the_header_field_order():
     text = open(os.path.join(ROOT, "crates", "graph-render-fa2-sys", "src", "lib.rs")).read()
     for name, fields in (("gr_fa2_params", A.GrFa2Params._fields_),
                          ("gr_fa2_ext", A.GrFa2Ext._fields_),
-                         ("gr_forcesim_params", A.GrForceSimParams._fields_)):
+                         ("gr_forcesim_params", A.GrForceSimParams._fields_),
+                         ("gr_d3_params", A.GrD3Params._fields_)):
         body = text[text.index(f"pub struct {name} {{"):]
         body = body[:body.index("}")]
         got = re.findall(r"pub (\w+):", body)

@@ -514,6 +514,63 @@ def bench_d3(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_d3sim(args):
+    """layout.force.barnes_hut, the frozen d3 force simulation, on the device: the
+    reference's gate model at 100k nodes (FORCE_CEILING, registry/force.rs:31), 112 ticks of
+    link, many-body over the quadtree, center, collide over the quadtree of the projected
+    positions and integration, FP64 in the reference's orders. Layouts/s with the graph
+    uploaded (the tick loop timed with HIP events); the charge walk's roofline position; the
+    state after a sample of ticks compared bit for bit with the oracle's, whose run of the same
+    ticks on one core gives the CPU rate (scaled to 112 ticks)."""
+    from graph_render_amd import barnes_hut as BH
+    from graph_render_amd import synth
+    import graph_render_amd as G
+    n = args.nodes or 100_000
+    s, t, w = synth.gate_model(n)
+    top = G.Topology(n, s, t)
+    BH.BarnesHut.time_run(top, w, ticks=8)  # warm-up: code objects, allocator
+    runs = []
+    for _ in range(max(args.steps, 1)):
+        geo, ms = BH.BarnesHut.time_run(top, w, ticks=BH.TICKS)
+        runs.append(ms)
+        print(f"d3sim n={n}: 112 ticks {ms:.2f} ms", file=sys.stderr, flush=True)
+    ms = float(np.median(runs))
+    line = {
+        "metric": "layout.force.barnes_hut layouts/s @100k nodes (112 ticks), FP64",
+        "value": round(1e3 / ms, 4), "unit": "layouts/s", "n_gpus": 1,
+        "steps": len(runs), "warmup": 1, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"the gate model (synthetic.rs seeded_model, Mulberry32 "
+                               f"0x051042) at {n} nodes, {len(s)} raw edges, golden-spiral "
+                               "start, ForceParams defaults, 112 ticks", "nodes": n,
+                   "raw_edges": int(len(s)), "ticks": BH.TICKS},
+        "ms_per_tick": round(ms / BH.TICKS, 4),
+        "finite": bool(np.isfinite(geo.x).all() and np.isfinite(geo.y).all()),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        k = 12  # about 6 s of oracle work at 100k nodes
+        with BH.ForceSession(top, w) as S:
+            S.step(k)
+            dev = S.state()
+        info = cpu_info()
+        core = pin_one_core()
+        t0 = time.perf_counter()
+        want = O.d3sim(n, s, t, w, ticks=k, velocities=True)
+        el = time.perf_counter() - t0
+        line["bit_exact_vs_oracle"] = all(
+            bool(np.array_equal(a.view(np.uint64), b.view(np.uint64))) for a, b in zip(dev, want))
+        line["cpu_baseline"] = {
+            "value": round(1.0 / (el * BH.TICKS / k), 5), "unit": "layouts/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the first {k} ticks of the same layout through oracle/fa2_oracle.c "
+                      f"(gro_d3sim_run, {el:.1f}s, one thread pinned to core {core}), scaled "
+                      "to 112 ticks; the device state after those ticks compared bit for bit",
+            **info}
+    print(json.dumps(line), flush=True)
+
+
 def bench_forcesim(args):
     """layout.forceatlas2.forcesim over a batch of small graphs, one launch (a workgroup per
     graph): graphs with n uniform in [10, 400] (the reference's DIRECT_MAX) and random edges
@@ -588,7 +645,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
-                                           "autotune", "d3", "forcesim"],
+                                           "autotune", "d3", "forcesim", "d3sim"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -610,6 +667,9 @@ def main():
         return
     if args.workload == "d3":
         bench_d3(args)
+        return
+    if args.workload == "d3sim":
+        bench_d3sim(args)
         return
     if args.workload == "forcesim":
         bench_forcesim(args)

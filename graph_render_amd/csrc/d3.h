@@ -65,11 +65,13 @@ struct D3SimDev {
     double *x = nullptr, *y = nullptr;    // positions (the charge tree session's columns)
     double *vx = nullptr, *vy = nullptr;  // velocities
     double *px = nullptr, *py = nullptr;  // projected positions (the collide tree's columns)
-    const uint32_t *lo = nullptr, *hi = nullptr;      // simple edges, lo < hi
-    const uint32_t *roff = nullptr, *rows = nullptr;  // node -> its edges, ascending
+    const uint32_t *lo = nullptr, *hi = nullptr;  // simple edges, lo < hi
+    const uint32_t *roff = nullptr;  // n + 1: each node's row of slots (its edges, ascending)
+    const uint32_t *slot = nullptr;  // 2m: edge e's slot in lo's row, then in hi's
     const double *ld = nullptr, *ls = nullptr, *lb = nullptr;  // link distance, strength, bias
-    double *lf = nullptr;  // 2m: each edge's force this tick
-    double *dl = nullptr;  // 2n: deltas of the collide pass
+    double *sh = nullptr;     // 2m slots x (x, y): each slot's share of its edge's force
+    double *dl = nullptr;     // 2n: deltas of the collide pass
+    double *shift = nullptr;  // 2: the centring shift of this tick
     uint32_t seed = 0;
     double decay = 0.58, center_strength = 1.0, reach2 = 1024.0;
     hipStream_t stream = nullptr;
@@ -84,15 +86,17 @@ struct D3CollideTree {
     const double *sx = nullptr, *sy = nullptr;  // projected positions in tree order
 };
 
-// link.rs: every edge's force from the displaced positions, then every node's share added
-// to its velocity (as share + 0).
+// link.rs: every edge's force from the displaced positions, then every node's shares added
+// to its velocity (as sum + 0).
 void d3sim_link(const D3SimDev &s, double alpha, uint32_t tick);
 // step.rs merge: deltas (2n, node order) added to the velocities as d + 0.
 void d3sim_merge(const D3SimDev &s, const double *deltas);
-// sim.rs center: the mean position (summed in node order) times the strength, subtracted.
-void d3sim_center(const D3SimDev &s);
-// collide.rs prepare: px = x + vx, py = y + vy.
-void d3sim_project(const D3SimDev &s);
+// sim.rs center_shift: the mean position (summed in node order) times the strength, into
+// s.shift; reads only the positions, so it may run on `stream` beside the link and charge
+// passes.
+void d3sim_center_sum(const D3SimDev &s, hipStream_t stream);
+// sim.rs center, then collide.rs prepare: positions -= shift; px = x + vx, py = y + vy.
+void d3sim_center_project(const D3SimDev &s);
 // collide.rs: every node's push from the projected positions within the reach, into s.dl.
 void d3sim_collide(const D3SimDev &s, const D3CollideTree &t, uint32_t tick);
 // sim.rs integrate: v *= decay; position += v.

@@ -335,8 +335,10 @@ void d3_launch_walk(const D3Dev &d) {
 
 // ---------------------------------------------------------------- the rest of the tick
 
-// link.rs force: the spring of one edge on the displaced positions (x + vx), an exactly
-// zero axis jiggled (passes 0 and 1, keyed on the edge's ends), nothing at zero length.
+// link.rs force and share: the spring of one edge on the displaced positions (x + vx), an
+// exactly zero axis jiggled (passes 0 and 1, keyed on the edge's ends), nothing at zero
+// length; its two shares (the lower end f * (1 - bias), the higher -f * bias) go to the
+// edge's slots in its ends' rows.
 __global__ __launch_bounds__(GR_BLOCK) void k_sim_link_force(const D3SimDev s, double alpha,
                                                              uint32_t tick) {
     const uint32_t e = blockIdx.x * GR_BLOCK + threadIdx.x;
@@ -353,27 +355,35 @@ __global__ __launch_bounds__(GR_BLOCK) void k_sim_link_force(const D3SimDev s, d
         fx = dx * factor;
         fy = dy * factor;
     }
-    s.lf[2 * (size_t)e] = fx;
-    s.lf[2 * (size_t)e + 1] = fy;
+    const double bias = s.lb[e];
+    double2 *sh = (double2 *)s.sh;
+    sh[s.slot[2 * (size_t)e]] = make_double2(fx * (1.0 - bias), fy * (1.0 - bias));
+    sh[s.slot[2 * (size_t)e + 1]] = make_double2(-fx * bias, -fy * bias);
 }
 
-// step.rs LinkPass: the node's shares over its edges in ascending order (the higher end
-// takes -f * bias, the lower f * (1 - bias)), then the merge.
+// step.rs LinkPass: each node's shares summed in its row's order (ascending edge), then the
+// merge (share + 0). The row is contiguous, so eight slots are loaded ahead of their sums.
 __global__ __launch_bounds__(GR_BLOCK) void k_sim_link_gather(const D3SimDev s) {
     const uint32_t v = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (v >= s.n) return;
+    const double2 *sh = (const double2 *)s.sh;
+    const uint32_t q1 = s.roff[v + 1];
+    uint32_t q = s.roff[v];
     double sx = 0.0, sy = 0.0;
-    for (uint32_t q = s.roff[v]; q < s.roff[v + 1]; q++) {
-        const uint32_t e = s.rows[q];
-        const double b = s.lb[e];
-        const double fx = s.lf[2 * (size_t)e], fy = s.lf[2 * (size_t)e + 1];
-        if (s.hi[e] == v) {
-            sx += -fx * b;
-            sy += -fy * b;
-        } else {
-            sx += fx * (1.0 - b);
-            sy += fy * (1.0 - b);
+    for (; q + 8 <= q1; q += 8) {
+        double2 r[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) r[j] = sh[q + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            sx += r[j].x;
+            sy += r[j].y;
         }
+    }
+    for (; q < q1; q++) {
+        const double2 r = sh[q];
+        sx += r.x;
+        sy += r.y;
     }
     s.vx[v] += sx + 0.0;
     s.vy[v] += sy + 0.0;
@@ -386,15 +396,32 @@ __global__ __launch_bounds__(GR_BLOCK) void k_sim_merge(const D3SimDev s, const 
     s.vy[i] += dl[2 * (size_t)i + 1] + 0.0;
 }
 
-// sim.rs center_shift: the two column sums in node order (one lane of one wave each, the
-// other waves staging the next tile in LDS), (sum / n) * strength, then every position moves.
+// sim.rs center_shift: the two column sums in node order, (sum / n) * strength, into
+// s.shift. One chain each in lane 0 of waves 0 (x) and 1 (y); the other waves stage the next
+// tile in LDS; each chain reads 32 values of its tile into registers ahead of their adds.
 constexpr uint32_t SIM_TILE = 2048;
 
-__global__ __launch_bounds__(1024) void k_sim_center(const D3SimDev s) {
+__device__ __forceinline__ double chain_sum(double acc, const double *v, uint32_t len) {
+    uint32_t q = 0;
+    for (; q + 32 <= len; q += 32) {
+        double r0[16], r1[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) r0[j] = v[q + j];
+#pragma unroll
+        for (int j = 0; j < 16; j++) r1[j] = v[q + 16 + j];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc = acc + r0[j];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc = acc + r1[j];
+    }
+    for (; q < len; q++) acc = acc + v[q];
+    return acc;
+}
+
+__global__ __launch_bounds__(1024) void k_sim_center_sum(const D3SimDev s) {
     __shared__ double tile[2][2][SIM_TILE];
-    __shared__ double shift[2];
     const uint32_t n = s.n, tid = threadIdx.x;
-    double acc = 0.0;  // lane 0 of wave 0 sums x, lane 0 of wave 1 sums y
+    double acc = 0.0;
     const uint32_t ntiles = (n + SIM_TILE - 1) / SIM_TILE;
     for (uint32_t q = tid; q < SIM_TILE && q < n; q += blockDim.x) {
         tile[0][0][q] = s.x[q];
@@ -405,8 +432,7 @@ __global__ __launch_bounds__(1024) void k_sim_center(const D3SimDev s) {
         const uint32_t cur = t & 1, nxt = cur ^ 1;
         const uint32_t base = t * SIM_TILE, len = min(SIM_TILE, n - base);
         if (tid == 0 || tid == GR_WAVE) {
-            const double *v = tile[cur][tid == 0 ? 0 : 1];
-            for (uint32_t q = 0; q < len; q++) acc = acc + v[q];
+            acc = chain_sum(acc, tile[cur][tid == 0 ? 0 : 1], len);
         } else if (tid >= 2 * GR_WAVE && t + 1 < ntiles) {
             const uint32_t nb = base + SIM_TILE, nl = min(SIM_TILE, n - nb);
             for (uint32_t q = tid - 2 * GR_WAVE; q < nl; q += blockDim.x - 2 * GR_WAVE) {
@@ -416,21 +442,19 @@ __global__ __launch_bounds__(1024) void k_sim_center(const D3SimDev s) {
         }
         __syncthreads();
     }
-    if (tid == 0) shift[0] = (acc / (double)n) * s.center_strength;
-    if (tid == GR_WAVE) shift[1] = (acc / (double)n) * s.center_strength;
-    __syncthreads();
-    const double hx = shift[0], hy = shift[1];
-    for (uint32_t i = tid; i < n; i += blockDim.x) {
-        s.x[i] -= hx;
-        s.y[i] -= hy;
-    }
+    if (tid == 0) s.shift[0] = (acc / (double)n) * s.center_strength;
+    if (tid == GR_WAVE) s.shift[1] = (acc / (double)n) * s.center_strength;
 }
 
-__global__ __launch_bounds__(GR_BLOCK) void k_sim_project(const D3SimDev s) {
+// sim.rs center, then collide.rs prepare: x -= shift; px = x + vx.
+__global__ __launch_bounds__(GR_BLOCK) void k_sim_center_project(const D3SimDev s) {
     const uint32_t i = blockIdx.x * GR_BLOCK + threadIdx.x;
     if (i >= s.n) return;
-    s.px[i] = s.x[i] + s.vx[i];
-    s.py[i] = s.y[i] + s.vy[i];
+    const double x = s.x[i] - s.shift[0], y = s.y[i] - s.shift[1];
+    s.x[i] = x;
+    s.y[i] = y;
+    s.px[i] = x + s.vx[i];
+    s.py[i] = y + s.vy[i];
 }
 
 __global__ __launch_bounds__(GR_BLOCK) void k_sim_integrate(const D3SimDev s) {
@@ -515,12 +539,12 @@ void d3sim_merge(const D3SimDev &s, const double *deltas) {
                        s, deltas);
 }
 
-void d3sim_center(const D3SimDev &s) {
-    if (s.n) hipLaunchKernelGGL(k_sim_center, dim3(1), dim3(1024), 0, s.stream, s);
+void d3sim_center_sum(const D3SimDev &s, hipStream_t stream) {
+    if (s.n) hipLaunchKernelGGL(k_sim_center_sum, dim3(1), dim3(1024), 0, stream, s);
 }
 
-void d3sim_project(const D3SimDev &s) {
-    hipLaunchKernelGGL(k_sim_project, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
+void d3sim_center_project(const D3SimDev &s) {
+    hipLaunchKernelGGL(k_sim_center_project, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0,
                        s.stream, s);
 }
 

@@ -1781,7 +1781,17 @@ struct D3Sim {
     D3Pass charge{nullptr, {}, {}, 0, false};
     D3Pass collide{nullptr, {}, {}, 0, false};  // its tree only
     gr::D3SimDev sd;
+    // the centring sum runs on its own stream beside the link and charge passes: ev_x marks
+    // the positions the last tick left, ev_c the shift it computed
+    hipStream_t side = nullptr;
+    hipEvent_t ev_x = nullptr, ev_c = nullptr;
     ~D3Sim() {
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (ev_x) (void)hipEventDestroy(ev_x);
+        if (ev_c) (void)hipEventDestroy(ev_c);
         if (cs) (void)hipStreamSynchronize(cs->d.stream);
         if (ks) {
             if (ks_stream) ks->d.stream = ks_stream;
@@ -1814,7 +1824,7 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     SimpleGraph g;
     build_simple(n, src, dst, m_raw, nullptr, g);
     const uint32_t m = (uint32_t)g.lo.size();
-    std::vector<uint32_t> deg(n, 0), roff((size_t)n + 1, 0), rows((size_t)2 * m);
+    std::vector<uint32_t> deg(n, 0), roff((size_t)n + 1, 0), slot((size_t)2 * m);
     for (uint32_t e = 0; e < m; e++) {
         deg[g.lo[e]]++;
         deg[g.hi[e]]++;
@@ -1822,8 +1832,8 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     for (uint32_t v = 0; v < n; v++) roff[v + 1] = roff[v] + deg[v];
     std::vector<uint32_t> fillp(roff.begin(), roff.end() - 1);
     for (uint32_t e = 0; e < m; e++) {
-        rows[fillp[g.lo[e]]++] = e;
-        rows[fillp[g.hi[e]]++] = e;
+        slot[2 * (size_t)e] = fillp[g.lo[e]]++;
+        slot[2 * (size_t)e + 1] = fillp[g.hi[e]]++;
     }
     // link.rs geometry
     std::vector<double> ld(m), ls(m), lb(m);
@@ -1862,22 +1872,27 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     sd.center_strength = p.center_strength;
     sd.reach2 = (2.0 * p.collide_radius) * (2.0 * p.collide_radius);
     const size_t nn = n ? n : 1, mm = m ? m : 1;
-    uint32_t *lo = nullptr, *hi = nullptr, *ro = nullptr, *rw = nullptr;
+    uint32_t *lo = nullptr, *hi = nullptr, *ro = nullptr, *sl = nullptr;
     double *dld = nullptr, *dls = nullptr, *dlb = nullptr;
     gr_status r;
     if ((r = c->alloc(&sd.vx, nn * 8)) || (r = c->alloc(&sd.vy, nn * 8)) ||
-        (r = c->alloc(&sd.dl, nn * 16)) || (r = c->alloc(&sd.lf, mm * 16)) ||
-        (r = c->alloc(&lo, mm * 4)) || (r = c->alloc(&hi, mm * 4)) ||
-        (r = c->alloc(&ro, (nn + 1) * 4)) || (r = c->alloc(&rw, 2 * mm * 4)) ||
-        (r = c->alloc(&dld, mm * 8)) || (r = c->alloc(&dls, mm * 8)) || (r = c->alloc(&dlb, mm * 8)))
+        (r = c->alloc(&sd.dl, nn * 16)) || (r = c->alloc(&sd.sh, 2 * mm * 16)) ||
+        (r = c->alloc(&sd.shift, 16)) || (r = c->alloc(&lo, mm * 4)) ||
+        (r = c->alloc(&hi, mm * 4)) || (r = c->alloc(&ro, (nn + 1) * 4)) ||
+        (r = c->alloc(&sl, 2 * mm * 4)) || (r = c->alloc(&dld, mm * 8)) ||
+        (r = c->alloc(&dls, mm * 8)) || (r = c->alloc(&dlb, mm * 8)))
         return r;
+    if (hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&S.ev_x, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S.ev_c, hipEventDisableTiming) != hipSuccess)
+        return fail(GR_ERR_DEVICE, "hipStreamCreate", "stream or event creation failed");
     HIP_TRY(hipMemsetAsync(sd.vx, 0, nn * 8, sd.stream));
     HIP_TRY(hipMemsetAsync(sd.vy, 0, nn * 8, sd.stream));
     HIP_TRY(hipStreamSynchronize(sd.stream));
     if (m) {
         HIP_TRY(hipMemcpy(lo, g.lo.data(), (size_t)m * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(hi, g.hi.data(), (size_t)m * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(rw, rows.data(), (size_t)2 * m * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sl, slot.data(), (size_t)2 * m * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(dld, ld.data(), (size_t)m * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(dls, ls.data(), (size_t)m * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(dlb, lb.data(), (size_t)m * 8, hipMemcpyHostToDevice));
@@ -1886,7 +1901,7 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     sd.lo = lo;
     sd.hi = hi;
     sd.roff = ro;
-    sd.rows = rw;
+    sd.slot = sl;
     sd.ld = dld;
     sd.ls = dls;
     sd.lb = dlb;
@@ -1903,6 +1918,11 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
 // One tick (sim.rs tick): alpha decays, then link, many-body, center, collide, integrate.
 static gr_status d3sim_tick(D3Sim &S, double alpha, uint32_t tick) {
     gr::D3SimDev &sd = S.sd;
+    // the centring sum reads only the positions the last tick left: beside link and charge
+    HIP_TRY(hipEventRecord(S.ev_x, sd.stream));
+    HIP_TRY(hipStreamWaitEvent(S.side, S.ev_x, 0));
+    gr::d3sim_center_sum(sd, S.side);
+    HIP_TRY(hipEventRecord(S.ev_c, S.side));
     gr::d3sim_link(sd, alpha, tick);
     gr_status r;
     // many-body over the quadtree of the positions (bounds of the moved positions first)
@@ -1913,9 +1933,9 @@ static gr_status d3sim_tick(D3Sim &S, double alpha, uint32_t tick) {
     if ((r = S.charge.bodies())) return r;
     gr::d3_launch_walk(S.charge.g);
     gr::d3sim_merge(sd, S.charge.g.out);
-    gr::d3sim_center(sd);
-    // collide over the quadtree of the projected positions
-    gr::d3sim_project(sd);
+    // center, then collide over the quadtree of the projected positions
+    HIP_TRY(hipStreamWaitEvent(sd.stream, S.ev_c, 0));
+    gr::d3sim_center_project(sd);
     gr::launch_bounds(S.ks->d);
     if ((r = S.collide.tree())) return r;
     gr::D3CollideTree t;

@@ -175,6 +175,50 @@ def batch_workload(count: int = 4096, lo: int = 50, hi: int = 20000):
     return sizes, srcs, dsts, np.concatenate(starts)
 
 
+def _mulberry32(state: int):
+    """synthetic.rs:44-50: the next state and draw (integer steps, then / 2^32)."""
+    state = (state + 0x6D2B79F5) & 0xFFFFFFFF
+    t = ((state ^ (state >> 15)) * (1 | state)) & 0xFFFFFFFF
+    t = ((t + (((t ^ (t >> 7)) * (61 | t)) & 0xFFFFFFFF)) & 0xFFFFFFFF) ^ t
+    return state, (t ^ (t >> 14)) / 4294967296.0
+
+
+def gate_model(count: int):
+    """The reference's gate model (synthetic.rs:92-170, seeded_model at REFERENCE_DEGREE):
+    preferential attachment by x*y*i draws plus 5% extra links, from Mulberry32 seeded
+    0x051042; the raw edge columns and each kept edge's strength (0.4 + 0.4 * a draw)."""
+    st = 0x051042
+    for _ in range(count):  # the node draws: a roll, then one pick for 80% of nodes
+        st, roll = _mulberry32(st)
+        if roll < 0.8:
+            st, _ = _mulberry32(st)
+    src, dst, strength = [], [], []
+
+    def push(a, b):
+        nonlocal st
+        if a != b:
+            src.append(a)
+            dst.append(b)
+            st, u = _mulberry32(st)
+            strength.append(0.4 + u * 0.4)
+
+    for i in range(1, count):
+        st, x = _mulberry32(st)
+        st, y = _mulberry32(st)
+        push(i, int(x * y * i))
+        st, u = _mulberry32(st)
+        if u < 0.5:
+            st, x = _mulberry32(st)
+            st, y = _mulberry32(st)
+            push(i, int(x * y * i))
+    for _ in range(int(count * 0.05)):
+        st, u = _mulberry32(st)
+        a = int(u * count)
+        st, u = _mulberry32(st)
+        push(a, int(u * count))
+    return (np.array(src, np.uint32), np.array(dst, np.uint32), np.array(strength, np.float64))
+
+
 # The named workloads of the benchmark configurations. Each returns the graph (raw edge
 # columns in edge-index order), its edge weights (None: unit), the start positions (None:
 # the reference's Mulberry32 starts from `seed`), the opening angle and the parameters.

@@ -719,18 +719,30 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
     uint32_t *hi = (uint32_t *)malloc((m_raw ? m_raw : 1) * sizeof(uint32_t));
     double *str = (double *)malloc((m_raw ? m_raw : 1) * sizeof(double));
     uint32_t m = 0;
+    /* seen pairs in an open-addressed table (linear probing, at most half full) */
+    size_t cap = 16;
+    while (cap < 2 * m_raw + 2) cap <<= 1;
+    uint64_t *seen = (uint64_t *)malloc(cap * sizeof(uint64_t));
+    for (size_t q = 0; q < cap; q++) seen[q] = ~0ull;
     for (size_t e = 0; e < m_raw; e++) {
         uint32_t a = src[e], b = dst[e];
         if (a == b) continue;
         const uint32_t l = a < b ? a : b, h = a < b ? b : a;
+        const uint64_t key = ((uint64_t)l << 32) | h;
+        size_t q = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
         int dup = 0;
-        for (uint32_t f = 0; f < m && !dup; f++) dup = lo[f] == l && hi[f] == h;
+        while (seen[q] != ~0ull && !dup) {
+            dup = seen[q] == key;
+            q = (q + 1) & (cap - 1);
+        }
         if (dup) continue;
+        seen[q] = key;
         lo[m] = l;
         hi[m] = h;
         str[m] = strength ? strength[e] : 1.0;
         m++;
     }
+    free(seen);
     const size_t nn = n ? n : 1;
     uint32_t *deg = (uint32_t *)calloc(nn, sizeof(uint32_t));
     for (uint32_t e = 0; e < m; e++) {

@@ -60,6 +60,19 @@ def walk_traffic(n: int):
     return None
 
 
+def collide_traffic(n: int):
+    """HBM bytes per k_sim_collide launch at the 100k-node gate model, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/d3sim_100k_kernel_stats.md), or None for
+    other sizes."""
+    path = os.path.join(ROOT, "profiles", "d3sim_100k_kernel_stats.md")
+    if n != 100_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_collide_100k"):
+            return int(line.split("=")[1])
+    return None
+
+
 def cpu_info() -> dict:
     """The host CPU as the baseline states it: model, logical cores the process may use."""
     model = "unknown"
@@ -535,6 +548,16 @@ def bench_d3sim(args):
         runs.append(ms)
         print(f"d3sim n={n}: 112 ticks {ms:.2f} ms", file=sys.stderr, flush=True)
     ms = float(np.median(runs))
+    # the collide walk, the longest kernel on the tick's critical path, timed again on the
+    # final tick's tree; algorithmic bytes per launch: per node its tree-order position (16 B),
+    # point index (4 B) and delta (16 B); per cell its skip/start/end (12 B) and square (32 B)
+    with BH.ForceSession(top, w) as S:
+        S.step(BH.TICKS)
+        ms_charge, ms_collide, c_charge, c_collide = S.time_walks(10)
+    wb = 36 * n + 44 * c_collide
+    achieved = wb / (ms_collide * 1e-3) / 1e9
+    print(f"d3sim walks: many-body {ms_charge:.3f} ms ({c_charge} cells), collide "
+          f"{ms_collide:.3f} ms ({c_collide} cells)", file=sys.stderr, flush=True)
     line = {
         "metric": "layout.force.barnes_hut layouts/s @100k nodes (112 ticks), FP64",
         "value": round(1e3 / ms, 4), "unit": "layouts/s", "n_gpus": 1,
@@ -546,6 +569,11 @@ def bench_d3sim(args):
                                "start, ForceParams defaults, 112 ticks", "nodes": n,
                    "raw_edges": int(len(s)), "ticks": BH.TICKS},
         "ms_per_tick": round(ms / BH.TICKS, 4),
+        "walk_ms": {"many_body": round(ms_charge, 4), "collide": round(ms_collide, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": collide_traffic(n), "kernel": "k_sim_collide",
+                     "algorithmic_bytes": wb},
         "finite": bool(np.isfinite(geo.x).all() and np.isfinite(geo.y).all()),
     }
     if not args.no_cpu:

@@ -208,6 +208,8 @@ extern "C" {
                               settled_out: *mut i32) -> c_int;
     pub fn gr_d3_session_state(s: *mut gr_d3_session, x: *mut f64, y: *mut f64, vx: *mut f64,
                                vy: *mut f64) -> c_int;
+    pub fn gr_d3_session_time(s: *mut gr_d3_session, reps: u32, ms_out: *mut f64,
+                              cells_out: *mut u32) -> c_int;
     pub fn gr_d3_session_alpha(s: *const gr_d3_session) -> f64;
     pub fn gr_d3_session_reheat(s: *mut gr_d3_session, alpha: f64) -> c_int;
     pub fn gr_d3_session_set_alpha_target(s: *mut gr_d3_session, target: f64) -> c_int;

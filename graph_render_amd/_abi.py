@@ -122,6 +122,7 @@ SIGNATURES = {
                                      C.POINTER(C.c_int32)]),
     "gr_d3_session_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
+    "gr_d3_session_time": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "gr_d3_session_alpha": (C.c_double, [C.c_void_p]),
     "gr_d3_session_reheat": (C.c_int, [C.c_void_p, C.c_double]),
     "gr_d3_session_set_alpha_target": (C.c_int, [C.c_void_p, C.c_double]),

@@ -104,6 +104,14 @@ class ForceSession:
     def ys(self):
         return self.state()[1]
 
+    def time_walks(self, reps: int = 10):
+        """The last tick's many-body and collide walks timed again on their own trees:
+        (many-body ms, collide ms, many-body cells, collide cells)."""
+        ms = np.zeros(2)
+        cells = np.zeros(2, np.uint32)
+        _check(A.lib().gr_d3_session_time(self._h, int(reps), ms.ctypes.data, cells.ctypes.data))
+        return float(ms[0]), float(ms[1]), int(cells[0]), int(cells[1])
+
     @property
     def alpha(self) -> float:
         return float(A.lib().gr_d3_session_alpha(self._h))

@@ -2042,6 +2042,32 @@ gr_status gr_d3_session_state(gr_d3_session *h, double *x, double *y, double *vx
     return ok();
 }
 
+gr_status gr_d3_session_time(gr_d3_session *h, uint32_t reps, double *ms_out,
+                             uint32_t *cells_out) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    if (!ms_out || !cells_out) return fail(GR_ERR_PARAM, "out", "two times and two counts");
+    D3Sim &S = h->S;
+    if (!h->tick || !S.sd.n) return fail(GR_ERR_PARAM, "session", "a session stepped at least once");
+    if (reps == 0) reps = 1;
+    gr::D3CollideTree t;
+    t.C = S.collide.g.C;
+    t.n = S.sd.n;
+    t.wc = S.collide.g.wc;
+    t.cb = S.collide.g.cb;
+    t.order = S.collide.g.order;
+    t.sx = S.collide.g.sx;
+    t.sy = S.collide.g.sy;
+    gr_status r;
+    // both walks write only their scratch deltas: the state is left as the last tick left it
+    if ((r = time_median(S.cs, reps, [&] { gr::d3_launch_walk(S.charge.g); }, &ms_out[0])) ||
+        (r = time_median(S.cs, reps, [&] { gr::d3sim_collide(S.sd, t, h->tick - 1); },
+                         &ms_out[1])))
+        return r;
+    cells_out[0] = S.charge.g.C;
+    cells_out[1] = S.collide.g.C;
+    return ok();
+}
+
 double gr_d3_session_alpha(const gr_d3_session *h) { return h ? h->alpha : NAN; }
 
 gr_status gr_d3_session_reheat(gr_d3_session *h, double alpha) {

@@ -367,6 +367,12 @@ gr_status gr_d3_session_step(gr_d3_session *s, uint32_t ticks, double *alpha_out
 /* xs(), ys() (session.rs:223-231) and, with no reference counterpart, the velocity columns:
  * f64, node order; any pointer may be NULL. */
 gr_status gr_d3_session_state(gr_d3_session *s, double *x, double *y, double *vx, double *vy);
+/* Measurement, no reference counterpart: the last tick's two walks timed again on their own
+ * trees (HIP events; medians of three timings of `reps` launches): ms_out[0] the many-body
+ * walk, ms_out[1] the collide walk; cells_out[0..1] the two trees' cell counts. The state is
+ * left as the last tick left it. Refused before the first step. */
+gr_status gr_d3_session_time(gr_d3_session *s, uint32_t reps, double *ms_out,
+                             uint32_t *cells_out);
 /* alpha() (session.rs:238); NaN for a NULL handle. */
 double gr_d3_session_alpha(const gr_d3_session *s);
 /* reheat (session.rs:247-252): alpha taken exactly, refused outside 0..=1. */

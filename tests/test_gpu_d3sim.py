@@ -190,6 +190,25 @@ def test_step_counts_give_the_same_bytes(oracle):
     assert alpha == pytest.approx(0.94 ** 12) and not settled
 
 
+def test_timing_the_walks_leaves_the_state(oracle):
+    s, t, w = oracle.gate_model(2000)
+    with barnes_hut.ForceSession(Topology(2000, s, t), w) as S:
+        with pytest.raises(StageError):
+            S.time_walks(2)  # nothing to time before the first tick
+        S.step(5)
+        before = S.state()
+        mc, mk, cc, ck = S.time_walks(3)
+        assert mc > 0 and mk > 0 and cc >= 1 and ck >= 1
+        for a, b in zip(before, S.state()):
+            assert same_bits(a, b)
+        S.step(3)  # and the next ticks are the ones an untimed session takes
+        after = S.state()
+    with barnes_hut.ForceSession(Topology(2000, s, t), w) as S:
+        S.step(8)
+        for a, b in zip(after, S.state()):
+            assert same_bits(a, b)
+
+
 def test_a_diverging_layout_is_refused_as_non_finite(oracle):
     # a charge near the f64 range overflows the velocities: planar_points refuses the
     # non-finite coordinates (StageError::NonFinite{node.x}) and the columns come back zero

@@ -707,8 +707,9 @@ static void d3_merge(uint32_t n, double *vx, double *vy, const double *d) {
  * link_distance, link_strength_scale, collide_radius, center_strength, velocity_decay,
  * alpha_decay, initial_alpha. x_out, y_out: n each. */
 void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
-                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
-                   double *y_out, double *vx_out, double *vy_out) {
+                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed,
+                   uint32_t mesh_side, double *x_out, double *y_out, double *vx_out,
+                   double *vy_out) {
     const double charge = p[0], theta = p[1], dmin = p[2], dmax = p[3], ldist = p[4];
     const double lscale = p[5], radius = p[6], cstrength = p[7], decay = p[8];
     const double adecay = p[9];
@@ -782,6 +783,7 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
         y[i] = gro_libm_sin(a) * r;
     }
     const double reach2 = (2.0 * radius) * (2.0 * radius);
+    gro_pmd3 *mesh = mesh_side ? gro_pmd3_new(n, (int)mesh_side) : NULL;
     for (uint32_t tick = 0; tick < ticks; tick++) {
         alpha += (0.0 - alpha) * adecay;
         /* link (link.rs force + step.rs LinkPass): per edge, then each node's share */
@@ -818,9 +820,14 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
             d[2 * v + 1] = sy;
         }
         d3_merge(n, vx, vy, d);
-        /* charge */
-        d3_charge_core(n, x, y, theta, charge, alpha, dmin, dmax, seed, tick, d);
-        d3_merge(n, vx, vy, d);
+        /* charge: the quadtree's (barnes_hut/charge.rs), or the mesh's (particle_mesh/
+         * charge.rs: no merge when the mesh places no frame) */
+        if (!mesh) {
+            d3_charge_core(n, x, y, theta, charge, alpha, dmin, dmax, seed, tick, d);
+            d3_merge(n, vx, vy, d);
+        } else if (gro_pmd3_charge(mesh, x, y, dmin, dmax, charge * alpha, d, NULL)) {
+            d3_merge(n, vx, vy, d);
+        }
         /* center (sim.rs center_shift): the mean in node order, times the strength */
         if (n > 0) {
             double sx = 0.0, sy = 0.0;
@@ -839,11 +846,17 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
             px[i] = x[i] + vx[i];
             py[i] = y[i] + vy[i];
         }
-        gro_tree *ct = gro_tree_build(n, 2, px, py, NULL);
-        for (uint32_t i = 0; i < n; i++)
-            d3_collide_delta(ct, i, px, py, reach2, seed, tick, &d[2 * i], &d[2 * i + 1]);
-        gro_tree_free(ct);
-        d3_merge(n, vx, vy, d);
+        if (!mesh) {
+            gro_tree *ct = gro_tree_build(n, 2, px, py, NULL);
+            for (uint32_t i = 0; i < n; i++)
+                d3_collide_delta(ct, i, px, py, reach2, seed, tick, &d[2 * i], &d[2 * i + 1]);
+            gro_tree_free(ct);
+            d3_merge(n, vx, vy, d);
+        } else if (gro_pmd3_collide(mesh, px, py, radius, seed, tick, d)) {
+            /* particle_mesh/collide.rs: the hashed grid (its order kept for the next
+             * deposit); no pass at zero reach */
+            d3_merge(n, vx, vy, d);
+        }
         /* integrate (sim.rs): velocity decays, position moves */
         for (uint32_t i = 0; i < n; i++) {
             vx[i] *= decay;
@@ -856,6 +869,7 @@ void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const d
     memcpy(y_out, y, (size_t)n * sizeof(double));
     if (vx_out) memcpy(vx_out, vx, (size_t)n * sizeof(double));
     if (vy_out) memcpy(vy_out, vy, (size_t)n * sizeof(double));
+    gro_pmd3_free(mesh);
     free(lo);
     free(hi);
     free(str);

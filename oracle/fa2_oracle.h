@@ -90,9 +90,27 @@ void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, co
 /* ---- the frozen layout.force.barnes_hut (barnes_hut.rs, sim.rs tick) ----
  * p: charge, theta, distance_min, distance_max, link_distance, link_strength_scale,
  * collide_radius, center_strength, velocity_decay, alpha_decay, initial_alpha */
+/* mesh_side 0: layout.force.barnes_hut; otherwise layout.force.particle_mesh on a mesh of
+ * that side (its charge and collide passes from pm_d3_oracle.c) */
 void gro_d3sim_run(uint32_t n, const uint32_t *src, const uint32_t *dst, const double *strength,
-                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed, double *x_out,
-                   double *y_out, double *vx_out, double *vy_out);
+                   size_t m_raw, uint32_t ticks, const double *p, uint32_t seed,
+                   uint32_t mesh_side, double *x_out, double *y_out, double *vx_out,
+                   double *vy_out);
+
+/* ---- layout.force.particle_mesh's mesh passes, d3's law (pm_d3_oracle.c) ---- */
+typedef struct gro_pmd3 gro_pmd3;
+gro_pmd3 *gro_pmd3_new(uint32_t n, int side);
+void gro_pmd3_free(gro_pmd3 *m);
+int gro_pmd3_charge(gro_pmd3 *m, const double *x, const double *y, double dmin, double dmax,
+                    double strength, double *out, double *frame_out);
+int gro_pmd3_collide(gro_pmd3 *m, const double *px, const double *py, double radius,
+                     uint32_t seed, uint32_t tick, double *out);
+void gro_pmd3_spectrum(const gro_pmd3 *m, double *re, double *im);
+int gro_pmd3_charge_once(uint32_t n, const double *xy, int side, const uint32_t *order,
+                         double charge, double alpha, double dmin, double dmax, double *out,
+                         double *frame_out, double *spec_re, double *spec_im);
+int gro_pmd3_collide_once(uint32_t n, const double *xy, double radius, uint32_t seed,
+                          uint32_t tick, double *out, uint32_t *order_out);
 /* libm 0.2.16's cos and sin (libm_trig.c); NaN past |x| = 2^20 pi/2 */
 double gro_libm_cos(double x);
 double gro_libm_sin(double x);

@@ -76,8 +76,13 @@ def lib():
         L.gro_gate_model.restype = C.c_size_t
         L.gro_gate_model.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_size_t]
         L.gro_d3sim_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
-                                    C.c_uint32, f64p, C.c_uint32, f64p, f64p, C.c_void_p,
-                                    C.c_void_p]
+                                    C.c_uint32, f64p, C.c_uint32, C.c_uint32, f64p, f64p,
+                                    C.c_void_p, C.c_void_p]
+        L.gro_pmd3_charge_once.argtypes = [C.c_uint32, f64p, C.c_int, C.c_void_p, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, f64p, f64p,
+                                           C.c_void_p, C.c_void_p]
+        L.gro_pmd3_collide_once.argtypes = [C.c_uint32, f64p, C.c_double, C.c_uint32,
+                                            C.c_uint32, f64p, C.c_void_p]
         L.gro_libm_cos.restype = C.c_double
         L.gro_libm_cos.argtypes = [C.c_double]
         L.gro_libm_sin.restype = C.c_double
@@ -409,9 +414,10 @@ def gate_model(count: int):
 
 
 def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0,
-          velocities: bool = False, **params):
+          velocities: bool = False, mesh_side: int = 0, **params):
     """The frozen layout.force.barnes_hut's f64 positions after `ticks` ticks: (x, y), and
-    (x, y, vx, vy) with velocities=True."""
+    (x, y, vx, vy) with velocities=True. mesh_side > 0: layout.force.particle_mesh on a mesh
+    of that side instead (mesh.rs side_for(n) is the stage's)."""
     p = dict(D3_DEFAULTS)
     p.update(params)
     vec = np.array([p[k] for k in ("charge", "theta", "distance_min", "distance_max",
@@ -426,8 +432,8 @@ def d3sim(n: int, source, target, strength, ticks: int = D3_TICKS, seed: int = 0
     vx = np.zeros(max(n, 1))
     vy = np.zeros(max(n, 1))
     lib().gro_d3sim_run(n, s.ctypes.data if s.size else None, t.ctypes.data if t.size else None,
-                        w.ctypes.data if w.size else None, s.size, ticks, vec, seed, x, y,
-                        vx.ctypes.data, vy.ctypes.data)
+                        w.ctypes.data if w.size else None, s.size, ticks, vec, seed,
+                        int(mesh_side), x, y, vx.ctypes.data, vy.ctypes.data)
     if velocities:
         return x[:n], y[:n], vx[:n], vy[:n]
     return x[:n], y[:n]
@@ -447,6 +453,49 @@ def golden_spiral(n: int):
     x = np.array([libm_cos(i * 2.399963229728653) * (12.0 * np.sqrt(i + 1.0)) for i in range(n)])
     y = np.array([libm_sin(i * 2.399963229728653) * (12.0 * np.sqrt(i + 1.0)) for i in range(n)])
     return x, y
+
+
+def pm_side_for(n: int) -> int:
+    """mesh.rs:31-34: ceil(sqrt n) rounded up to a power of two, held to 128..1024."""
+    root = int(np.ceil(np.sqrt(float(n))))
+    side = 1
+    while side < root:
+        side <<= 1
+    return min(max(side, 128), 1024)
+
+
+def pmd3_charge(xy, side: int, charge: float = -90.0, alpha: float = 1.0,
+                distance_min: float = 1.0, distance_max: float = 520.0, order=None):
+    """particle_mesh/charge.rs from a fresh mesh (identity grid order unless `order`):
+    (deltas (n, 2) node order, frame dict or None, kernel spectrum (re, im) or None)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    fr = np.zeros(7)
+    sre = np.zeros(side * side)
+    sim_ = np.zeros(side * side)
+    o = None if order is None else np.ascontiguousarray(order, np.uint32)
+    placed = lib().gro_pmd3_charge_once(n, xy if n else np.zeros(2), side,
+                                        o.ctypes.data if o is not None else None, charge, alpha,
+                                        distance_min, distance_max, out, fr, sre.ctypes.data,
+                                        sim_.ctypes.data)
+    frame = None
+    if placed:
+        frame = dict(step=int(fr[0]), h=float(fr[1]), ox=float(fr[2]), oy=float(fr[3]),
+                     cells=int(fr[4]), reach=int(fr[5]), side=int(fr[6]))
+    return out[: 2 * n].reshape(n, 2), frame, (sre, sim_) if placed else None
+
+
+def pmd3_collide(xy, radius: float = 16.0, seed: int = 0, tick: int = 0):
+    """particle_mesh/collide.rs over these positions (the hashed grid): (deltas (n, 2) node
+    order, the grid order it sorted), deltas zero at radius 0."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    order = np.zeros(max(n, 1), np.uint32)
+    lib().gro_pmd3_collide_once(n, xy if n else np.zeros(2), radius, seed & 0xFFFFFFFF,
+                                tick & 0xFFFFFFFF, out, order.ctypes.data)
+    return out[: 2 * n].reshape(n, 2), order[:n]
 
 
 def d3_collide(xy, radius: float = 16.0, seed: int = 0, tick: int = 0) -> np.ndarray:

@@ -72,6 +72,20 @@ static int place(double lox, double loy, double hix, double hiy, int side, doubl
     }
 }
 
+/* place() for the d3-law mesh of pm_d3_oracle.c: 1 and the frame's fields when one exists */
+int gro_pm_place(double lox, double loy, double hix, double hiy, int side, double dmax,
+                 int *step, double *h, double *ox, double *oy, int *cells, int *reach) {
+    pm_frame f;
+    if (!place(lox, loy, hix, hiy, side, dmax, &f)) return 0;
+    *step = f.step;
+    *h = f.h;
+    *ox = f.ox;
+    *oy = f.oy;
+    *cells = f.cells;
+    *reach = f.reach;
+    return 1;
+}
+
 /* frame.rs:160-187: scaled position, cell, fraction. */
 static void stencil(const pm_frame *f, double px, double py, int *cx, int *cy, double *fx,
                     double *fy) {

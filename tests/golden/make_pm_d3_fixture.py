@@ -4,10 +4,12 @@ reference's fixtures/gpu/README.md): for each state, the node positions, the fra
 reference placed (step, h, origin, cells, reach) and the velocity each pass adds from rest at
 alpha 1 (delta_charge of the mesh, delta_link, delta_collide of its hashed grid at tick 0);
 plus the collapsed simple edges with their strengths (shared by both states) and the
-forward twiddle table of libm's cos and sin. Only inputs and expected outputs are kept; the
-spectrum is left out. Run here, where the reference is mounted:
+forward twiddle table of libm's cos and sin, and the SHA-256 of each state's kernel spectrum
+(its re column's little-endian f64 bytes, then its im column's). Only inputs and expected
+outputs are kept. Run here, where the reference is mounted:
     python tests/golden/make_pm_d3_fixture.py /root/reference
 """
+import hashlib
 import os
 import struct
 import sys
@@ -37,12 +39,14 @@ def load(path):
 
     xy = pair(n)
     twiddle = pair(side)
-    off += 16 * side * side  # the spectrum
+    spectrum = hashlib.sha256(b[off:off + 16 * side * side]).hexdigest()
+    off += 16 * side * side
     link, charge, collide = pair(n), pair(n), pair(n)
     assert off == len(b)
     return {"xy": xy, "side": side, "step": step, "h": h, "origin": (ox, oy),
             "cells": cells, "reach": reach, "delta_charge": charge, "delta_link": link,
-            "delta_collide": collide, "twiddle": twiddle, "edges": (lo, hi, strength)}
+            "delta_collide": collide, "twiddle": twiddle, "edges": (lo, hi, strength),
+            "spectrum_sha256": spectrum}
 
 
 def main(ref_root):
@@ -53,6 +57,7 @@ def main(ref_root):
         out[f"{state}_delta_charge"] = f["delta_charge"]
         out[f"{state}_delta_link"] = f["delta_link"]
         out[f"{state}_delta_collide"] = f["delta_collide"]
+        out[f"{state}_spectrum_sha256"] = np.array(f["spectrum_sha256"])
         out["twiddle"] = f["twiddle"]
         out["edge_lo"], out["edge_hi"], out["edge_strength"] = f["edges"]
         out[f"{state}_frame"] = np.array([f["step"], f["h"], f["origin"][0], f["origin"][1],

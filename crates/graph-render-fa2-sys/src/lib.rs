@@ -201,6 +201,14 @@ extern "C" {
     pub fn gr_d3_collide(n: u32, xy: *const f64, radius: f64, seed: u32, tick: u32,
                          out: *mut f64) -> c_int;
     pub fn gr_d3_params_default(out: *mut gr_d3_params);
+    pub fn gr_pm_layout_run(n: u32, source: *const u32, target: *const u32, strength: *const f64,
+                            m: usize, params: *const gr_d3_params, ticks: u32, out_x: *mut f32,
+                            out_y: *mut f32, ms_ticks_out: *mut f64) -> c_int;
+    pub fn gr_d3_session_with_particle_mesh(s: *mut gr_d3_session) -> c_int;
+    pub fn gr_pm_d3_charge(n: u32, xy: *const f64, charge: f64, alpha: f64, distance_min: f64,
+                           distance_max: f64, out: *mut f64, frame_out: *mut f64) -> c_int;
+    pub fn gr_pm_d3_collide(n: u32, xy: *const f64, radius: f64, seed: u32, tick: u32,
+                            out: *mut f64, order_out: *mut u32) -> c_int;
     pub fn gr_d3_session_create(n: u32, source: *const u32, target: *const u32,
                                 strength: *const f64, m: usize, params: *const gr_d3_params)
                                 -> *mut gr_d3_session;

@@ -116,6 +116,14 @@ SIGNATURES = {
     "gr_d3_collide": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_uint32, C.c_uint32,
                                 C.c_void_p]),
     "gr_d3_params_default": (None, [C.c_void_p]),
+    "gr_pm_layout_run": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                   C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_double)]),
+    "gr_d3_session_with_particle_mesh": (C.c_int, [C.c_void_p]),
+    "gr_pm_d3_charge": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_void_p, C.c_void_p]),
+    "gr_pm_d3_collide": (C.c_int, [C.c_uint32, C.c_void_p, C.c_double, C.c_uint32, C.c_uint32,
+                                   C.c_void_p, C.c_void_p]),
     "gr_d3_session_create": (C.c_void_p, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_size_t, C.c_void_p]),
     "gr_d3_session_step": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),

@@ -55,7 +55,7 @@ class ForceSession:
     ticks on the device; step(112), 112 x step(1) and 7 x step(16) give the same bytes."""
 
     def __init__(self, topology: Topology, strength=None,
-                 params: Optional[ForceParams] = None):
+                 params: Optional[ForceParams] = None, particle_mesh: bool = False):
         self.n = int(topology.n)
         s = _strength(topology, strength)
         cp = (params or ForceParams())._c()
@@ -67,6 +67,14 @@ class ForceSession:
                                                m, C.byref(cp))
         if not self._h:
             _raise()
+        if particle_mesh:
+            self.with_particle_mesh()
+
+    def with_particle_mesh(self):
+        """session.rs with_particle_mesh: later ticks run the many-body pass on the mesh and
+        collision on the hashed grid."""
+        _check(A.lib().gr_d3_session_with_particle_mesh(self._h))
+        return self
 
     def close(self):
         if self._h:
@@ -130,8 +138,10 @@ class BarnesHut:
 
     ID = "layout.force.barnes_hut"
 
-    @staticmethod
-    def _call(topology: Topology, strength, params, ticks, ms):
+    RUN = "gr_d3_layout_run"
+
+    @classmethod
+    def _call(cls, topology: Topology, strength, params, ticks, ms):
         n, m = int(topology.n), int(topology.m)
         s = _strength(topology, strength)
         cp = (params or ForceParams())._c()
@@ -139,10 +149,10 @@ class BarnesHut:
         ys = np.zeros(max(n, 1), np.float32)
         src = topology.source if m else np.zeros(1, np.uint32)
         dst = topology.target if m else np.zeros(1, np.uint32)
-        _check(A.lib().gr_d3_layout_run(n, src.ctypes.data, dst.ctypes.data,
-                                        s.ctypes.data if s is not None and m else None, m,
-                                        C.byref(cp), int(ticks), xs.ctypes.data,
-                                        ys.ctypes.data, ms))
+        _check(getattr(A.lib(), cls.RUN)(n, src.ctypes.data, dst.ctypes.data,
+                                         s.ctypes.data if s is not None and m else None, m,
+                                         C.byref(cp), int(ticks), xs.ctypes.data,
+                                         ys.ctypes.data, ms))
         return Geometry(2, xs[:n], ys[:n], None)
 
     @classmethod
@@ -219,3 +229,47 @@ def node_ids(xy) -> np.ndarray:
     out = np.zeros(max(cells.value, 1), np.uint32)
     _check(A.lib().gr_quadtree_node_ids(n, ptr, cells.value, C.byref(cells), out.ctypes.data))
     return out[: cells.value]
+
+
+class ParticleMesh(BarnesHut):
+    """layout/force/particle_mesh.rs:96-131: BarnesHut::run's session with the many-body pass
+    on a side_for(n) mesh (d3's law) and collision on the hashed grid."""
+
+    ID = "layout.force.particle_mesh"
+    RUN = "gr_pm_layout_run"
+
+
+def pm_charge_pass(xy, charge: float = -90.0, alpha: float = 1.0, distance_min: float = 1.0,
+                   distance_max: float = 520.0):
+    """particle_mesh/charge.rs from rest on a fresh mesh over these positions: (deltas (n, 2)
+    node order, frame dict or None)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    fr = np.zeros(7)
+    _check(A.lib().gr_pm_d3_charge(n, xy.ctypes.data if n else None, float(charge),
+                                   float(alpha), float(distance_min), float(distance_max),
+                                   out.ctypes.data, fr.ctypes.data))
+    frame = None if fr[6] == 0 else dict(step=int(fr[0]), h=float(fr[1]), ox=float(fr[2]),
+                                         oy=float(fr[3]), cells=int(fr[4]), reach=int(fr[5]),
+                                         side=int(fr[6]))
+    return out[: 2 * n].reshape(n, 2), frame
+
+
+def pm_collide_pass(xy, radius: float = 16.0, seed: int = 0, tick: int = 0):
+    """particle_mesh/collide.rs over these positions (the hashed grid): (deltas (n, 2) node
+    order, the grid order)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    out = np.zeros(max(n, 1) * 2)
+    order = np.zeros(max(n, 1), np.uint32)
+    _check(A.lib().gr_pm_d3_collide(n, xy.ctypes.data if n else None, float(radius),
+                                    int(seed) & 0xFFFFFFFF, int(tick) & 0xFFFFFFFF,
+                                    out.ctypes.data, order.ctypes.data))
+    return out[: 2 * n].reshape(n, 2), order[:n]
+
+
+def pm_side_for(n: int) -> int:
+    """mesh.rs:31-34: ceil(sqrt n) rounded up to a power of two, held to 128..1024."""
+    return int(A.lib().gr_pm_side_for(int(n)))
+

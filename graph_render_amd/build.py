@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libgraph_render_fa2.so")
 SOURCES = ["fa2_session.cpp", "fa2_kernels.hip", "sort_scan.hip", "pm.hip", "adjoint.hip",
-           "snapshot.cpp", "noalloc.hip", "d3.hip", "forcesim.hip"]
+           "snapshot.cpp", "noalloc.hip", "d3.hip", "forcesim.hip", "pm_d3.hip"]
 ARCH = os.environ.get("GR_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: FP64 kernels must round as the reference's Rust does (no FMA fusion).

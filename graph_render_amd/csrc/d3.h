@@ -102,4 +102,44 @@ void d3sim_collide(const D3SimDev &s, const D3CollideTree &t, uint32_t tick);
 // sim.rs integrate: v *= decay; position += v.
 void d3sim_integrate(const D3SimDev &s);
 
+// ---- layout.force.particle_mesh's mesh passes (pm_d3.hip) ----
+
+struct PmD3Frame {
+    double h, ox, oy;
+    int32_t step, cells, reach, valid;
+};
+
+struct PmD3Dev {
+    uint32_t side = 0, bits = 0;      // mesh side (a power of two) and its log2
+    uint32_t nparts = 0;              // bounds partials
+    uint32_t bucket_bits = 0;         // the hashed grid: 2^bucket_bits buckets
+    double dmin2 = 1.0, dmax2 = 0.0;  // the law's cutoffs, squared
+    const double2 *tw_fwd = nullptr, *tw_inv = nullptr;  // side: fft.rs Plan tables
+    double2 *a = nullptr, *b = nullptr;  // side * side: density / field, transform scratch
+    double2 *ks = nullptr;               // side * side: the kernel's spectrum
+    PmD3Frame *fr = nullptr;
+    double *part = nullptr;    // nparts * 4 bounds partials
+    double *origin = nullptr;  // 2: the hash origin
+    uint32_t *order = nullptr, *slot = nullptr;  // n: the grid (slot -> node, node -> slot)
+    uint32_t *cell = nullptr;  // n: each slot's stencil cell (side * side: none)
+    double *fxy = nullptr;     // 2n: each slot's stencil fractions
+    uint64_t *keys = nullptr, *keys_alt = nullptr;  // n: sort keys
+    uint32_t *vals = nullptr, *vals_alt = nullptr;  // n
+    void *sort_tmp = nullptr;
+    uint32_t *cstart = nullptr;  // side * side + 1: first sorted slot of each stencil cell
+    uint32_t *bstart = nullptr;  // 2^bucket_bits + 1: first slot of each bucket
+    double2 *at = nullptr;       // n: projected positions in grid order
+};
+
+// The frame over the positions (bounds partials folded, frame.rs place) into m.fr.
+void pmd3_frame(const D3SimDev &s, const PmD3Dev &m);
+// The deposit into m.a's first `cells` rows, each cell in ascending grid slot.
+void pmd3_deposit(const D3SimDev &s, const PmD3Dev &m, uint32_t cells);
+// The kernel's spectrum at this rung and reach into m.ks (m.b as scratch).
+void pmd3_spectrum(const PmD3Dev &m, double h, int reach, hipStream_t stream);
+// Forward transform, spectrum product, inverse, and the read into s.dl (node order).
+void pmd3_solve_read(const D3SimDev &s, const PmD3Dev &m, uint32_t cells, double strength);
+// The hashed grid over s.px, s.py (m.order rebuilt) and every node's pushes into s.dl.
+void pmd3_collide(const D3SimDev &s, const PmD3Dev &m, uint32_t tick);
+
 }  // namespace gr

@@ -1785,6 +1785,11 @@ struct D3Sim {
     // the positions the last tick left, ev_c the shift it computed
     hipStream_t side = nullptr;
     hipEvent_t ev_x = nullptr, ev_c = nullptr;
+    // layout.force.particle_mesh: the many-body and collide passes on the mesh instead
+    bool mesh = false;
+    gr::PmD3Dev pm;
+    bool k_valid = false;  // the spectrum in pm.ks was built for (k_step, k_reach)
+    int k_step = 0, k_reach = 0;
     ~D3Sim() {
         if (side) {
             (void)hipStreamSynchronize(side);
@@ -1915,6 +1920,65 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     return GR_OK;
 }
 
+// mesh.rs Mesh::new on the device: fft.rs Plan's twiddle tables from libm's cos and sin,
+// the transform and spectrum grids, and a fresh grid (the identity order, collide.rs
+// Grid::new).
+static gr_status pmd3_setup(D3Sim &S) {
+    gr_fa2_session *c = S.cs;
+    gr::PmD3Dev &m = S.pm;
+    const uint32_t n = S.sd.n, side = gr_pm_side_for(n);
+    m.side = side;
+    m.bits = 0;
+    while ((1u << m.bits) < side) m.bits++;
+    m.nparts = std::min<uint32_t>(gr_blocks(n, GR_BLOCK), 256);
+    uint32_t buckets = 4;
+    while (buckets < 2 * n) buckets <<= 1;
+    m.bucket_bits = 0;
+    while ((1u << m.bucket_bits) < buckets) m.bucket_bits++;
+    m.dmin2 = S.charge.g.dmin2;
+    m.dmax2 = S.charge.g.dmax2;
+    const size_t p2 = (size_t)side * side, nn = n ? n : 1;
+    double2 *tf = nullptr, *ti = nullptr;
+    gr_status r;
+    if ((r = c->alloc(&tf, side * 16)) || (r = c->alloc(&ti, side * 16)) ||
+        (r = c->alloc(&m.a, p2 * 16)) || (r = c->alloc(&m.b, p2 * 16)) ||
+        (r = c->alloc(&m.ks, p2 * 16)) || (r = c->alloc(&m.fr, sizeof(gr::PmD3Frame))) ||
+        (r = c->alloc(&m.part, (size_t)m.nparts * 32)) || (r = c->alloc(&m.origin, 16)) ||
+        (r = c->alloc(&m.order, nn * 4)) || (r = c->alloc(&m.slot, nn * 4)) ||
+        (r = c->alloc(&m.cell, nn * 4)) || (r = c->alloc(&m.fxy, nn * 16)) ||
+        (r = c->alloc(&m.keys, nn * 8)) || (r = c->alloc(&m.keys_alt, nn * 8)) ||
+        (r = c->alloc(&m.vals, nn * 4)) || (r = c->alloc(&m.vals_alt, nn * 4)) ||
+        (r = c->alloc(&m.sort_tmp, gr::radix_tmp_bytes((uint32_t)nn))) ||
+        (r = c->alloc(&m.cstart, (p2 + 1) * 4)) ||
+        (r = c->alloc(&m.bstart, ((size_t)buckets + 1) * 4)) || (r = c->alloc(&m.at, nn * 16)))
+        return r;
+    // fft.rs Plan::new: root k = (cos, sin)(-tau k / side); forward[k] = root[(k mod half) *
+    // side / (2 half)], half the largest power of two <= max(k, 1); inverse conjugated
+    std::vector<double2> root(side / 2), fwd(side), inv(side);
+    const double tau = 6.283185307179586;
+    for (uint32_t k = 0; k < side / 2; k++) {
+        const double angle = -tau * (double)k / (double)side;
+        root[k] = make_double2(gr_libm::cos(angle), gr_libm::sin(angle));
+    }
+    for (uint32_t k = 0; k < side; k++) {
+        uint32_t half = 1;
+        while (half * 2 <= std::max<uint32_t>(k, 1)) half *= 2;
+        fwd[k] = root[(k % half) * (side / (2 * half))];
+        inv[k] = make_double2(fwd[k].x, -fwd[k].y);
+    }
+    HIP_TRY(hipMemcpy(tf, fwd.data(), side * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ti, inv.data(), side * 16, hipMemcpyHostToDevice));
+    m.tw_fwd = tf;
+    m.tw_inv = ti;
+    std::vector<uint32_t> iota(nn);
+    for (uint32_t i = 0; i < nn; i++) iota[i] = i;
+    HIP_TRY(hipMemcpy(m.order, iota.data(), nn * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m.slot, iota.data(), nn * 4, hipMemcpyHostToDevice));
+    S.mesh = true;
+    S.k_valid = false;
+    return GR_OK;
+}
+
 // One tick (sim.rs tick): alpha decays, then link, many-body, center, collide, integrate.
 static gr_status d3sim_tick(D3Sim &S, double alpha, uint32_t tick) {
     gr::D3SimDev &sd = S.sd;
@@ -1925,17 +1989,49 @@ static gr_status d3sim_tick(D3Sim &S, double alpha, uint32_t tick) {
     HIP_TRY(hipEventRecord(S.ev_c, S.side));
     gr::d3sim_link(sd, alpha, tick);
     gr_status r;
-    // many-body over the quadtree of the positions (bounds of the moved positions first)
-    gr::launch_bounds(S.cs->d);
-    if ((r = S.charge.tree()) || (r = S.charge.alloc(false))) return r;
-    S.charge.g.alpha = alpha;
-    S.charge.g.tick = tick;
-    if ((r = S.charge.bodies())) return r;
-    gr::d3_launch_walk(S.charge.g);
-    gr::d3sim_merge(sd, S.charge.g.out);
+    if (S.mesh) {
+        // particle_mesh/charge.rs: the mesh's frame over the positions; no frame, no pass
+        gr::pmd3_frame(sd, S.pm);
+        gr::PmD3Frame f;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&f, S.pm.fr, sizeof(f), hipMemcpyDeviceToHost, sd.stream));
+        HIP_TRY(hipStreamSynchronize(sd.stream));
+        if (f.valid) {
+            if (f.cells < 2 || (uint32_t)f.cells > S.pm.side || f.reach < 0)
+                return fail(GR_ERR_DEVICE, "mesh", "a frame outside the mesh");
+            gr::pmd3_deposit(sd, S.pm, (uint32_t)f.cells);
+            if (!S.k_valid || S.k_step != f.step || S.k_reach != f.reach) {
+                gr::pmd3_spectrum(S.pm, f.h, f.reach, sd.stream);
+                S.k_valid = true;
+                S.k_step = f.step;
+                S.k_reach = f.reach;
+            }
+            gr::pmd3_solve_read(sd, S.pm, (uint32_t)f.cells, S.charge.g.charge * alpha);
+            gr::d3sim_merge(sd, sd.dl);
+        }
+    } else {
+        // many-body over the quadtree of the positions (bounds of the moved positions first)
+        gr::launch_bounds(S.cs->d);
+        if ((r = S.charge.tree()) || (r = S.charge.alloc(false))) return r;
+        S.charge.g.alpha = alpha;
+        S.charge.g.tick = tick;
+        if ((r = S.charge.bodies())) return r;
+        gr::d3_launch_walk(S.charge.g);
+        gr::d3sim_merge(sd, S.charge.g.out);
+    }
     // center, then collide over the quadtree of the projected positions
     HIP_TRY(hipStreamWaitEvent(sd.stream, S.ev_c, 0));
     gr::d3sim_center_project(sd);
+    if (S.mesh) {
+        // particle_mesh/collide.rs: the hashed grid; no pass at zero reach
+        if (sd.reach2 != 0.0) {
+            gr::pmd3_collide(sd, S.pm, tick);
+            gr::d3sim_merge(sd, sd.dl);
+        }
+        gr::d3sim_integrate(sd);
+        HIP_TRY(hipGetLastError());
+        return GR_OK;
+    }
     gr::launch_bounds(S.ks->d);
     if ((r = S.collide.tree())) return r;
     gr::D3CollideTree t;
@@ -2048,6 +2144,7 @@ gr_status gr_d3_session_time(gr_d3_session *h, uint32_t reps, double *ms_out,
     if (!ms_out || !cells_out) return fail(GR_ERR_PARAM, "out", "two times and two counts");
     D3Sim &S = h->S;
     if (!h->tick || !S.sd.n) return fail(GR_ERR_PARAM, "session", "a session stepped at least once");
+    if (S.mesh) return fail(GR_ERR_UNSUPPORTED, "session", "a session on the quadtree passes");
     if (reps == 0) reps = 1;
     gr::D3CollideTree t;
     t.C = S.collide.g.C;
@@ -2087,9 +2184,132 @@ gr_status gr_d3_session_set_alpha_target(gr_d3_session *h, double target) {
     return ok();
 }
 
+// One mesh pass over given positions: a session of n edge-free nodes on a fresh mesh whose
+// position (or projected position) columns are set to xy.
+static gr_status pm_pass_session(uint32_t n, const double *xy, double radius, double dmin,
+                                 double dmax, bool projected, gr_d3_session **out) {
+    gr_d3_params p;
+    gr_d3_params_default(&p);
+    p.collide_radius = radius;
+    p.distance_min = dmin;
+    p.distance_max = dmax;
+    gr_d3_session *h = gr_d3_session_create(n, nullptr, nullptr, nullptr, 0, &p);
+    if (!h) return g_err.code;
+    *out = h;
+    gr_status r = gr_d3_session_with_particle_mesh(h);
+    if (r) return r;
+    std::vector<double> x(n), y(n);
+    for (uint32_t i = 0; i < n; i++) {
+        x[i] = xy[2 * (size_t)i];
+        y[i] = xy[2 * (size_t)i + 1];
+    }
+    gr::D3SimDev &sd = h->S.sd;
+    HIP_TRY(hipMemcpy(projected ? sd.px : sd.x, x.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(projected ? sd.py : sd.y, y.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    return GR_OK;
+}
+
+gr_status gr_pm_d3_charge(uint32_t n, const double *xy, double charge, double alpha,
+                          double distance_min, double distance_max, double *out,
+                          double *frame_out) {
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (frame_out)
+        for (int k = 0; k < 7; k++) frame_out[k] = 0.0;
+    if (!std::isfinite(charge) || !std::isfinite(alpha) || !(distance_min >= 0.0) ||
+        !(distance_max >= 0.0) || !std::isfinite(distance_max))
+        return fail(GR_ERR_PARAM, "charge", "finite charge, alpha and distances >= 0");
+    if (n == 0) return ok();
+    for (size_t q = 0; q < (size_t)n * 2; q++) out[q] = 0.0;
+    gr_d3_session *h = nullptr;
+    gr_status r = pm_pass_session(n, xy, 16.0, distance_min, distance_max, false, &h);
+    if (!r) {
+        D3Sim &S = h->S;
+        gr::pmd3_frame(S.sd, S.pm);
+        gr::PmD3Frame f;
+        // the session's stream does not wait for the null stream's copies: read on it
+        if (hipMemcpyAsync(&f, S.pm.fr, sizeof(f), hipMemcpyDeviceToHost, S.sd.stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(S.sd.stream) != hipSuccess) {
+            r = fail(GR_ERR_DEVICE, "mesh", "frame readback");
+        } else if (f.valid) {
+            gr::pmd3_deposit(S.sd, S.pm, (uint32_t)f.cells);
+            gr::pmd3_spectrum(S.pm, f.h, f.reach, S.sd.stream);
+            gr::pmd3_solve_read(S.sd, S.pm, (uint32_t)f.cells, charge * alpha);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(S.sd.stream) != hipSuccess ||
+                hipMemcpy(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess)
+                r = fail(GR_ERR_DEVICE, "mesh", "the mesh pass failed on the device");
+            if (frame_out) {
+                frame_out[0] = f.step;
+                frame_out[1] = f.h;
+                frame_out[2] = f.ox;
+                frame_out[3] = f.oy;
+                frame_out[4] = f.cells;
+                frame_out[5] = f.reach;
+                frame_out[6] = S.pm.side;
+            }
+        }
+    }
+    gr_d3_session_release(h);
+    return r ? r : ok();
+}
+
+gr_status gr_pm_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed,
+                           uint32_t tick, double *out, uint32_t *order_out) {
+    if (n > 0 && (!xy || !out)) return fail(GR_ERR_PARAM, "xy", "n*2 positions and an output");
+    if (!std::isfinite(radius) || radius < 0.0)
+        return fail(GR_ERR_PARAM, "radius", "a finite radius >= 0");
+    if (n == 0) return ok();
+    for (size_t q = 0; q < (size_t)n * 2; q++) out[q] = 0.0;
+    gr_d3_session *h = nullptr;
+    gr_status r = pm_pass_session(n, xy, radius, 1.0, 520.0, true, &h);
+    if (!r && h->S.sd.reach2 != 0.0) {
+        D3Sim &S = h->S;
+        S.sd.seed = seed;
+        gr::pmd3_collide(S.sd, S.pm, tick);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(S.sd.stream) != hipSuccess ||
+            hipMemcpy(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+            (order_out &&
+             hipMemcpy(order_out, S.pm.order, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess))
+            r = fail(GR_ERR_DEVICE, "mesh", "the collide pass failed on the device");
+    }
+    gr_d3_session_release(h);
+    return r ? r : ok();
+}
+
+gr_status gr_d3_session_with_particle_mesh(gr_d3_session *h) {
+    if (!h) return fail(GR_ERR_INVALID_HANDLE, "session", "a live session");
+    if (h->S.mesh) return ok();  // a mesh is made once; the reference's builder replaces it
+    if (!h->S.sd.n) {
+        h->S.mesh = true;
+        return ok();
+    }
+    gr_status r = pmd3_setup(h->S);
+    return r ? r : ok();
+}
+
+static gr_status d3_layout(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, bool mesh, float *out_x, float *out_y,
+                           double *ms_ticks_out);
+
 gr_status gr_d3_layout_run(uint32_t n, const uint32_t *source, const uint32_t *target,
                            const double *strength, size_t m, const gr_d3_params *params,
                            uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out) {
+    return d3_layout(n, source, target, strength, m, params, ticks, false, out_x, out_y,
+                     ms_ticks_out);
+}
+
+gr_status gr_pm_layout_run(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out) {
+    return d3_layout(n, source, target, strength, m, params, ticks, true, out_x, out_y,
+                     ms_ticks_out);
+}
+
+static gr_status d3_layout(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, bool mesh, float *out_x, float *out_y,
+                           double *ms_ticks_out) {
     if (n >= 0xFFFFFFFFu) return fail(GR_ERR_CAPACITY, "nodes", "node count fits the u32 index space");
     auto zero = [&]() {
         if (out_x) std::memset(out_x, 0, (size_t)n * 4);
@@ -2111,6 +2331,11 @@ gr_status gr_d3_layout_run(uint32_t n, const uint32_t *source, const uint32_t *t
     if (!h) {
         zero();
         return g_err.code;
+    }
+    if (mesh && (r = gr_d3_session_with_particle_mesh(h))) {
+        gr_d3_session_release(h);
+        zero();
+        return r;
     }
     std::vector<double> x(n), y(n);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2163,9 +2388,10 @@ gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *sourc
         return fail(GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id");
     }
     std::string id(layout_id);
-    if (id == "layout.force.barnes_hut") {
-        // registered with no published parameters (registry/layouts.rs:92-97); the frozen
-        // ForceParams, 112 ticks; the edges' strength column arrives as ext->edge_weights
+    if (id == "layout.force.barnes_hut" || id == "layout.force.particle_mesh") {
+        // registered with no published parameters (registry/layouts.rs:92-97 and
+        // registry/force.rs:291-298); the frozen ForceParams, 112 ticks; the edges' strength
+        // column arrives as ext->edge_weights
         if (params_len != 0) {
             zero();
             return fail(GR_ERR_PARAMS_MALFORMED, "params",
@@ -2180,13 +2406,13 @@ gr_status gr_layout_run(const char *layout_id, uint32_t n, const uint32_t *sourc
                     (ext->repulsion != GR_REP_DEFAULT && ext->repulsion != GR_REP_TREE))) {
             zero();
             return fail(GR_ERR_UNSUPPORTED, "ext",
-                        "layout.force.barnes_hut takes edge_weights and theta only");
+                        "the force layouts take edge_weights and theta only");
         }
         gr_d3_params p;
         gr_d3_params_default(&p);
         if (ext && ext->theta >= 0.0) p.theta = ext->theta;
-        return gr_d3_layout_run(n, source, target, ext ? ext->edge_weights : nullptr, m, &p,
-                                112, out_x, out_y, nullptr);
+        return d3_layout(n, source, target, ext ? ext->edge_weights : nullptr, m, &p, 112,
+                         id == "layout.force.particle_mesh", out_x, out_y, nullptr);
     }
     if (id == "layout.forceatlas2.forcesim") {
         // registered with no published parameters (registry/forceatlas2_forcesim.rs:67-72):

@@ -296,6 +296,19 @@ LAYOUTS["layout.force.barnes_hut"] = Capability(
     "d3-force@3.0.0 src/{manyBody,link,center,collide}.js at TICKS=112")
 
 
+def _pm_run(topology: Topology, params=None, ext=None) -> Geometry:
+    if params is not None:
+        raise StageError(A.GR_ERR_UNSUPPORTED, "params",
+                         "layout.force.particle_mesh publishes none")
+    return _run_id("layout.force.particle_mesh", topology, b"", ext, 2)
+
+
+# registry/force.rs:291-298: no published parameters; ceiling PM_CEILING 800 000 (:137)
+LAYOUTS["layout.force.particle_mesh"] = Capability(
+    "layout.force.particle_mesh", _pm_run, (), 800_000,
+    "graph-cli stress --oracle d3 --layout layout.force.particle_mesh")
+
+
 def find(layout_id: str) -> Capability:
     if layout_id not in LAYOUTS:
         raise StageError(A.GR_ERR_UNKNOWN_LAYOUT, "layout_id", "a registered layout id")

@@ -105,8 +105,9 @@ void gr_fa2_ext_default(gr_fa2_ext *out);
 gr_status gr_fa2_params_decode(const uint8_t *bytes, size_t len, gr_fa2_params *out);
 
 /* Stage::run for one of the registered ids (stage.rs:23-31); the FA2 ids below, and
- * "layout.forceatlas2.forcesim" and "layout.force.barnes_hut", which publish no parameters
- * (params_len 0; for the latter ext->edge_weights carries the edges' strength column). For
+ * "layout.forceatlas2.forcesim", "layout.force.barnes_hut" and "layout.force.particle_mesh",
+ * which publish no parameters (params_len 0; for the force layouts ext->edge_weights carries
+ * the edges' strength column and ext->theta, if >= 0, the opening angle). For
  * the three FA2 ids: dedups the raw edge list as
  * simple_graph does (layout/force/mod.rs:101-120), runs to max_iter or the 1e-10 early
  * exit, and writes f32 columns (x, y, and z for the 3D id) narrowed as settle() narrows
@@ -350,6 +351,26 @@ gr_status gr_d3_layout_run(uint32_t n, const uint32_t *source, const uint32_t *t
                            const double *strength, size_t m, const gr_d3_params *params,
                            uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out);
 
+/* ParticleMesh::run (layout/force/particle_mesh.rs:96-131): gr_d3_layout_run with the
+ * many-body pass on the mesh (d3's law on a side_for(n) mesh: frame, slot-ordered CIC
+ * deposit, the reference's radix-2 transforms, kernel spectrum, CIC read) and collision on
+ * the hashed grid (particle_mesh/collide.rs), FP64 in the reference's operations and orders.
+ * "layout.force.particle_mesh" in gr_layout_run. */
+gr_status gr_pm_layout_run(uint32_t n, const uint32_t *source, const uint32_t *target,
+                           const double *strength, size_t m, const gr_d3_params *params,
+                           uint32_t ticks, float *out_x, float *out_y, double *ms_ticks_out);
+
+/* The mesh's two passes alone over given positions (a fresh mesh: the identity grid order):
+ * particle_mesh/charge.rs apply from rest (deltas = field times charge * alpha, node order;
+ * zero where the mesh places no frame; frame_out: step, h, origin x, origin y, cells, reach,
+ * side) and particle_mesh/collide.rs apply (the hashed grid over xy as the projected
+ * positions; order_out, may be NULL: the grid order it sorted). FP64, bit for bit. */
+gr_status gr_pm_d3_charge(uint32_t n, const double *xy, double charge, double alpha,
+                          double distance_min, double distance_max, double *out,
+                          double *frame_out);
+gr_status gr_pm_d3_collide(uint32_t n, const double *xy, double radius, uint32_t seed,
+                           uint32_t tick, double *out, uint32_t *order_out);
+
 /* ---- the resident simulation: ForceSession (layout/force/session.rs:118-270) in HBM ---- */
 typedef struct gr_d3_session gr_d3_session;
 
@@ -367,10 +388,13 @@ gr_status gr_d3_session_step(gr_d3_session *s, uint32_t ticks, double *alpha_out
 /* xs(), ys() (session.rs:223-231) and, with no reference counterpart, the velocity columns:
  * f64, node order; any pointer may be NULL. */
 gr_status gr_d3_session_state(gr_d3_session *s, double *x, double *y, double *vx, double *vy);
+/* with_particle_mesh (session.rs:156-159): later ticks take the many-body pass on the mesh
+ * and collision on the hashed grid (a fresh mesh: the identity grid order). */
+gr_status gr_d3_session_with_particle_mesh(gr_d3_session *s);
 /* Measurement, no reference counterpart: the last tick's two walks timed again on their own
  * trees (HIP events; medians of three timings of `reps` launches): ms_out[0] the many-body
  * walk, ms_out[1] the collide walk; cells_out[0..1] the two trees' cell counts. The state is
- * left as the last tick left it. Refused before the first step. */
+ * left as the last tick left it. Refused before the first step and on a mesh session. */
 gr_status gr_d3_session_time(gr_d3_session *s, uint32_t reps, double *ms_out,
                              uint32_t *cells_out);
 /* alpha() (session.rs:238); NaN for a NULL handle. */

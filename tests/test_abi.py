@@ -88,10 +88,11 @@ def test_unknown_layout_id():
     assert err.value.code == A.GR_ERR_UNKNOWN_LAYOUT
 
 
-def test_registry_lists_the_five_ids():
+def test_registry_lists_the_six_ids():
     assert set(G.LAYOUTS) == {"layout.forceatlas2", "layout.forceatlas2.barnes_hut",
                               "layout.forceatlas2.3d", "layout.forceatlas2.forcesim",
-                              "layout.force.barnes_hut"}
+                              "layout.force.barnes_hut", "layout.force.particle_mesh"}
+    assert G.find("layout.force.particle_mesh").scale_ceiling == 800_000
     assert G.find("layout.forceatlas2.barnes_hut").scale_ceiling == 250_000
     assert G.find("layout.forceatlas2.forcesim").scale_ceiling == 2_000
     assert G.find("layout.forceatlas2.forcesim").params == ()

@@ -225,6 +225,24 @@ def max_rel(approx, exact) -> float:
     return float((d[ok] / e[ok]).max())
 
 
+def conditioned(approx, exact, parts):
+    """The per-node error against the node's force scale: |a - e| / (|attraction| + |repulsion|
+    + |gravity|) of the FP64 parts at the same positions. Where the three nearly cancel, |e| is
+    a small fraction of that scale and |a - e| / |e| grows by the same factor whatever the
+    arithmetic; this is the error with that factor taken out. Also the count of nodes whose
+    error relative to |e| passes 1e-5, and the cancellation |e| / scale at the worst of them."""
+    d = np.sqrt(((approx - exact) ** 2).sum(1))
+    e = np.sqrt((exact ** 2).sum(1))
+    scale = sum(np.sqrt((q ** 2).sum(1)) for q in parts)
+    ok = scale > 0
+    r = np.zeros_like(d)
+    r[e > 0] = d[e > 0] / e[e > 0]
+    worst = int(np.argmax(r))
+    return {"max_vs_force_scale": float((d[ok] / scale[ok]).max()),
+            "nodes_over_1e-5": int((r > 1e-5).sum()),
+            "cancellation_at_max": float(e[worst] / scale[worst]) if scale[worst] > 0 else None}
+
+
 def rel_errors(approx, exact):
     """Per-node relative error |a - e| / |e| (max and mean over nodes with e != 0) and the
     whole-field ratio sqrt(sum |a - e|^2 / sum |e|^2)."""
@@ -558,11 +576,15 @@ def bench_d3sim(args):
     achieved = wb / (ms_collide * 1e-3) / 1e9
     print(f"d3sim walks: many-body {ms_charge:.3f} ms ({c_charge} cells), collide "
           f"{ms_collide:.3f} ms ({c_collide} cells)", file=sys.stderr, flush=True)
+    # the reference's own stage time on this graph (gate model, 100k nodes, 154 978 edges):
+    # 13 735.36 ms, one thread, one run (docs/measurements/perf-p2-pm.md:36)
+    published_ms = 13735.36 if (n == 100_000 and len(s) == 154_978) else None
     line = {
         "metric": "layout.force.barnes_hut layouts/s @100k nodes (112 ticks), FP64",
         "value": round(1e3 / ms, 4), "unit": "layouts/s", "n_gpus": 1,
         "steps": len(runs), "warmup": 1, "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(published_ms / ms, 2) if published_ms else None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"the gate model (synthetic.rs seeded_model, Mulberry32 "
                                f"0x051042) at {n} nodes, {len(s)} raw edges, golden-spiral "
@@ -576,6 +598,9 @@ def bench_d3sim(args):
                      "algorithmic_bytes": wb},
         "finite": bool(np.isfinite(geo.x).all() and np.isfinite(geo.y).all()),
     }
+    if published_ms:
+        line["baseline"] = {"value_ms": published_ms, "source": "the reference's stage time, "
+                            "one thread on a shared x86_64 host (perf-p2-pm.md:36)"}
     if not args.no_cpu:
         from oracle import oracle as O
         k = 12  # about 6 s of oracle work at 100k nodes
@@ -595,6 +620,74 @@ def bench_d3sim(args):
             "sample": f"the first {k} ticks of the same layout through oracle/fa2_oracle.c "
                       f"(gro_d3sim_run, {el:.1f}s, one thread pinned to core {core}), scaled "
                       "to 112 ticks; the device state after those ticks compared bit for bit",
+            **info}
+    print(json.dumps(line), flush=True)
+
+
+def bench_pmsim(args):
+    """layout.force.particle_mesh on the device: the gate model at 1M nodes (1 549 929 raw
+    edges, the graph of the reference's own stage timing), 112 ticks of link, many-body on
+    the side_for(n) mesh (frame, slot-ordered CIC deposit, radix-2 FP64 transforms, kernel
+    spectrum, CIC read), center, collide on the hashed grid and integration, FP64 in the
+    reference's orders. Layouts/s (the tick loop timed with HIP events) against the
+    reference's published 68 483.65 ms per 1M layout; the state after a sample of ticks
+    compared bit for bit with the oracle's, whose run of those ticks gives the CPU rate."""
+    from graph_render_amd import barnes_hut as BH
+    from graph_render_amd import synth
+    import graph_render_amd as G
+    n = args.nodes or 1_000_000
+    s, t, w = synth.gate_model(n)
+    top = G.Topology(n, s, t)
+    BH.ParticleMesh.time_run(top, w, ticks=8)  # warm-up: code objects, allocator
+    runs = []
+    for _ in range(max(args.steps, 1)):
+        geo, ms = BH.ParticleMesh.time_run(top, w, ticks=BH.TICKS)
+        runs.append(ms)
+        print(f"pmsim n={n}: 112 ticks {ms:.2f} ms", file=sys.stderr, flush=True)
+    ms = float(np.median(runs))
+    side = BH.pm_side_for(n)
+    # the reference's stage time on this graph: 68 483.65 ms, median of 3, one thread
+    # (docs/measurements/perf-p2-pm.md:39)
+    published_ms = 68483.65 if (n == 1_000_000 and len(s) == 1_549_929) else None
+    line = {
+        "metric": "layout.force.particle_mesh layouts/s @1M nodes (112 ticks), FP64",
+        "value": round(1e3 / ms, 4), "unit": "layouts/s", "n_gpus": 1,
+        "steps": len(runs), "warmup": 1, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(published_ms / ms, 2) if published_ms else None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"the gate model (synthetic.rs seeded_model, Mulberry32 "
+                               f"0x051042) at {n} nodes, {len(s)} raw edges, golden-spiral "
+                               f"start, ForceParams defaults, {side}x{side} mesh, 112 ticks",
+                   "nodes": n, "raw_edges": int(len(s)), "ticks": BH.TICKS, "mesh_side": side},
+        "ms_per_tick": round(ms / BH.TICKS, 4),
+        "finite": bool(np.isfinite(geo.x).all() and np.isfinite(geo.y).all()),
+    }
+    if published_ms:
+        line["baseline"] = {"value_ms": published_ms, "source": "the reference's stage time, "
+                            "median of 3, one thread on a shared x86_64 host "
+                            "(perf-p2-pm.md:39)"}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        k = 4  # about 2 s of oracle work at 1M nodes
+        with BH.ForceSession(top, w) as S:
+            S.with_particle_mesh()
+            S.step(k)
+            dev = S.state()
+        info = cpu_info()
+        core = pin_one_core()
+        t0 = time.perf_counter()
+        want = O.d3sim(n, s, t, w, ticks=k, velocities=True, mesh_side=side)
+        el = time.perf_counter() - t0
+        line["bit_exact_vs_oracle"] = all(
+            bool(np.array_equal(a.view(np.uint64), b.view(np.uint64))) for a, b in zip(dev, want))
+        line["cpu_baseline"] = {
+            "value": round(1.0 / (el * BH.TICKS / k), 6), "unit": "layouts/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the first {k} ticks of the same layout through oracle/pm_d3_oracle.c "
+                      f"(gro_d3sim_run on the mesh, {el:.1f}s with setup, one thread pinned to "
+                      f"core {core}), scaled to 112 ticks; the device state after those ticks "
+                      "compared bit for bit",
             **info}
     print(json.dumps(line), flush=True)
 
@@ -673,7 +766,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
-                                           "autotune", "d3", "forcesim", "d3sim"],
+                                           "autotune", "d3", "forcesim", "d3sim", "pmsim"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -698,6 +791,9 @@ def main():
         return
     if args.workload == "d3sim":
         bench_d3sim(args)
+        return
+    if args.workload == "pmsim":
+        bench_pmsim(args)
         return
     if args.workload == "forcesim":
         bench_forcesim(args)
@@ -780,6 +876,7 @@ def main():
     s64, fp64_its = fp64_rate()
     s64.set_positions(pos32)
     f64 = s64.forces(7)
+    f64_parts = [s64.forces(1), s64.forces(2), s64.forces(4)]  # attraction, repulsion, gravity
     s64.close()
     s64b, fp64_block_its = fp64_rate(block_sums=True)
     s64b.close()
@@ -842,11 +939,13 @@ def main():
         "fp64_node_order_sums_iters_per_s": round(fp64_its, 3),
         "fp64_steps": args.fp64_steps,
         "force_err_fp32_vs_fp64": {"per_node_rms": node_err, "whole_field": field_err,
-                                   "max_per_node": max_rel(f32, f64)},
+                                   "max_per_node": max_rel(f32, f64),
+                                   **conditioned(f32, f64, f64_parts)},
         "fp32_kahan_iters_per_s": round(kahan_its, 3),
         "force_err_fp32_kahan_vs_fp64": {"per_node_rms": knode_err,
                                          "whole_field": kfield_err,
-                                         "max_per_node": max_rel(fk, f64)},
+                                         "max_per_node": max_rel(fk, f64),
+                                         **conditioned(fk, f64, f64_parts)},
         "pm_2048_iters_per_s": round(pm_its, 3),
         "pm_2048_repulsion_ms": round(pm_ms, 4),
         "pm_vs_bh_fp64_repulsion_err": {"per_node_rms": pm_node, "whole_field": pm_field},

@@ -2,9 +2,8 @@
 // libm::cos, libm::sin of the Rust libm crate 0.2.16, Cargo.lock:271-273), host side: that
 // crate ports musl's math library, i.e. Sun's fdlibm. Arguments within 2^20 pi/2 are reduced
 // as rem_pio2 does (the first quadrants directly, then rint(x 2/pi) and up to three rounds of
-// pi/2 split into 33-bit pieces); larger ones by an exact reduction (x 2/pi over 256 bits of
-// 2/pi, the remainder times 256 bits of pi/2, rounded to a head and a tail), where fdlibm's
-// Payne-Hanek routine is expected to agree except in its last-bit cases. Then the kernels
+// pi/2 split into 33-bit pieces); larger ones as its rem_pio2_large does (fdlibm's
+// __kernel_rem_pio2 over 24-bit chunks of 2/pi and pi/2 from pi_bits.h). Then the kernels
 // __cos / __sin: fdlibm's degree-14 and degree-13 minimax polynomials on [-pi/4, pi/4].
 #pragma once
 #include <stdint.h>
@@ -72,142 +71,166 @@ static inline int reduce_medium(double x, uint32_t ix, double &y0, double &y1) {
     return n;
 }
 
-// ---- exact reduction past 2^20 pi/2: little-endian 64-bit limbs ----
+// ---- past 2^20 pi/2: libm's rem_pio2_large (fdlibm __kernel_rem_pio2, double: jk = 4) ----
+//
+// The reference's bits depend on this routine's own double arithmetic (its remainder's tail is
+// not always the exact remainder's rounded tail), so its steps are taken as they are: |x| as
+// 24-bit integer chunks times the 24-bit chunks of 2/pi that reach its integer part modulo 8
+// and its fraction, the fraction as 24-bit integers (one minus it past one half, more chunks
+// while its leading pieces are all zero), those times pi/2 in 24-bit pieces, summed small to
+// large into a head and a tail.
+struct LargeReduction {
+    static constexpr int JK = 4;
+    double chunk[3];  // |x| * 2^-e0 in 24-bit integer chunks
+    int nchunk, e0;
 
-// out[0 .. na + nb) = a * b
-static inline void mul_limbs(const uint64_t *a, int na, const uint64_t *b, int nb, uint64_t *out) {
-    for (int i = 0; i < na + nb; i++) out[i] = 0;
-    for (int i = 0; i < na; i++) {
-        unsigned __int128 carry = 0;
-        for (int j = 0; j < nb; j++) {
-            carry += (unsigned __int128)a[i] * b[j] + out[i + j];
-            out[i + j] = (uint64_t)carry;
-            carry >>= 64;
+    // f[k] * x summed over the chunks for fraction piece i (f indexed as the routine does)
+    static double dot(const double *x, int jx, const double *f, int i) {
+        double acc = 0.0;
+        for (int j = 0; j <= jx; j++) acc += x[j] * f[jx + i - j];
+        return acc;
+    }
+
+    int run(double &head, double &tail) const {
+        int32_t iq[20];
+        double f[20], fq[20], q[20];
+        const int jx = nchunk - 1;
+        const int jv = (e0 - 3) / 24 > 0 ? (e0 - 3) / 24 : 0;
+        int q0 = e0 - 24 * (jv + 1);
+        for (int i = 0; i <= jx + JK; i++) {
+            const int j = jv - jx + i;
+            f[i] = j < 0 ? 0.0 : (double)GR_TWO_OVER_PI_24[j];
         }
-        out[i + nb] = (uint64_t)carry;
-    }
-}
-
-static inline int bit_at(const uint64_t *v, int k) { return (int)((v[k >> 6] >> (k & 63)) & 1u); }
-
-// v >> s, keeping nout limbs
-static inline void shr_limbs(const uint64_t *v, int nv, int s, uint64_t *out, int nout) {
-    for (int i = 0; i < nout; i++) {
-        const int lo = i + (s >> 6), sh = s & 63;
-        uint64_t w = lo < nv ? v[lo] >> sh : 0;
-        if (sh && lo + 1 < nv) w |= v[lo + 1] << (64 - sh);
-        out[i] = w;
-    }
-}
-
-static inline int top_bit(const uint64_t *v, int nv) {
-    for (int i = nv - 1; i >= 0; i--)
-        if (v[i]) return 64 * i + 63 - __builtin_clzll(v[i]);
-    return -1;
-}
-
-// The double nearest v * 2^scale (v < 2^448), ties to even; *mant_out, *shift_out give it
-// back as mant << shift in v's units.
-static inline double nearest(const uint64_t *v, int nv, int scale, uint64_t *mant_out,
-                             int *shift_out) {
-    const int t = top_bit(v, nv);
-    if (t < 0) {
-        *mant_out = 0;
-        *shift_out = 0;
-        return 0.0;
-    }
-    int shift = t > 52 ? t - 52 : 0;
-    uint64_t m[1];
-    shr_limbs(v, nv, shift, m, 1);
-    uint64_t mant = m[0] & ((1ull << 53) - 1);
-    if (shift > 0 && bit_at(v, shift - 1)) {
-        const int below = shift - 1;  // bits [0, below) decide a tie
-        bool sticky = false;
-        for (int i = 0; i < (below >> 6) && !sticky; i++) sticky = v[i] != 0;
-        if (!sticky && (below & 63)) sticky = (v[below >> 6] & ((1ull << (below & 63)) - 1)) != 0;
-        if (sticky || (mant & 1u)) mant++;
-        if (mant >> 53) {
-            mant >>= 1;
-            shift++;
+        for (int i = 0; i <= JK; i++) q[i] = dot(chunk, jx, f, i);
+        int jz = JK, n = 0, ih = 0;
+        double z = 0.0;
+        for (bool again = true; again;) {
+            again = false;
+            // q[jz] .. q[0] as 24-bit integers, least significant first; z the integer part
+            z = q[jz];
+            for (int i = 0, j = jz; j > 0; i++, j--) {
+                const double hi = (double)(int32_t)(0x1p-24 * z);
+                iq[i] = (int32_t)(z - 0x1p24 * hi);
+                z = q[j - 1] + hi;
+            }
+            z = __builtin_scalbn(z, q0);
+            z -= 8.0 * __builtin_floor(z * 0.125);
+            n = (int32_t)z;
+            z -= (double)n;
+            ih = 0;
+            if (q0 > 0) {
+                const int32_t carry_n = iq[jz - 1] >> (24 - q0);
+                n += carry_n;
+                iq[jz - 1] -= carry_n << (24 - q0);
+                ih = iq[jz - 1] >> (23 - q0);
+            } else if (q0 == 0) {
+                ih = iq[jz - 1] >> 23;
+            } else if (z >= 0.5) {
+                ih = 2;
+            }
+            if (ih > 0) {
+                n += 1;
+                bool borrowed = false;
+                for (int i = 0; i < jz; i++) {
+                    const int32_t v = iq[i];
+                    if (!borrowed) {
+                        if (v != 0) {
+                            borrowed = true;
+                            iq[i] = 0x1000000 - v;
+                        }
+                    } else {
+                        iq[i] = 0xffffff - v;
+                    }
+                }
+                if (q0 == 1) iq[jz - 1] &= 0x7fffff;
+                if (q0 == 2) iq[jz - 1] &= 0x3fffff;
+                if (ih == 2) {
+                    z = 1.0 - z;
+                    if (borrowed) z -= __builtin_scalbn(1.0, q0);
+                }
+            }
+            if (z == 0.0) {
+                int32_t any = 0;
+                for (int i = jz - 1; i >= JK; i--) any |= iq[i];
+                if (any == 0) {
+                    int more = 1;
+                    while (iq[JK - more] == 0) more++;
+                    for (int i = jz + 1; i <= jz + more; i++) {
+                        f[jx + i] = (double)GR_TWO_OVER_PI_24[jv + i];
+                        q[i] = dot(chunk, jx, f, i);
+                    }
+                    jz += more;
+                    again = true;
+                }
+            }
         }
+        if (z == 0.0) {
+            jz -= 1;
+            q0 -= 24;
+            while (iq[jz] == 0) {
+                jz--;
+                q0 -= 24;
+            }
+        } else {
+            z = __builtin_scalbn(z, -q0);
+            if (z >= 0x1p24) {
+                const double hi = (double)(int32_t)(0x1p-24 * z);
+                iq[jz] = (int32_t)(z - 0x1p24 * hi);
+                jz += 1;
+                q0 += 24;
+                iq[jz] = (int32_t)hi;
+            } else {
+                iq[jz] = (int32_t)z;
+            }
+        }
+        double w = __builtin_scalbn(1.0, q0);
+        for (int i = jz; i >= 0; i--) {
+            q[i] = w * (double)iq[i];
+            w *= 0x1p-24;
+        }
+        for (int i = jz; i >= 0; i--) {
+            double acc = 0.0;
+            for (int k = 0; k <= JK && k <= jz - i; k++) acc += GR_PIO2_24[k] * q[i + k];
+            fq[jz - i] = acc;
+        }
+        double sum = 0.0;
+        for (int i = jz; i >= 0; i--) sum += fq[i];
+        head = ih == 0 ? sum : -sum;
+        double rest = fq[0] - sum;
+        for (int i = 1; i <= jz; i++) rest += fq[i];
+        tail = ih == 0 ? rest : -rest;
+        return n & 7;
     }
-    *mant_out = mant;
-    *shift_out = shift;
-    return __builtin_ldexp((double)mant, shift + scale);
-}
+};
 
-// |x| in [2^20 pi/2, 2^60): n and y0 + y1 = x - n pi/2, the remainder computed exactly from
-// 256 bits of each constant and rounded to a head and a tail.
+// rem_pio2's large case: the significand of |x| in 24-bit chunks (trailing zero chunks
+// dropped), the remainder and the quadrant signed as x
 static inline int reduce_large(double x, double &y0, double &y1) {
     const uint64_t u = bits_of(x);
-    const bool neg = u >> 63;
-    const int e = (int)((u >> 52) & 0x7ff) - 1075;        // |x| = mant * 2^e
-    const uint64_t mant[1] = {(u & ((1ull << 52) - 1)) | (1ull << 52)};
-    uint64_t t2[4], p[5];
-    for (int i = 0; i < 4; i++) t2[i] = GR_TWO_OVER_PI_256[3 - i];
-    mul_limbs(mant, 1, t2, 4, p);                          // |x| 2/pi = p * 2^(e - 256)
-    const int s = 256 - e;                                 // fraction bits (e < 0 here)
-    uint64_t ip[1];
-    shr_limbs(p, 5, s, ip, 1);
-    unsigned q = (unsigned)(ip[0] & 3u);
-    // the top 192 bits of the fraction: g = G * 2^-192
-    uint64_t g[3];
-    shr_limbs(p, 5, s - 192, g, 3);
-    bool flip = false;
-    if (g[2] >> 63) {                                      // g >= 1/2: the next quadrant
-        q++;
-        flip = true;
-        unsigned __int128 borrow = 0;                      // G = 2^192 - G
-        for (int i = 0; i < 3; i++) {
-            const unsigned __int128 d = (unsigned __int128)0 - g[i] - borrow;
-            g[i] = (uint64_t)d;
-            borrow = (d >> 64) ? 1 : 0;
-        }
+    const uint32_t ix = (uint32_t)(u >> 32) & 0x7fffffffu;
+    const uint64_t scaled = (u & ((uint64_t)-1 >> 12)) | ((uint64_t)(0x3ff + 23) << 52);
+    double z;
+    memcpy(&z, &scaled, 8);
+    LargeReduction r;
+    for (int i = 0; i < 2; i++) {
+        r.chunk[i] = (double)(int32_t)z;
+        z = (z - r.chunk[i]) * 0x1p24;
     }
-    uint64_t pio2[4], r[7];
-    for (int i = 0; i < 4; i++) pio2[i] = GR_PIO2_256[3 - i];
-    mul_limbs(g, 3, pio2, 4, r);                           // remainder = r * 2^-447
-    uint64_t m0;
-    int sh0;
-    double h = nearest(r, 7, -447, &m0, &sh0);
-    // tail = r - m0 << sh0, signed
-    uint64_t y[7], d[7];
-    for (int i = 0; i < 7; i++) y[i] = 0;
-    {
-        const int li = sh0 >> 6, b = sh0 & 63;
-        y[li] = m0 << b;
-        if (b && li + 1 < 7) y[li + 1] = m0 >> (64 - b);
-    }
-    bool tail_neg = false;
-    {
-        // d = r - y, or y - r when y > r
-        int cmp = 0;
-        for (int i = 6; i >= 0 && !cmp; i--) cmp = r[i] > y[i] ? 1 : (r[i] < y[i] ? -1 : 0);
-        const uint64_t *a = cmp >= 0 ? r : y, *b = cmp >= 0 ? y : r;
-        tail_neg = cmp < 0;
-        unsigned __int128 borrow = 0;
-        for (int i = 0; i < 7; i++) {
-            const unsigned __int128 v = (unsigned __int128)a[i] - b[i] - borrow;
-            d[i] = (uint64_t)v;
-            borrow = (v >> 64) ? 1 : 0;
-        }
-    }
-    uint64_t m1;
-    int sh1;
-    double tl = nearest(d, 7, -447, &m1, &sh1);
-    if (tail_neg) tl = -tl;
-    if (flip) {
-        h = -h;
-        tl = -tl;
-    }
-    if (neg) {
+    r.chunk[2] = z;
+    int last = 2;
+    while (last > 0 && r.chunk[last] == 0.0) last--;
+    r.nchunk = last + 1;
+    r.e0 = (int)(ix >> 20) - (0x3ff + 23);
+    double h, t;
+    const int n = r.run(h, t);
+    if (u >> 63) {
         y0 = -h;
-        y1 = -tl;
-        return -(int)q;
+        y1 = -t;
+        return -n;
     }
     y0 = h;
-    y1 = tl;
-    return (int)q;
+    y1 = t;
+    return n;
 }
 
 static inline int reduce(double x, double &y0, double &y1) {
@@ -238,7 +261,7 @@ static inline int reduce(double x, double &y0, double &y1) {
     return -(int)k;
 }
 
-// libm::cos. Arguments: finite, |x| < 2^60 (the spiral's angles stay below 2^34).
+// libm::cos of a finite x (the spiral's angles stay below 2^34).
 static inline double cos(double x) {
     const uint32_t ix = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
     if (ix <= 0x3fe921fb) return ix < 0x3e46a09e ? 1.0 : k_cos(x, 0.0);

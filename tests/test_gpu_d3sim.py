@@ -141,6 +141,20 @@ def test_the_session_starts_on_the_references_spiral(oracle, mesh):
     assert not vx.any() and not vy.any()
 
 
+def test_the_spiral_past_the_medium_reduction_is_the_oracles(oracle):
+    # from node 686 302 on, the spiral's angles pass 2^20 pi/2: the device reduces them
+    # exactly over 256 bits of 2/pi and pi/2, the oracle by libm's rem_pio2_large (24-bit
+    # chunks of 2/pi); both must give libm's bits for every node up to a million
+    n = 1_000_000
+    e = np.zeros(0, np.uint32)
+    print(f"spiral n={n}", flush=True)
+    with barnes_hut.ForceSession(Topology(n, e, e), None) as S:
+        x, y, _, _ = S.state()
+    wx, wy = oracle.golden_spiral(n)
+    assert np.isfinite(wx).all() and np.isfinite(wy).all()
+    assert same_bits(x, wx) and same_bits(y, wy)
+
+
 def test_the_link_pass_from_rest_is_the_references(oracle, mesh):
     # one tick with only the springs acting at alpha 1 (no charge, no collision reach, no
     # centring, no decay): the velocities are the fixture's link column, bit for bit

@@ -120,6 +120,22 @@ def test_a_large_mesh_layout_over_twenty_ticks(oracle):
         assert same_bits(a, b), first_diff(a, b)
 
 
+def test_the_mesh_layout_at_a_million_nodes(oracle):
+    # the reference's own timed graph (the gate model at 1M nodes, 1 549 929 raw edges, a
+    # 1024 x 1024 mesh), its spiral past the medium reduction included: four ticks
+    n = 1_000_000
+    s, t, w = oracle.gate_model(n)
+    assert len(s) == 1_549_929
+    print(f"pm layout gate model n={n} ticks=4", flush=True)
+    with barnes_hut.ForceSession(Topology(n, s, t), w, particle_mesh=True) as S:
+        S.step(4)
+        dev = S.state()
+    want = oracle.d3sim(n, s, t, w, ticks=4, mesh_side=oracle.pm_side_for(n), velocities=True)
+    for a, b in zip(dev, want):
+        assert np.isfinite(b).all()
+        assert same_bits(a, b), first_diff(a, b)
+
+
 def test_the_registry_id_and_the_session_agree(oracle):
     s, t, w = oracle.gate_model(500)
     top = Topology(500, s, t)

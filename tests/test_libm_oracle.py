@@ -111,6 +111,72 @@ def test_libm_agrees_with_the_exact_cosine_to_one_ulp(oracle):
             assert abs(Fraction(got) - want) <= Fraction(math.ulp(float(want))), a
 
 
+def _ipio2_tables():
+    src = open(os.path.join(HERE, "..", "oracle", "ipio2.h")).read()
+    body = src.split("IPIO2[66] = {")[1].split("};")[0]
+    ipio2 = [int(v, 16) for v in body.replace("\n", " ").split(",") if v.strip()]
+    body = src.split("PIO2_24[8] = {")[1].split("};")[0]
+    pio2 = [float.fromhex(v.strip()) for v in body.split(",") if v.strip()]
+    return ipio2, pio2
+
+
+def test_the_large_reduction_tables_follow_from_pi():
+    # 2/pi in 24-bit chunks and pi/2 in 24-bit pieces, against pi computed exactly, and their
+    # leading entries against the values fdlibm publishes for __kernel_rem_pio2
+    ipio2, pio2 = _ipio2_tables()
+    assert len(ipio2) == 66 and len(pio2) == 8
+    pi = _pi(1800)
+    for k, v in enumerate(ipio2):
+        assert v == int((2 / pi) * 2 ** (24 * (k + 1))) % 2 ** 24, k
+    rest = pi / 2
+    for k, v in enumerate(pio2):
+        scale = Fraction(2) ** (23 + 24 * k)
+        assert Fraction(v) == Fraction(int(rest * scale), 1) / scale, k
+        rest -= Fraction(v)
+    assert ipio2[:4] == [0xA2F983, 0x6E4E44, 0x1529FC, 0x2757D1]
+    assert pio2[:4] == [1.57079625129699707031e+00, 7.54978941586159635335e-08,
+                        5.39030252995776476554e-15, 3.28200341580791294123e-22]
+
+
+def _exact_cos_sin(a, pi):
+    """cos and sin of the double a as Fractions: a reduced exactly, then Taylor series."""
+    q = round(Fraction(a) / (pi / 2))
+    r = Fraction(a) - q * (pi / 2)
+    c, s, term = Fraction(0), Fraction(0), Fraction(1)
+    for k in range(40):
+        if k % 2 == 0:
+            c += term * (-1) ** (k // 2)
+        else:
+            s += term * (-1) ** (k // 2)
+        term = term * r / (k + 1)
+    return [c, -s, -c, s][q % 4], [s, c, -s, -c][q % 4]
+
+
+def test_large_arguments_are_within_one_ulp_of_exact(oracle):
+    # past 2^20 pi/2 (the golden spiral from node 686 302 on) the reduction is rem_pio2_large's;
+    # fdlibm's cos/sin stay within one ulp of the exact values there too
+    pi = _pi(600)
+    rng = np.random.default_rng(11)
+    nodes = [686_302, 686_303, 999_999, 4_000_000, 2 ** 31 + 5] + list(
+        rng.integers(686_302, 1 << 32, 20))
+    args = [float(i) * 2.399963229728653 for i in nodes]
+    args += [2.0 ** 20 * math.pi / 2 * 1.0000001, -3.0e9, 1.0e15, 2.0 ** 60 + 2.0 ** 10]
+    for a in args:
+        cos_exact, sin_exact = _exact_cos_sin(a, pi)
+        for got, want in ((oracle.libm_cos(a), cos_exact), (oracle.libm_sin(a), sin_exact)):
+            assert math.isfinite(got)
+            assert abs(Fraction(got) - want) <= Fraction(math.ulp(float(want))), a
+
+
+def test_the_spiral_is_finite_past_the_medium_range(oracle):
+    x, y = oracle.golden_spiral(686_400)
+    assert np.isfinite(x).all() and np.isfinite(y).all()
+    i = np.arange(686_290, 686_400, dtype=np.float64)
+    r = 12.0 * np.sqrt(i + 1.0)
+    # the radius is the reference's 12 sqrt(i + 1) to rounding, on either side of the switch
+    assert np.allclose(np.hypot(x[686_290:], y[686_290:]), r, rtol=1e-14, atol=0)
+
+
 def test_the_gate_model_collapses_to_the_references_edges(oracle, mesh):
     s, t, w = oracle.gate_model(1000)
     seen, lo, hi, st = set(), [], [], []

@@ -73,6 +73,18 @@ def collide_traffic(n: int):
     return None
 
 
+def profile_figure(name: str, n: int, profile: str, at_n: int):
+    """A `name = value` figure from a committed rocprofv3 summary under profiles/ (HBM bytes
+    per launch from FETCH_SIZE/WRITE_SIZE), or None for other sizes."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if n != at_n or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(name + " ="):
+            return int(line.split("=")[1])
+    return None
+
+
 def cpu_info() -> dict:
     """The host CPU as the baseline states it: model, logical cores the process may use."""
     model = "unknown"
@@ -646,6 +658,21 @@ def bench_pmsim(args):
         print(f"pmsim n={n}: 112 ticks {ms:.2f} ms", file=sys.stderr, flush=True)
     ms = float(np.median(runs))
     side = BH.pm_side_for(n)
+    # the tick's longest kernel, the node-order centring sums (two dependent chains of n FP64
+    # adds, on a side stream beside link and charge; the tick waits for them), and the main
+    # stream's longest, the hashed grid's pushes, timed again after the layout's last tick.
+    # Algorithmic bytes: the sums read x and y (16 B per node); the pushes read a node's
+    # projected position (16 B), its grid slot (4 B) and its three rows' bucket bounds (24 B)
+    # and write its delta (16 B): 60 B per node
+    with BH.ForceSession(top, w) as S:
+        S.with_particle_mesh()
+        S.step(BH.TICKS)
+        ms_center, ms_gather, _, buckets = S.time_walks(10)
+    cb, gb = 16 * n, 60 * n
+    ach_c = cb / (ms_center * 1e-3) / 1e9
+    ach_g = gb / (ms_gather * 1e-3) / 1e9
+    print(f"pmsim kernels: centring sums {ms_center:.3f} ms, hashed-grid pushes "
+          f"{ms_gather:.3f} ms ({buckets} buckets)", file=sys.stderr, flush=True)
     # the reference's stage time on this graph: 68 483.65 ms, median of 3, one thread
     # (docs/measurements/perf-p2-pm.md:39)
     published_ms = 68483.65 if (n == 1_000_000 and len(s) == 1_549_929) else None
@@ -661,6 +688,21 @@ def bench_pmsim(args):
                                f"start, ForceParams defaults, {side}x{side} mesh, 112 ticks",
                    "nodes": n, "raw_edges": int(len(s)), "ticks": BH.TICKS, "mesh_side": side},
         "ms_per_tick": round(ms / BH.TICKS, 4),
+        "kernel_ms": {"centring_sums": round(ms_center, 4), "grid_pushes": round(ms_gather, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(ach_c, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach_c / HBM_PEAK_GBS, 6),
+                     "traffic": profile_figure("traffic_bytes_center_sum_1M", n,
+                                               "pmsim_1M_kernel_stats.md", 1_000_000),
+                     "kernel": "k_sim_center_sum", "algorithmic_bytes": cb,
+                     "note": "two chains of n dependent FP64 adds in node order (the "
+                             "reference's rounding): bound by the add latency, not by bytes"},
+        "roofline_grid_pushes": {"bound": "hbm", "achieved": round(ach_g, 2),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(ach_g / HBM_PEAK_GBS, 5),
+                                 "traffic": profile_figure("traffic_bytes_grid_pushes_1M", n,
+                                                           "pmsim_1M_kernel_stats.md",
+                                                           1_000_000),
+                                 "kernel": "k_pmd3_gather", "algorithmic_bytes": gb},
         "finite": bool(np.isfinite(geo.x).all() and np.isfinite(geo.y).all()),
     }
     if published_ms:

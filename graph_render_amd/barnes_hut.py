@@ -114,7 +114,9 @@ class ForceSession:
 
     def time_walks(self, reps: int = 10):
         """The last tick's many-body and collide walks timed again on their own trees:
-        (many-body ms, collide ms, many-body cells, collide cells)."""
+        (many-body ms, collide ms, many-body cells, collide cells). On a mesh session: (the
+        node-order centring sums' ms, the hashed grid's pushes' ms, mesh cells, grid
+        buckets)."""
         ms = np.zeros(2)
         cells = np.zeros(2, np.uint32)
         _check(A.lib().gr_d3_session_time(self._h, int(reps), ms.ctypes.data, cells.ctypes.data))

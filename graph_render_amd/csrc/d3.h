@@ -141,5 +141,7 @@ void pmd3_spectrum(const PmD3Dev &m, double h, int reach, hipStream_t stream);
 void pmd3_solve_read(const D3SimDev &s, const PmD3Dev &m, uint32_t cells, double strength);
 // The hashed grid over s.px, s.py (m.order rebuilt) and every node's pushes into s.dl.
 void pmd3_collide(const D3SimDev &s, const PmD3Dev &m, uint32_t tick);
+// collide.rs's pushes alone over the grid order the last pmd3_collide sorted (writes s.dl)
+void pmd3_collide_gather(const D3SimDev &s, const PmD3Dev &m, uint32_t tick);
 
 }  // namespace gr

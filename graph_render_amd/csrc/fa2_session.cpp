@@ -398,17 +398,17 @@ struct gr_fa2_session {
     }
 };
 
-// The median of three timings of `reps` launches (HIP events on the session's stream).
+// The median of three timings of `reps` launches (HIP events on the stream they go to).
 template <typename F>
-static gr_status time_median(gr_fa2_session *s, uint32_t reps, F launch, double *ms) {
+static gr_status time_median(hipStream_t stream, uint32_t reps, F launch, double *ms) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     double t[3];
     for (int k = 0; k < 3; k++) {
-        HIP_TRY(hipEventRecord(e0, s->d.stream));
+        HIP_TRY(hipEventRecord(e0, stream));
         for (uint32_t q = 0; q < reps; q++) launch();
-        HIP_TRY(hipEventRecord(e1, s->d.stream));
+        HIP_TRY(hipEventRecord(e1, stream));
         HIP_TRY(hipEventSynchronize(e1));
         float f = 0.f;
         HIP_TRY(hipEventElapsedTime(&f, e0, e1));
@@ -420,6 +420,12 @@ static gr_status time_median(gr_fa2_session *s, uint32_t reps, F launch, double 
     *ms = t[1];
     HIP_TRY(hipGetLastError());
     return GR_OK;
+}
+
+// ... on an FA2 session's stream
+template <typename F>
+static gr_status time_median(gr_fa2_session *s, uint32_t reps, F launch, double *ms) {
+    return time_median(s->d.stream, reps, launch, ms);
 }
 
 // The simple graph (layout/force/mod.rs:70-180) for another translation unit of this
@@ -2144,8 +2150,23 @@ gr_status gr_d3_session_time(gr_d3_session *h, uint32_t reps, double *ms_out,
     if (!ms_out || !cells_out) return fail(GR_ERR_PARAM, "out", "two times and two counts");
     D3Sim &S = h->S;
     if (!h->tick || !S.sd.n) return fail(GR_ERR_PARAM, "session", "a session stepped at least once");
-    if (S.mesh) return fail(GR_ERR_UNSUPPORTED, "session", "a session on the quadtree passes");
     if (reps == 0) reps = 1;
+    gr_status r;
+    if (S.mesh) {
+        // the centring chain (it rewrites only the shift the next tick recomputes) and the
+        // hashed grid's pushes over the last tick's grid order (scratch deltas only)
+        if (S.sd.reach2 == 0.0)
+            return fail(GR_ERR_UNSUPPORTED, "session", "a mesh session with a collide reach");
+        if ((r = time_median(S.sd.stream, reps, [&] { gr::d3sim_center_sum(S.sd, S.sd.stream); },
+                             &ms_out[0])) ||
+            (r = time_median(S.sd.stream, reps,
+                             [&] { gr::pmd3_collide_gather(S.sd, S.pm, h->tick - 1); },
+                             &ms_out[1])))
+            return r;
+        cells_out[0] = S.pm.side * S.pm.side;
+        cells_out[1] = 1u << S.pm.bucket_bits;
+        return ok();
+    }
     gr::D3CollideTree t;
     t.C = S.collide.g.C;
     t.n = S.sd.n;
@@ -2154,7 +2175,6 @@ gr_status gr_d3_session_time(gr_d3_session *h, uint32_t reps, double *ms_out,
     t.order = S.collide.g.order;
     t.sx = S.collide.g.sx;
     t.sy = S.collide.g.sy;
-    gr_status r;
     // both walks write only their scratch deltas: the state is left as the last tick left it
     if ((r = time_median(S.cs, reps, [&] { gr::d3_launch_walk(S.charge.g); }, &ms_out[0])) ||
         (r = time_median(S.cs, reps, [&] { gr::d3sim_collide(S.sd, t, h->tick - 1); },

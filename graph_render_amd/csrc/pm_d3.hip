@@ -145,15 +145,33 @@ __global__ __launch_bounds__(GR_BLOCK) void k_pmd3_stencil(const D3SimDev s, con
     m.fxy[2 * (size_t)k + 1] = fy;
 }
 
-// start[t] = the first sorted entry with key >= t, for t in [0, nkeys]: entry j writes the
-// starts of the keys in (key[j - 1], key[j]], and one more thread the ones past the last key
+// start[t] = the first sorted entry with key >= t, for t in [0, nkeys], one thread per key:
+// two lanes bracket the workgroup's keys [t0, t0 + GR_BLOCK) among the entries, then each
+// thread searches its key within that bracket. Empty stretches of keys (mesh cells outside
+// the layout, unused hash buckets) cost no more than full ones.
+__device__ __forceinline__ uint32_t lower_bound_u64(const uint64_t *keys, uint32_t lo,
+                                                    uint32_t hi, uint64_t want) {
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < want)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(GR_BLOCK) void k_pmd3_starts(const uint64_t *keys, uint32_t n,
                                                           uint32_t nkeys, uint32_t *start) {
-    const uint32_t j = blockIdx.x * GR_BLOCK + threadIdx.x;
-    if (j > n) return;
-    const uint32_t lo = j == 0 ? 0u : (uint32_t)keys[j - 1] + 1u;
-    const uint32_t hi = j == n ? nkeys : (uint32_t)keys[j];
-    for (uint32_t t = lo; t <= hi && t <= nkeys; t++) start[t] = j;
+    __shared__ uint32_t bracket[2];
+    const uint64_t t0 = (uint64_t)blockIdx.x * GR_BLOCK;
+    if (t0 > nkeys) return;  // block-uniform
+    if (threadIdx.x < 2)
+        bracket[threadIdx.x] = lower_bound_u64(keys, 0, n, t0 + (threadIdx.x ? GR_BLOCK : 0));
+    __syncthreads();
+    const uint64_t t = t0 + threadIdx.x;
+    if (t > nkeys) return;
+    start[t] = lower_bound_u64(keys, bracket[0], bracket[1], t);
 }
 
 // deposit.rs Deposit: cell t sums, from zero, the weight each node whose stencil covers it
@@ -452,8 +470,8 @@ void pmd3_deposit(const D3SimDev &s, const PmD3Dev &m, uint32_t cells) {
     const bool odd = radix_digits(bits) & 1;
     const uint64_t *sk = odd ? m.keys_alt : m.keys;
     const uint32_t *sv = odd ? m.vals_alt : m.vals;
-    hipLaunchKernelGGL(k_pmd3_starts, dim3(gr_blocks((uint64_t)n + 1, GR_BLOCK)), dim3(GR_BLOCK),
-                       0, s.stream, sk, n, nkeys, m.cstart);
+    hipLaunchKernelGGL(k_pmd3_starts, dim3(gr_blocks((uint64_t)nkeys + 1, GR_BLOCK)),
+                       dim3(GR_BLOCK), 0, s.stream, sk, n, nkeys, m.cstart);
     hipLaunchKernelGGL(k_pmd3_deposit, dim3(gr_blocks((uint64_t)cells * m.side, GR_BLOCK)),
                        dim3(GR_BLOCK), 0, s.stream, m, sv, cells);
 }
@@ -497,12 +515,16 @@ void pmd3_collide(const D3SimDev &s, const PmD3Dev &m, uint32_t tick) {
     const bool odd = radix_digits(bits) & 1;
     const uint64_t *sk = odd ? m.keys_alt : m.keys;
     const uint32_t *sv = odd ? m.vals_alt : m.vals;
-    hipLaunchKernelGGL(k_pmd3_starts, dim3(gr_blocks((uint64_t)n + 1, GR_BLOCK)), dim3(GR_BLOCK),
-                       0, s.stream, sk, n, 1u << m.bucket_bits, m.bstart);
+    hipLaunchKernelGGL(k_pmd3_starts, dim3(gr_blocks(((uint64_t)1 << m.bucket_bits) + 1, GR_BLOCK)),
+                       dim3(GR_BLOCK), 0, s.stream, sk, n, 1u << m.bucket_bits, m.bstart);
     hipLaunchKernelGGL(k_pmd3_order, dim3(gr_blocks(n, GR_BLOCK)), dim3(GR_BLOCK), 0, s.stream,
                        s, m, sv);
-    hipLaunchKernelGGL(k_pmd3_gather, dim3(gr_blocks(n, GR_BLOCK)), dim3(GR_BLOCK), 0, s.stream,
-                       s, m, size, tick);
+    pmd3_collide_gather(s, m, tick);
+}
+
+void pmd3_collide_gather(const D3SimDev &s, const PmD3Dev &m, uint32_t tick) {
+    hipLaunchKernelGGL(k_pmd3_gather, dim3(gr_blocks(s.n, GR_BLOCK)), dim3(GR_BLOCK), 0, s.stream,
+                       s, m, sqrt(s.reach2), tick);
 }
 
 }  // namespace gr

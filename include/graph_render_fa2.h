@@ -393,8 +393,11 @@ gr_status gr_d3_session_state(gr_d3_session *s, double *x, double *y, double *vx
 gr_status gr_d3_session_with_particle_mesh(gr_d3_session *s);
 /* Measurement, no reference counterpart: the last tick's two walks timed again on their own
  * trees (HIP events; medians of three timings of `reps` launches): ms_out[0] the many-body
- * walk, ms_out[1] the collide walk; cells_out[0..1] the two trees' cell counts. The state is
- * left as the last tick left it. Refused before the first step and on a mesh session. */
+ * walk, ms_out[1] the collide walk; cells_out[0..1] the two trees' cell counts. On a mesh
+ * session: ms_out[0] the node-order centring sums, ms_out[1] the hashed grid's pushes over the
+ * last tick's grid order; cells_out[0] the mesh's cells, cells_out[1] the grid's buckets
+ * (refused at zero collide reach). The state is left as the last tick left it. Refused before
+ * the first step. */
 gr_status gr_d3_session_time(gr_d3_session *s, uint32_t reps, double *ms_out,
                              uint32_t *cells_out);
 /* alpha() (session.rs:238); NaN for a NULL handle. */

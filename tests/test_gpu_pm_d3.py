@@ -156,3 +156,24 @@ def test_without_charge_or_collision_the_mesh_layout_is_the_tree_layout(oracle):
     a = pm_device(800, s, t, w, params=p, ticks=40)
     b = barnes_hut.BarnesHut.run(Topology(800, s, t), w, params=p, ticks=40)
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_timing_the_mesh_passes_leaves_the_state(oracle):
+    # the centring sums and the hashed grid's pushes timed again on a stepped mesh session:
+    # the state is the last tick's, and the next ticks are an untimed session's
+    s, t, w = oracle.gate_model(3000)
+    top = Topology(3000, s, t)
+    with barnes_hut.ForceSession(top, w, particle_mesh=True) as S:
+        S.step(5)
+        before = S.state()
+        mc, mg, cells, buckets = S.time_walks(3)
+        assert mc > 0 and mg > 0
+        assert cells == oracle.pm_side_for(3000) ** 2 and buckets == 8192  # next_pow2(2n)
+        for a, b in zip(before, S.state()):
+            assert same_bits(a, b)
+        S.step(3)
+        after = S.state()
+    with barnes_hut.ForceSession(top, w, particle_mesh=True) as S:
+        S.step(8)
+        for a, b in zip(after, S.state()):
+            assert same_bits(a, b), first_diff(a, b)

@@ -196,9 +196,12 @@ gr_status gr_fa2_session_record(gr_fa2_session *s, uint32_t iters, uint32_t chec
 gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, double *weight_bar,
                                  double *param_bar2, double *start_bar);
 
-/* The Barnes-Hut walk on the current tree, measured (no reference counterpart): runs a
- * counting variant of the walk once and writes out4 = {wave scan steps, lanes acting summed
- * over the steps, cells accepted, leaf pairs taken}; accepted + pairs are the interactions. */
+/* The Barnes-Hut walk measured (no reference counterpart), on the tree of the last force
+ * evaluation or iteration (as gr_fa2_session_tree; the tree of the current positions after
+ * gr_fa2_session_forces): runs a counting variant of the walk once and writes out4 = {wave
+ * scan steps, lanes acting summed over the steps, cells accepted, leaf pairs taken}; accepted
+ * + pairs are the interactions. Each wave scans 64 consecutive tree positions and visits the
+ * union of their reference walks, so in FP64 these are the oracle's gro_walk_counts. */
 gr_status gr_fa2_session_walk_stats(gr_fa2_session *s, uint64_t *out4);
 
 /* ---- schedules and the autotuner (no reference counterpart) ----

@@ -1152,6 +1152,58 @@ static void repulsion_tree(gro_fa2 *s) {
     for (uint32_t a = 0; a < d; a++) free(cols[a]);
 }
 
+/* The walk of barnes_hut.rs:112-159 replayed per tree position, counted as a wave-uniform scan
+ * over groups of `group` consecutive positions visits it: out[0] the steps, the size of the
+ * union of the group's visit sets summed over groups; out[1] the visits (each position's
+ * cells); out[2] the cells accepted; out[3] the leaf pairs taken (a leaf's other points).
+ * Points given in node order; the tree is the reference's over them. */
+void gro_walk_counts(uint32_t n, uint32_t dim, const double *xs, const double *ys,
+                     const double *zs, const double *mass, double theta2, uint32_t group,
+                     uint64_t *out) {
+    const uint32_t d = dim, w = 1 + d;
+    const double *cols[MAXD] = {xs, ys, zs};
+    for (int q = 0; q < 4; q++) out[q] = 0;
+    if (n == 0 || group == 0) return;
+    gro_tree *t = gro_tree_build(n, d, xs, ys, d == 3 ? zs : NULL);
+    double *bodies = (double *)malloc((t->ncells ? t->ncells : 1) * w * sizeof(double));
+    aggregate(t, mass, cols, bodies);
+    uint32_t *mark = (uint32_t *)malloc((t->ncells ? t->ncells : 1) * sizeof(uint32_t));
+    for (size_t c = 0; c < t->ncells; c++) mark[c] = UINT32_MAX;
+    for (uint32_t p = 0; p < n; p++) {
+        const uint32_t i = t->order[p], g = p / group;
+        double xi[MAXD];
+        for (uint32_t a = 0; a < d; a++) xi[a] = cols[a][i];
+        size_t k = 0;
+        while (k < t->ncells) {
+            out[1]++;
+            if (mark[k] != g) {
+                mark[k] = g;
+                out[0]++;
+            }
+            int holds = t->start[k] <= p && p < t->end[k];
+            const double *b = &bodies[k * w];
+            double dx[MAXD] = {0.0, 0.0, 0.0};
+            for (uint32_t a = 0; a < d; a++) dx[a] = xi[a] - b[1 + a];
+            double d2 = dx[0] * dx[0] + dx[1] * dx[1];
+            if (d == 3) d2 = d2 + dx[2] * dx[2];
+            double wd = t->cbox[k].hi[0] - t->cbox[k].lo[0];
+            if (!holds && d2 > 0.0 && wd * wd < theta2 * d2) {
+                out[2]++;
+                k = t->skip[k];
+            } else if (t->skip[k] == k + 1) {
+                for (uint32_t q = t->start[k]; q < t->end[k]; q++)
+                    if (t->order[q] != i) out[3]++;
+                k += 1;
+            } else {
+                k += 1;
+            }
+        }
+    }
+    free(mark);
+    free(bodies);
+    gro_tree_free(t);
+}
+
 /* state.rs:208-251: sequential centroid, then the unit vector scaled by gravity*mass. */
 static void gravity_pull(gro_fa2 *s) {
     uint32_t n = s->n, d = s->dim;

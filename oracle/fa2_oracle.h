@@ -81,6 +81,12 @@ void gro_tree_order(const gro_tree *t, uint32_t *order);
 /* each cell's node id: the shape slot the insertion created it as (quadtree.rs:64-88) */
 void gro_tree_node_ids(const gro_tree *t, uint32_t *out);
 /* bodies of the tree for per-point masses: mass then dim centre coordinates per cell */
+/* The reference walk replayed per tree position and counted as a scan over groups of `group`
+ * consecutive positions: steps (the union of each group's visit sets), visits, accepted
+ * cells, leaf pairs (out[0..3]). */
+void gro_walk_counts(uint32_t n, uint32_t dim, const double *xs, const double *ys,
+                     const double *zs, const double *mass, double theta2, uint32_t group,
+                     uint64_t *out);
 void gro_tree_bodies(const gro_tree *t, const double *mass, const double *xs, const double *ys,
                      const double *zs, double *out);
 

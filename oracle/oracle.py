@@ -73,6 +73,8 @@ def lib():
         L.gro_tree_order.argtypes = [C.c_void_p, u32p]
         L.gro_tree_bodies.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p]
         L.gro_tree_node_ids.argtypes = [C.c_void_p, u32p]
+        L.gro_walk_counts.argtypes = [C.c_uint32, C.c_uint32, f64p, f64p, C.c_void_p, f64p,
+                                      C.c_double, C.c_uint32, C.POINTER(C.c_uint64)]
         L.gro_gate_model.restype = C.c_size_t
         L.gro_gate_model.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_size_t]
         L.gro_d3sim_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
@@ -273,6 +275,20 @@ class Tree:
         if h:
             lib().gro_tree_free(h)
             self._h = None
+
+
+def walk_counts(pts, mass, theta2: float = THETA2_REFERENCE, group: int = 64) -> dict:
+    """The reference walk replayed per tree position, counted as a wave-uniform scan over
+    groups of `group` consecutive positions: steps (the union of each group's visit sets,
+    summed), visits, cells accepted, leaf pairs."""
+    pts = np.asarray(pts, np.float64)
+    n, dim = pts.shape
+    cols = [np.ascontiguousarray(pts[:, a]) for a in range(dim)]
+    m = np.ascontiguousarray(mass, np.float64)
+    out = (C.c_uint64 * 4)()
+    lib().gro_walk_counts(n, dim, cols[0], cols[1], cols[2].ctypes.data if dim == 3 else None,
+                          m, float(theta2), int(group), out)
+    return dict(zip(("steps", "acting_lanes", "accepted", "pairs"), (int(v) for v in out)))
 
 
 def fft2(z: np.ndarray, sign: int = -1) -> np.ndarray:

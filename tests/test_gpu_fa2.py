@@ -482,3 +482,21 @@ def test_tree_export_refuses_a_short_buffer():
         code = A.lib().gr_fa2_session_tree(sess._h, C.byref(nc), nc.value - 1, None,
                                            short.ctypes.data, None, None, None, None)
         assert code == A.GR_ERR_CAPACITY
+
+
+@pytest.mark.parametrize("theta", [0.8, 1.2])
+def test_the_walks_scan_is_the_union_of_its_lanes_walks(oracle, theta):
+    # the wave-uniform scan visits each cell its 64 lanes' reference walks visit, once: its
+    # step, acting-lane, accepted-cell and leaf-pair counts equal the oracle's replay of the
+    # reference walk per tree position (FP64: every decision is the reference's)
+    n = 60000
+    s, t = bh_graph(n)
+    print(f"walk counts n={n} theta={theta}", flush=True)
+    with session(n, s, t, theta=theta) as sess:
+        sess.step(3)
+        pos = sess.positions()
+        sess.forces(7)  # the tree of these positions (the last iteration's predates its step)
+        got = sess.walk_stats()
+    mass = oracle.Fa2(n, s, t).mass()
+    want = oracle.walk_counts(pos, mass, theta * theta, 64)
+    assert got == want

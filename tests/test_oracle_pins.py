@@ -188,3 +188,61 @@ def test_rung_ladder_is_the_c_library_exp2(oracle):
         assert oracle.lib().gro_pm_h(q) == np.exp2(q / 4.0), q
         # and each quarter step is what it claims: four of them make one octave
         assert math.isclose(oracle.lib().gro_pm_h(q) ** 4, 2.0 ** q, rel_tol=1e-15)
+
+
+def _walk_brute(pts, mass, theta2, group):
+    """The reference walk in plain Python over the oracle's tree, each visit recorded."""
+    from oracle import oracle as O
+    t = O.Tree(pts)
+    bodies = t.bodies(mass)
+    n = len(pts)
+    visits = [[] for _ in range(n)]
+    acc = pairs = 0
+    for p in range(n):
+        i = int(t.order[p])
+        k = 0
+        while k < t.ncells:
+            visits[p].append(k)
+            holds = t.start[k] <= p < t.end[k]
+            dx = pts[i] - bodies[k, 1:]
+            d2 = float(dx[0] * dx[0] + dx[1] * dx[1])
+            w = t.bounds[k, 2] - t.bounds[k, 0]
+            if not holds and d2 > 0 and w * w < theta2 * d2:
+                acc += 1
+                k = int(t.skip[k])
+            else:
+                if t.skip[k] == k + 1:
+                    pairs += sum(1 for q in range(t.start[k], t.end[k]) if t.order[q] != i)
+                k += 1
+    steps = sum(len(set().union(*visits[g:g + group])) for g in range(0, n, group))
+    return dict(steps=steps, acting_lanes=sum(map(len, visits)), accepted=acc, pairs=pairs)
+
+
+@pytest.mark.parametrize("theta2", [0.64, 1.44])
+def test_walk_counts_are_the_replayed_walks_unions(oracle, theta2):
+    # the scan of 64 consecutive tree positions visits exactly the union of their walks
+    rng = np.random.default_rng(4)
+    pts = np.concatenate([rng.normal(0, 30, (500, 2)), rng.uniform(-200, 200, (300, 2))])
+    pts[7] = pts[8]  # a coincident pair
+    mass = rng.integers(1, 6, len(pts)).astype(np.float64)
+    assert oracle.walk_counts(pts, mass, theta2, 64) == _walk_brute(pts, mass, theta2, 64)
+
+
+def test_walk_counts_invariants(oracle):
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-100, 100, (2000, 2))
+    mass = np.ones(2000)
+    c1 = oracle.walk_counts(pts, mass, 1.44, 1)
+    assert c1["steps"] == c1["acting_lanes"]  # one position per group: every visit a step
+    prev = c1["steps"]
+    for g in (2, 8, 64, 256, 2000):
+        c = oracle.walk_counts(pts, mass, 1.44, g)
+        assert {k: c[k] for k in ("acting_lanes", "accepted", "pairs")} == \
+            {k: c1[k] for k in ("acting_lanes", "accepted", "pairs")}
+        assert c["steps"] <= prev and c["steps"] * g >= c["acting_lanes"]  # merging groups
+        prev = c["steps"]
+    # theta 0 accepts nothing: every position walks the whole arena and meets every other
+    ncells = oracle.Tree(pts).ncells
+    c0 = oracle.walk_counts(pts, mass, 0.0, 64)
+    assert c0["accepted"] == 0 and c0["pairs"] == 2000 * 1999
+    assert c0["acting_lanes"] == 2000 * ncells and c0["steps"] == 32 * ncells

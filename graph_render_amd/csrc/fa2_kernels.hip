@@ -1508,11 +1508,21 @@ __global__ __launch_bounds__(WB) void k_walk(const FaDev d) {
             }
         }
         const bool open = act & !accept;
-        // the lanes that opened k act on k + 1 next; an accepting lane waits for skip
+        // the lanes that opened k act on k + 1 next; an accepting lane acts at skip. When no
+        // lane opened k the scan goes to skip itself and the accepting lanes are carried
+        // there; only when it descends do they wait for it (a stale wait lies behind the scan
+        // and never matches again)
         carried = acting & ~accepted;
         const bool down = carried != 0;
+        if (!down) {
+            // on to skip, the accepting lanes with it
+            carried = accepted;
+            k = ks;
+            cur = nxs;
+            continue;
+        }
         wait = accept ? cell.skip : wait;
-        if (down && cell.skip == k1) {
+        if (cell.skip == k1) {
             const bool pairs = open;
             T lf[D], lc[D];
 #pragma unroll
@@ -1558,9 +1568,9 @@ __global__ __launch_bounds__(WB) void k_walk(const FaDev d) {
                 for (int a = 0; a < D; a++) acc_add<T, KAHAN>(f[a], fc[a], lf[a]);
             }
         }
-        // down to k + 1 when a lane opened k, else on to skip
-        k = down ? k1 : ks;
-        cur = down ? nx1 : nxs;
+        // down to k + 1: a lane opened k
+        k = k1;
+        cur = nx1;
     }
     if constexpr (COUNT) {
         if (gr_lane() == 0) {

@@ -186,7 +186,7 @@ float pairwise_host(const float *v, uint32_t n) {  // mass.mean()'s sum
 
 struct FsGraph {
     uint32_t n, node0, tok0, ntok;
-    uint32_t iters, pad[3];
+    uint32_t iters, gid, pad[2];  // gid: the graph's index in the caller's batch
     double k, rep, grav, jt, scale;
     float soften, pad2;
 };
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_fs_layout(const FsDev d) {
         bad |= !isfinite(v);
         d.out[3 * (size_t)G.node0 + q] = (float)v;
     }
-    if (__syncthreads_or(bad) && tid == 0) d.status[blockIdx.x] = 2;
+    if (__syncthreads_or(bad) && tid == 0) d.status[G.gid] = 2;
 }
 
 // ---------------------------------------------------------------- host driver
@@ -603,7 +603,16 @@ gr_status fs_upload(const FsBatch &b, FsDevice &dv) {
     const size_t N = b.N ? b.N : 1, G = b.graphs.size() ? b.graphs.size() : 1;
     FS_TRY(hipStreamCreateWithFlags(&dv.stream, hipStreamNonBlocking));
     FsDev &d = dv.d;
-    FS_TRY(dv.upload(&dv.graphs, b.graphs));
+    // Workgroup b lays out the b-th record: records in decreasing order of work (n^2 pairs per
+    // iteration times iterations), so the launch deals the largest graphs out first, one per
+    // CU round after round, rather than leaving some CUs several of them. Each record holds
+    // its graph's offsets and index, so the order moves no result.
+    std::vector<FsGraph> order = b.graphs;
+    for (size_t g = 0; g < order.size(); g++) order[g].gid = (uint32_t)g;
+    std::stable_sort(order.begin(), order.end(), [](const FsGraph &a, const FsGraph &c) {
+        return (uint64_t)a.n * a.n * a.iters > (uint64_t)c.n * c.n * c.iters;
+    });
+    FS_TRY(dv.upload(&dv.graphs, order));
     FS_TRY(dv.upload(&dv.start, b.start));
     float *mass = nullptr;
     uint32_t *aoff = nullptr, *adj = nullptr, *elo = nullptr, *ehi = nullptr, *tok = nullptr;

@@ -642,12 +642,14 @@ gr_status fs_upload(const FsBatch &b, FsDevice &dv) {
 }
 
 // One whole layout of every graph from the resident start positions.
-void fs_launch(const FsBatch &b, FsDevice &dv) {
+hipError_t fs_launch(const FsBatch &b, FsDevice &dv) {
     const size_t N = b.N;
-    (void)hipMemcpyAsync(dv.d.pos, dv.start, N * 12, hipMemcpyDeviceToDevice, dv.stream);
-    (void)hipMemsetAsync(dv.d.prev, 0, N * 12, dv.stream);
-    (void)hipMemsetAsync(dv.d.status, 0, b.graphs.size() * 4, dv.stream);
-    if (b.graphs.empty()) return;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dv.d.pos, dv.start, N * 12, hipMemcpyDeviceToDevice, dv.stream)) ||
+        (e = hipMemsetAsync(dv.d.prev, 0, N * 12, dv.stream)) ||
+        (e = hipMemsetAsync(dv.d.status, 0, b.graphs.size() * 4, dv.stream)))
+        return e;
+    if (b.graphs.empty()) return hipSuccess;
     uint32_t biggest = 0;
     for (const FsGraph &g : b.graphs) biggest = std::max(biggest, g.n);
     if (biggest <= FS_LDS_NODES) {
@@ -658,6 +660,7 @@ void fs_launch(const FsBatch &b, FsDevice &dv) {
     } else
         hipLaunchKernelGGL(k_fs_layout<false>, dim3((uint32_t)b.graphs.size()), dim3(FS_BLOCK),
                            0, dv.stream, dv.d);
+    return hipGetLastError();
 }
 
 gr_status fs_run(uint32_t ngraphs, const uint32_t *node_counts, const size_t *edge_counts,
@@ -669,8 +672,7 @@ gr_status fs_run(uint32_t ngraphs, const uint32_t *node_counts, const size_t *ed
     if (r) return r;
     FsDevice dv;
     if ((r = fs_upload(b, dv))) return r;
-    fs_launch(b, dv);
-    FS_TRY(hipGetLastError());
+    FS_TRY(fs_launch(b, dv));
     FS_TRY(hipStreamSynchronize(dv.stream));
     if (reps && ms_out) {
         hipEvent_t e0, e1;
@@ -679,7 +681,7 @@ gr_status fs_run(uint32_t ngraphs, const uint32_t *node_counts, const size_t *ed
         double t[3];
         for (int q = 0; q < 3; q++) {
             FS_TRY(hipEventRecord(e0, dv.stream));
-            for (uint32_t k = 0; k < reps; k++) fs_launch(b, dv);
+            for (uint32_t k = 0; k < reps; k++) FS_TRY(fs_launch(b, dv));
             FS_TRY(hipEventRecord(e1, dv.stream));
             FS_TRY(hipEventSynchronize(e1));
             float ms = 0.f;

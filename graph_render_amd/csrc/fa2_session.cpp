@@ -275,8 +275,8 @@ struct gr_fa2_session {
         part_rows[0] = 0;
         part_rows[parts] = n;
         HIP_TRY(hipStreamSynchronize(d.stream));
-        HIP_TRY(hipMemcpy(d.part_rows, part_rows.data(), part_rows.size() * 4,
-                          hipMemcpyHostToDevice));
+        HIP_TRY(gr_copy_wait(d.part_rows, part_rows.data(), part_rows.size() * 4,
+                             hipMemcpyHostToDevice, d.stream));
         d.nparts = parts;
         return GR_OK;
     }
@@ -343,11 +343,13 @@ struct gr_fa2_session {
             if (d.precision == gr::PREC_FP64) {
                 std::vector<double> col(n);
                 for (uint32_t i = 0; i < n; i++) col[i] = rowmajor[(size_t)i * dim + a];
-                HIP_TRY(hipMemcpy(d.pos[a], col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+                HIP_TRY(gr_copy_wait(d.pos[a], col.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                                     d.stream));
             } else {
                 std::vector<float> col(n);
                 for (uint32_t i = 0; i < n; i++) col[i] = (float)rowmajor[(size_t)i * dim + a];
-                HIP_TRY(hipMemcpy(d.pos[a], col.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+                HIP_TRY(gr_copy_wait(d.pos[a], col.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                                     d.stream));
             }
         }
         gr::launch_bounds(d);
@@ -781,10 +783,11 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
         std::vector<float2> tw(d.pm_side / 2);
         gr::pm_twiddles(d.pm_side, tw.data());
         std::vector<uint8_t> zero(gr::pm_frame_bytes(), 0);
-        if (hipMemcpy(d.pm_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) !=
-                hipSuccess ||
-            hipMemcpy(d.pm_fr, zero.data(), zero.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(d.pm_a, 0, P2 * sizeof(float2)) != hipSuccess) {
+        if (gr_copy_wait(d.pm_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice,
+                         d.stream) != hipSuccess ||
+            gr_copy_wait(d.pm_fr, zero.data(), zero.size(), hipMemcpyHostToDevice,
+                         d.stream) != hipSuccess ||
+            gr_fill_wait(d.pm_a, 0, P2 * sizeof(float2), d.stream) != hipSuccess) {
             fail(GR_ERR_DEVICE, "hipMemcpy", "particle-mesh setup failed");
             delete s;
             return nullptr;
@@ -797,7 +800,7 @@ static gr_fa2_session *create_impl(uint32_t ngraphs, const uint32_t *node_counts
 
     auto upload = [&](void *dst, const void *src, size_t bytes) -> bool {
         if (bytes == 0) return true;
-        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+        return gr_copy_wait(dst, src, bytes, hipMemcpyHostToDevice, d.stream) == hipSuccess;
     };
     bool good = upload(d.row_off, g.row_off.data(), ((size_t)n + 1) * 4) &&
                 upload(d.nbr, g.nbr.data(), g.nbr.size() * 4) &&
@@ -1089,14 +1092,17 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
         std::vector<double> col(n);
         for (uint32_t x = 0; x < dim; x++) {
             for (uint32_t i = 0; i < n; i++) col[i] = pos_bar[(size_t)i * dim + x];
-            HIP_TRY(hipMemcpy(a.pb[x], col.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            HIP_TRY(gr_copy_wait(a.pb[x], col.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                                 d.stream));
         }
         if (a.m) {
-            HIP_TRY(hipMemcpy(a.elo, s->lo.data(), (size_t)a.m * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(a.ehi, s->hi.data(), (size_t)a.m * 4, hipMemcpyHostToDevice));
+            HIP_TRY(gr_copy_wait(a.elo, s->lo.data(), (size_t)a.m * 4, hipMemcpyHostToDevice,
+                                 d.stream));
+            HIP_TRY(gr_copy_wait(a.ehi, s->hi.data(), (size_t)a.m * 4, hipMemcpyHostToDevice,
+                                 d.stream));
         }
-        HIP_TRY(hipMemset(a.wb, 0, (size_t)a.m * 8));
-        HIP_TRY(hipMemset(a.sc, 0, sizeof(gr::AdjScal)));
+        HIP_TRY(gr_fill_wait(a.wb, 0, (size_t)a.m * 8, d.stream));
+        HIP_TRY(gr_fill_wait(a.sc, 0, sizeof(gr::AdjScal), d.stream));
     }
     GrGraph *tg = (GrGraph *)tape_g;
     GrState *ts = (GrState *)tape_st;
@@ -1143,18 +1149,19 @@ gr_status gr_fa2_session_adjoint(gr_fa2_session *s, const double *pos_bar, doubl
     if (start_bar) {
         std::vector<double> col(n);
         for (uint32_t x = 0; x < dim; x++) {
-            HIP_TRY(hipMemcpy(col.data(), a.pb[x], (size_t)n * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(gr_copy_wait(col.data(), a.pb[x], (size_t)n * 8, hipMemcpyDeviceToHost,
+                                 d.stream));
             for (uint32_t i = 0; i < n; i++) start_bar[(size_t)i * dim + x] = col[i];
         }
     }
     if (weight_bar && a.m) {
         std::vector<double> wb(a.m);
-        HIP_TRY(hipMemcpy(wb.data(), a.wb, (size_t)a.m * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(wb.data(), a.wb, (size_t)a.m * 8, hipMemcpyDeviceToHost, d.stream));
         for (uint32_t e = 0; e < a.m; e++) weight_bar[s->simple_raw[e]] = wb[e];
     }
     if (param_bar) {
         gr::AdjScal sc;
-        HIP_TRY(hipMemcpy(&sc, a.sc, sizeof(sc), hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(&sc, a.sc, sizeof(sc), hipMemcpyDeviceToHost, d.stream));
         param_bar[0] = sc.kbar;
         param_bar[1] = sc.gbar;
     }
@@ -1169,11 +1176,13 @@ static gr_status read_columns(gr_fa2_session *s, void *const *cols, double *out)
     for (uint32_t a = 0; a < dim; a++) {
         if (s->d.precision == gr::PREC_FP64) {
             std::vector<double> c(n);
-            HIP_TRY(hipMemcpy(c.data(), cols[a], (size_t)n * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(gr_copy_wait(c.data(), cols[a], (size_t)n * 8, hipMemcpyDeviceToHost,
+                                 s->d.stream));
             for (uint32_t i = 0; i < n; i++) out[(size_t)i * dim + a] = c[i];
         } else {
             std::vector<float> c(n);
-            HIP_TRY(hipMemcpy(c.data(), cols[a], (size_t)n * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(gr_copy_wait(c.data(), cols[a], (size_t)n * 4, hipMemcpyDeviceToHost,
+                                 s->d.stream));
             for (uint32_t i = 0; i < n; i++) out[(size_t)i * dim + a] = (double)c[i];
         }
     }
@@ -1258,7 +1267,8 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
     const uint32_t C = st.n_cells, dim = s->d.dim;
     if (ncells_out) *ncells_out = C;
     if (order && s->d.n)
-        HIP_TRY(hipMemcpy(order, s->d.order, (size_t)s->d.n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(order, s->d.order, (size_t)s->d.n * 4, hipMemcpyDeviceToHost,
+                             s->d.stream));
     if (C == 0) return ok();
     if (cap < C) {  // a size query (no cell output) is the first call of two
         if (bounds || skip || start || end || bodies)
@@ -1268,12 +1278,13 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
     if (bounds) {
         if (!s->d.export_bounds)
             return fail(GR_ERR_UNSUPPORTED, "bounds", "create the session with GR_EXT_EXPORT_TREE");
-        HIP_TRY(hipMemcpy(bounds, s->d.cbounds, (size_t)C * 2 * dim * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(bounds, s->d.cbounds, (size_t)C * 2 * dim * 8, hipMemcpyDeviceToHost,
+                             s->d.stream));
     }
     if (skip || start || end) {
         size_t wb = gr::walkcell_bytes(s->d.precision, dim);
         std::vector<uint8_t> buf((size_t)C * wb);
-        HIP_TRY(hipMemcpy(buf.data(), s->d.wc, buf.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(buf.data(), s->d.wc, buf.size(), hipMemcpyDeviceToHost, s->d.stream));
         size_t es = s->esize();
         // [clo[dim] in the FP32 precisions], c[dim], mass, w2, then skip, start, end
         size_t off = es * (dim + 2) + (s->d.precision == gr::PREC_FP64 ? 0 : es * dim);
@@ -1288,7 +1299,8 @@ gr_status gr_fa2_session_tree(gr_fa2_session *s, uint32_t *ncells_out, uint32_t 
     }
     if (bodies) {
         std::vector<double> raw((size_t)C * (1 + dim));
-        HIP_TRY(hipMemcpy(raw.data(), s->d.raw, raw.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(raw.data(), s->d.raw, raw.size() * 8, hipMemcpyDeviceToHost,
+                             s->d.stream));
         for (uint32_t k = 0; k < C; k++) {
             double m = raw[(size_t)k * (1 + dim)];
             bodies[(size_t)k * (1 + dim)] = m;
@@ -1518,7 +1530,8 @@ gr_status gr_pm_charge(uint32_t n, const double *xy, uint32_t side, double charg
     if (!r) r = read_columns(s, d.rep, out);
     if (!r && frame_out) {
         std::vector<uint8_t> fr(gr::pm_frame_bytes());
-        if (hipMemcpy(fr.data(), d.pm_fr, fr.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        if (gr_copy_wait(fr.data(), d.pm_fr, fr.size(), hipMemcpyDeviceToHost,
+                         d.stream) != hipSuccess) {
             r = fail(GR_ERR_DEVICE, "pm", "frame readback");
         } else {
             gr::pm_frame_read(fr.data(), frame_out);
@@ -1601,8 +1614,8 @@ struct D3Pass {
             return fail(GR_ERR_DEVICE, "tree", "a level table outside the arena");
         if (nint > 0 && (size_t)maxd + 3 > first) {
             lvl.resize((size_t)maxd + 3);
-            HIP_TRY(hipMemcpy(lvl.data() + first, g.lvl + first, (lvl.size() - first) * 4,
-                              hipMemcpyDeviceToHost));
+            HIP_TRY(gr_copy_wait(lvl.data() + first, g.lvl + first, (lvl.size() - first) * 4,
+                                 hipMemcpyDeviceToHost, g.stream));
         }
         if (nint > 0)
             for (uint32_t l = maxd + 1; l-- > 0;) {
@@ -1658,8 +1671,9 @@ static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double 
     if (out) gr::d3_launch_walk(g);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (out) HIP_TRY(hipMemcpy(out, g.out, (size_t)n * 16, hipMemcpyDeviceToHost));
-    if (ids && C) HIP_TRY(hipMemcpy(node_id, g.nid_out, (size_t)C * 4, hipMemcpyDeviceToHost));
+    if (out) HIP_TRY(gr_copy_wait(out, g.out, (size_t)n * 16, hipMemcpyDeviceToHost, g.stream));
+    if (ids && C) HIP_TRY(gr_copy_wait(node_id, g.nid_out, (size_t)C * 4, hipMemcpyDeviceToHost,
+                                       g.stream));
     if (reps && ms_out) {
         gr_status e = GR_OK;
         if ((r = time_median(s, reps, [&] { if (!e) e = p.again(); }, &ms_out[0]))) return r;
@@ -1669,7 +1683,7 @@ static gr_status d3_pass(gr_fa2_session *s, double theta, double charge, double 
         // the timed runs recomputed the same deltas: they must carry the same bits
         std::vector<double> again((size_t)n * 2);
         HIP_TRY(hipStreamSynchronize(g.stream));
-        HIP_TRY(hipMemcpy(again.data(), g.out, (size_t)n * 16, hipMemcpyDeviceToHost));
+        HIP_TRY(gr_copy_wait(again.data(), g.out, (size_t)n * 16, hipMemcpyDeviceToHost, g.stream));
         if (out && std::memcmp(again.data(), out, (size_t)n * 16) != 0)
             return fail(GR_ERR_DEVICE, "d3", "a timed run gave other deltas");
     }
@@ -1755,7 +1769,8 @@ gr_status gr_d3_collide(uint32_t n, const double *xy, double radius, uint32_t se
             t.sy = p.g.sy;
             gr::d3sim_collide(sd, t, tick);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(sd.stream) != hipSuccess ||
-                hipMemcpy(out, dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess)
+                gr_copy_wait(out, dl, (size_t)n * 16, hipMemcpyDeviceToHost,
+                             sd.stream) != hipSuccess)
                 r = fail(GR_ERR_DEVICE, "hip", "the collide pass failed on the device");
         }
     }
@@ -1901,14 +1916,14 @@ static gr_status d3sim_setup(D3Sim &S, uint32_t n, const uint32_t *src, const ui
     HIP_TRY(hipMemsetAsync(sd.vy, 0, nn * 8, sd.stream));
     HIP_TRY(hipStreamSynchronize(sd.stream));
     if (m) {
-        HIP_TRY(hipMemcpy(lo, g.lo.data(), (size_t)m * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(hi, g.hi.data(), (size_t)m * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(sl, slot.data(), (size_t)2 * m * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(dld, ld.data(), (size_t)m * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(dls, ls.data(), (size_t)m * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(dlb, lb.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+        HIP_TRY(gr_copy_wait(lo, g.lo.data(), (size_t)m * 4, hipMemcpyHostToDevice, sd.stream));
+        HIP_TRY(gr_copy_wait(hi, g.hi.data(), (size_t)m * 4, hipMemcpyHostToDevice, sd.stream));
+        HIP_TRY(gr_copy_wait(sl, slot.data(), (size_t)2 * m * 4, hipMemcpyHostToDevice, sd.stream));
+        HIP_TRY(gr_copy_wait(dld, ld.data(), (size_t)m * 8, hipMemcpyHostToDevice, sd.stream));
+        HIP_TRY(gr_copy_wait(dls, ls.data(), (size_t)m * 8, hipMemcpyHostToDevice, sd.stream));
+        HIP_TRY(gr_copy_wait(dlb, lb.data(), (size_t)m * 8, hipMemcpyHostToDevice, sd.stream));
     }
-    HIP_TRY(hipMemcpy(ro, roff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(gr_copy_wait(ro, roff.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, sd.stream));
     sd.lo = lo;
     sd.hi = hi;
     sd.roff = ro;
@@ -1972,14 +1987,14 @@ static gr_status pmd3_setup(D3Sim &S) {
         fwd[k] = root[(k % half) * (side / (2 * half))];
         inv[k] = make_double2(fwd[k].x, -fwd[k].y);
     }
-    HIP_TRY(hipMemcpy(tf, fwd.data(), side * 16, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ti, inv.data(), side * 16, hipMemcpyHostToDevice));
+    HIP_TRY(gr_copy_wait(tf, fwd.data(), side * 16, hipMemcpyHostToDevice, c->d.stream));
+    HIP_TRY(gr_copy_wait(ti, inv.data(), side * 16, hipMemcpyHostToDevice, c->d.stream));
     m.tw_fwd = tf;
     m.tw_inv = ti;
     std::vector<uint32_t> iota(nn);
     for (uint32_t i = 0; i < nn; i++) iota[i] = i;
-    HIP_TRY(hipMemcpy(m.order, iota.data(), nn * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(m.slot, iota.data(), nn * 4, hipMemcpyHostToDevice));
+    HIP_TRY(gr_copy_wait(m.order, iota.data(), nn * 4, hipMemcpyHostToDevice, c->d.stream));
+    HIP_TRY(gr_copy_wait(m.slot, iota.data(), nn * 4, hipMemcpyHostToDevice, c->d.stream));
     S.mesh = true;
     S.k_valid = false;
     return GR_OK;
@@ -2137,10 +2152,10 @@ gr_status gr_d3_session_state(gr_d3_session *h, double *x, double *y, double *vx
     const size_t bytes = (size_t)h->S.sd.n * 8;
     if (!bytes) return ok();
     HIP_TRY(hipStreamSynchronize(h->S.sd.stream));
-    if (x) HIP_TRY(hipMemcpy(x, h->S.sd.x, bytes, hipMemcpyDeviceToHost));
-    if (y) HIP_TRY(hipMemcpy(y, h->S.sd.y, bytes, hipMemcpyDeviceToHost));
-    if (vx) HIP_TRY(hipMemcpy(vx, h->S.sd.vx, bytes, hipMemcpyDeviceToHost));
-    if (vy) HIP_TRY(hipMemcpy(vy, h->S.sd.vy, bytes, hipMemcpyDeviceToHost));
+    if (x) HIP_TRY(gr_copy_wait(x, h->S.sd.x, bytes, hipMemcpyDeviceToHost, h->S.sd.stream));
+    if (y) HIP_TRY(gr_copy_wait(y, h->S.sd.y, bytes, hipMemcpyDeviceToHost, h->S.sd.stream));
+    if (vx) HIP_TRY(gr_copy_wait(vx, h->S.sd.vx, bytes, hipMemcpyDeviceToHost, h->S.sd.stream));
+    if (vy) HIP_TRY(gr_copy_wait(vy, h->S.sd.vy, bytes, hipMemcpyDeviceToHost, h->S.sd.stream));
     return ok();
 }
 
@@ -2224,8 +2239,10 @@ static gr_status pm_pass_session(uint32_t n, const double *xy, double radius, do
         y[i] = xy[2 * (size_t)i + 1];
     }
     gr::D3SimDev &sd = h->S.sd;
-    HIP_TRY(hipMemcpy(projected ? sd.px : sd.x, x.data(), (size_t)n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(projected ? sd.py : sd.y, y.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(gr_copy_wait(projected ? sd.px : sd.x, x.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                         sd.stream));
+    HIP_TRY(gr_copy_wait(projected ? sd.py : sd.y, y.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                         sd.stream));
     return GR_OK;
 }
 
@@ -2256,7 +2273,8 @@ gr_status gr_pm_d3_charge(uint32_t n, const double *xy, double charge, double al
             gr::pmd3_spectrum(S.pm, f.h, f.reach, S.sd.stream);
             gr::pmd3_solve_read(S.sd, S.pm, (uint32_t)f.cells, charge * alpha);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(S.sd.stream) != hipSuccess ||
-                hipMemcpy(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess)
+                gr_copy_wait(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost,
+                             S.sd.stream) != hipSuccess)
                 r = fail(GR_ERR_DEVICE, "mesh", "the mesh pass failed on the device");
             if (frame_out) {
                 frame_out[0] = f.step;
@@ -2287,9 +2305,11 @@ gr_status gr_pm_d3_collide(uint32_t n, const double *xy, double radius, uint32_t
         S.sd.seed = seed;
         gr::pmd3_collide(S.sd, S.pm, tick);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(S.sd.stream) != hipSuccess ||
-            hipMemcpy(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+            gr_copy_wait(out, S.sd.dl, (size_t)n * 16, hipMemcpyDeviceToHost,
+                         S.sd.stream) != hipSuccess ||
             (order_out &&
-             hipMemcpy(order_out, S.pm.order, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess))
+             gr_copy_wait(order_out, S.pm.order, (size_t)n * 4, hipMemcpyDeviceToHost,
+                          S.sd.stream) != hipSuccess))
             r = fail(GR_ERR_DEVICE, "mesh", "the collide pass failed on the device");
     }
     gr_d3_session_release(h);

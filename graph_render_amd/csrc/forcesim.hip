@@ -589,7 +589,8 @@ struct FsDevice {
     hipError_t upload(P **p, const std::vector<P> &v) {
         hipError_t e = alloc(p, v.size() * sizeof(P));
         if (e != hipSuccess || v.empty()) return e;
-        return hipMemcpy((void *)*p, v.data(), v.size() * sizeof(P), hipMemcpyHostToDevice);
+        return gr_copy_wait((void *)*p, v.data(), v.size() * sizeof(P), hipMemcpyHostToDevice,
+                            stream);
     }
 };
 
@@ -696,9 +697,12 @@ gr_status fs_run(uint32_t ngraphs, const uint32_t *node_counts, const size_t *ed
     }
     std::vector<float> xyz((size_t)b.N * 3);
     std::vector<int32_t> st(b.graphs.size());
-    if (b.N) FS_TRY(hipMemcpy(xyz.data(), dv.d.out, xyz.size() * 4, hipMemcpyDeviceToHost));
+    if (b.N)
+        FS_TRY(gr_copy_wait(xyz.data(), dv.d.out, xyz.size() * 4, hipMemcpyDeviceToHost,
+                            dv.stream));
     if (!st.empty())
-        FS_TRY(hipMemcpy(st.data(), dv.d.status, st.size() * 4, hipMemcpyDeviceToHost));
+        FS_TRY(gr_copy_wait(st.data(), dv.d.status, st.size() * 4, hipMemcpyDeviceToHost,
+                            dv.stream));
     for (size_t i = 0; i < b.N; i++) {
         if (out_x) out_x[i] = xyz[3 * i];
         if (out_y) out_y[i] = xyz[3 * i + 1];

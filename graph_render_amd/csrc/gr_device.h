@@ -128,3 +128,19 @@ static inline uint32_t gr_blocks(uint64_t n, uint32_t per) {
     uint64_t b = (n + per - 1) / per;
     return b == 0 ? 1u : (uint32_t)b;
 }
+
+// Host copies and fills on the caller's own stream, waited for before they return. Nothing in
+// the library goes on the legacy (null) stream: a legacy-stream copy made while another thread
+// captures a graph on its stream is refused (hipErrorStreamCaptureImplicit), so sessions on
+// different threads would interfere.
+static inline hipError_t gr_copy_wait(void *dst, const void *src, size_t bytes,
+                                      hipMemcpyKind kind, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+static inline hipError_t gr_fill_wait(void *dst, int value, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}

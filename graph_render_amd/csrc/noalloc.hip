@@ -187,17 +187,18 @@ gr_status gr_pm_charge_noalloc(uint32_t n, const double *xy, uint32_t side, doub
     d.pm_dmax2 = distance_max * distance_max;
     d.pm_strength = charge * alpha;
     d.pm_fix = std::ldexp(1.0, 60 - (int)std::ceil(std::log2((double)n + 1.0)));  // unit charges
-    d.stream = nullptr;
+    d.stream = hipStreamPerThread;  // the calling thread's own stream: none is created
     if (layout(base, n, side, d) > SCRATCH_BYTES)  // the limits above keep every call inside
         return gr_internal_fail(GR_ERR_CAPACITY, "scratch", "the call's buffers fit the array");
 
     // the twiddles as a session makes them (host cos/sin), so the transforms are the same
     float2 tw[GR_NOALLOC_MAX_SIDE / 2];
     gr::pm_twiddles(side, tw);
-    hipError_t e = hipMemcpy(d.pm_tw, tw, (size_t)(side / 2) * sizeof(float2),
-                             hipMemcpyHostToDevice);
+    hipError_t e = gr_copy_wait(d.pm_tw, tw, (size_t)(side / 2) * sizeof(float2),
+                                hipMemcpyHostToDevice, d.stream);
     if (e == hipSuccess)
-        e = hipMemcpy(d.pi, xy, (size_t)n * 2 * sizeof(double), hipMemcpyHostToDevice);
+        e = gr_copy_wait(d.pi, xy, (size_t)n * 2 * sizeof(double), hipMemcpyHostToDevice,
+                         d.stream);
     if (e != hipSuccess) return gr_internal_fail(GR_ERR_DEVICE, "hipMemcpy", hipGetErrorString(e));
     const uint32_t frame_words = (uint32_t)(gr::pm_frame_bytes() / 4);
     hipLaunchKernelGGL(k_na_setup, dim3(gr_blocks(n, GR_BLOCK)), dim3(GR_BLOCK), 0, d.stream, d,
@@ -207,13 +208,13 @@ gr_status gr_pm_charge_noalloc(uint32_t n, const double *xy, uint32_t side, doub
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(d.stream);
     if (e != hipSuccess) return gr_internal_fail(GR_ERR_DEVICE, "pm", hipGetErrorString(e));
-    e = hipMemcpy(out, d.pi, (size_t)n * 2 * sizeof(double), hipMemcpyDeviceToHost);
+    e = gr_copy_wait(out, d.pi, (size_t)n * 2 * sizeof(double), hipMemcpyDeviceToHost, d.stream);
     if (e != hipSuccess) return gr_internal_fail(GR_ERR_DEVICE, "hipMemcpy", hipGetErrorString(e));
     if (frame_out) {
         unsigned char fr[256];
         const size_t fb = gr::pm_frame_bytes();
         if (fb > sizeof(fr)) return gr_internal_fail(GR_ERR_DEVICE, "pm", "frame record size");
-        e = hipMemcpy(fr, d.pm_fr, fb, hipMemcpyDeviceToHost);
+        e = gr_copy_wait(fr, d.pm_fr, fb, hipMemcpyDeviceToHost, d.stream);
         if (e != hipSuccess) return gr_internal_fail(GR_ERR_DEVICE, "hipMemcpy", hipGetErrorString(e));
         gr::pm_frame_read(fr, frame_out);
     }

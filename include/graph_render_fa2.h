@@ -14,7 +14,9 @@
  *     the fields of StageError::Param / NonFinite (stage.rs:35-52);
  *   - input pointers are borrowed for the call only; outputs go to caller buffers;
  *   - device memory belongs to a session and is freed by gr_fa2_session_release();
- *   - one HIP stream per session; distinct sessions may run concurrently on one device.
+ *   - one HIP stream per session; distinct sessions may run concurrently on one device,
+ *     from different threads (nothing is issued on the legacy stream); a handle is used by
+ *     one thread at a time.
  */
 #ifndef GRAPH_RENDER_FA2_H
 #define GRAPH_RENDER_FA2_H

@@ -8,6 +8,7 @@ namespace gr {
 
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = GR_BLOCK * SCAN_ITEMS;
+constexpr uint32_t SCAN_SUMMED_TILES = 4 * GR_BLOCK;  // up to here the down-sweep adds the partials
 
 // Block-wide inclusive scan of one value per thread (256 threads), LDS-based.
 __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t *lds) {
@@ -102,13 +103,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 
 // Each wave scans its own SCAN_ITEMS * 64 consecutive values of the tile in rounds of 64,
 // carrying its running total; one exchange of the wave totals then gives every wave its
-// base, so the tile takes one barrier instead of two per round.
+// base, so the tile takes one barrier instead of two per round. SUMMED: the partials are the
+// tiles' plain sums (no k_scan_partials launch): each workgroup adds those of the tiles
+// before it, and the last one writes the grand total.
+template <bool SUMMED>
 __global__ __launch_bounds__(GR_BLOCK) void k_scan_down(const uint32_t *in, uint32_t *out,
                                                         uint32_t max_n, const uint32_t *d_count,
-                                                        const uint32_t *partials) {
+                                                        const uint32_t *partials,
+                                                        uint32_t *d_total) {
     constexpr uint32_t WAVES = GR_BLOCK / GR_WAVE;
     constexpr uint32_t PER_WAVE = SCAN_ITEMS * GR_WAVE;
     __shared__ uint32_t wtot[WAVES];
+    __shared__ uint32_t before[WAVES];
     const uint32_t n = load_count(d_count, max_n);
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
     const uint32_t tn = base < n ? (n - base < SCAN_TILE ? (uint32_t)(n - base) : SCAN_TILE) : 0;
@@ -118,7 +124,21 @@ __global__ __launch_bounds__(GR_BLOCK) void k_scan_down(const uint32_t *in, uint
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; r++)
         vin[r] = gr_load_u32(rin, (wave * PER_WAVE + (uint32_t)r * GR_WAVE + lane) * 4);
-    const uint32_t tile_base = partials[blockIdx.x];
+    uint32_t tile_base;
+    if (SUMMED) {
+        uint32_t t = 0;
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += GR_BLOCK) t += partials[i];
+        t = gr_wave_sum_u32(t);
+        if (lane == 0) before[wave] = t;
+        __syncthreads();
+        tile_base = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < WAVES; w++) tile_base += before[w];
+        if (d_total && threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x)
+            *d_total = tile_base + partials[blockIdx.x];
+    } else {
+        tile_base = partials[blockIdx.x];
+    }
     uint32_t run = 0, ex[SCAN_ITEMS];
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; r++) {
@@ -147,10 +167,15 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint32_t max_n, const uin
     uint32_t *partials = (uint32_t *)tmp;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(GR_BLOCK), 0, stream, in, max_n, d_count,
                        partials);
+    if (nb <= SCAN_SUMMED_TILES) {  // each workgroup adds at most 4 partials per thread
+        hipLaunchKernelGGL(k_scan_down<true>, dim3(nb), dim3(GR_BLOCK), 0, stream, in, out,
+                           max_n, d_count, partials, d_total);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(GR_BLOCK), 0, stream, partials, nb,
                        d_total);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(GR_BLOCK), 0, stream, in, out, max_n, d_count,
-                       partials);
+    hipLaunchKernelGGL(k_scan_down<false>, dim3(nb), dim3(GR_BLOCK), 0, stream, in, out, max_n,
+                       d_count, partials, nullptr);
 }
 
 // ------------------------------------------------------------------ radix sort

@@ -205,7 +205,7 @@ __device__ __forceinline__ uint32_t d3_wave_min(uint32_t v) {
     return v;
 }
 
-// charge.rs:196-240 as one wave-uniform preorder scan per 64 consecutive tree positions:
+// charge.rs:196-240 as one wave-uniform preorder scan per g consecutive tree positions:
 // the wave visits the smallest of its lanes' next cells, so each lane still meets its own
 // cells in preorder and adds its terms in the reference's order; a body is read once per
 // wave (a scalar load).
@@ -218,14 +218,16 @@ __global__ __launch_bounds__(D3_WALK) void k_d3_walk(const D3Dev d,
                                                      const uint32_t *__restrict__ order,
                                                      const double *__restrict__ sx,
                                                      const double *__restrict__ sy,
-                                                     double *__restrict__ out) {
-    const uint32_t nb = (d.n + D3_WALK - 1) / D3_WALK;
+                                                     double *__restrict__ out, uint32_t g) {
+    const uint32_t per = (D3_WALK / GR_WAVE) * g;  // g consecutive tree positions per wave
+    const uint32_t nb = (d.n + per - 1) / per;
     const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
     if (lb >= nb) return;
-    const uint32_t p = lb * D3_WALK + threadIdx.x;
+    const uint32_t lane = threadIdx.x % GR_WAVE;
+    const uint32_t p = lb * per + (threadIdx.x / GR_WAVE) * g + lane;
     // every node walks, the non-finite ones (tree positions nf..n, in no leaf) too
-    if (__ballot(p < d.n) == 0) return;
-    const bool live = p < d.n;
+    const bool live = lane < g && p < d.n;
+    if (__ballot(live) == 0) return;
     const uint32_t pc = live ? p : d.n - 1;
     const double xi = sx[pc], yi = sy[pc];
     const uint32_t i = order[pc];
@@ -327,10 +329,20 @@ void d3_launch_ids(const D3Dev &d) {
                        sorted);
 }
 
+// Bodies per wave of the two walks: 64 when that gives 4096 waves or more. A smaller tree
+// gets narrower waves (32 bodies, the other lanes idle) and twice as many: its scans are
+// latency-bound with few waves per SIMD, and a wave's steps fall with the bodies it carries
+// (100k nodes: many-body 0.274 -> 0.257 ms, collide 0.266 -> 0.228 ms; 16 bodies gain no
+// more). Each lane's terms do not depend on the lanes beside it: the deltas are the same bits.
+static uint32_t d3_walk_group(uint32_t n) {
+    return (uint64_t)n < (uint64_t)GR_WAVE * 4096 ? GR_WAVE / 2 : GR_WAVE;
+}
+
 void d3_launch_walk(const D3Dev &d) {
-    const uint32_t nb = gr_blocks(d.n, D3_WALK);
+    const uint32_t g = d3_walk_group(d.n);
+    const uint32_t nb = gr_blocks(d.n, (D3_WALK / GR_WAVE) * g);
     hipLaunchKernelGGL(k_d3_walk, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, d.stream, d,
-                       (const D3Body *)d.body, d.order, d.sx, d.sy, d.out);
+                       (const D3Body *)d.body, d.order, d.sx, d.sy, d.out, g);
 }
 
 // ---------------------------------------------------------------- the rest of the tick
@@ -467,18 +479,20 @@ __global__ __launch_bounds__(GR_BLOCK) void k_sim_integrate(const D3SimDev s) {
     s.y[i] += vy;
 }
 
-// collide.rs node_delta as one wave-uniform preorder scan per 64 consecutive tree
+// collide.rs node_delta as one wave-uniform preorder scan per g consecutive tree
 // positions (as the charge walk): a leaf's points are resolved pair by pair in the leaf's
 // order, an internal cell is skipped by the lanes for which its square lies beyond the
 // reach on some axis. Every node walks, non-finite ones too.
 __global__ __launch_bounds__(D3_WALK) void k_sim_collide(const D3SimDev s, const D3CollideTree t,
-                                                         uint32_t tick) {
-    const uint32_t nb = (t.n + D3_WALK - 1) / D3_WALK;
+                                                         uint32_t tick, uint32_t g) {
+    const uint32_t per = (D3_WALK / GR_WAVE) * g;  // g consecutive tree positions per wave
+    const uint32_t nb = (t.n + per - 1) / per;
     const uint32_t lb = gr_xcd_block(blockIdx.x, nb);
     if (lb >= nb) return;
-    const uint32_t p = lb * D3_WALK + threadIdx.x;
-    if (__ballot(p < t.n) == 0) return;
-    const bool live = p < t.n;
+    const uint32_t lane = threadIdx.x % GR_WAVE;
+    const uint32_t p = lb * per + (threadIdx.x / GR_WAVE) * g + lane;
+    const bool live = lane < g && p < t.n;
+    if (__ballot(live) == 0) return;
     const uint32_t pc = live ? p : t.n - 1;
     const double xi = t.sx[pc], yi = t.sy[pc];
     const uint32_t i = t.order[pc];
@@ -549,9 +563,10 @@ void d3sim_center_project(const D3SimDev &s) {
 }
 
 void d3sim_collide(const D3SimDev &s, const D3CollideTree &t, uint32_t tick) {
-    const uint32_t nb = gr_blocks(t.n, D3_WALK);
+    const uint32_t g = d3_walk_group(t.n);
+    const uint32_t nb = gr_blocks(t.n, (D3_WALK / GR_WAVE) * g);
     hipLaunchKernelGGL(k_sim_collide, dim3(gr_xcd_grid(nb)), dim3(D3_WALK), 0, s.stream, s, t,
-                       tick);
+                       tick, g);
 }
 
 void d3sim_integrate(const D3SimDev &s) {

@@ -333,7 +333,9 @@ void d3_launch_ids(const D3Dev &d) {
 // gets narrower waves (32 bodies, the other lanes idle) and twice as many: its scans are
 // latency-bound with few waves per SIMD, and a wave's steps fall with the bodies it carries
 // (100k nodes: many-body 0.274 -> 0.257 ms, collide 0.266 -> 0.228 ms; 16 bodies gain no
-// more). Each lane's terms do not depend on the lanes beside it: the deltas are the same bits.
+// more; at 1M nodes, where 64-body waves already give 15 per SIMD, 32 bodies are slower: the
+// many-body walk 1.40 against 1.16 ms). Each lane's terms do not depend on the lanes beside
+// it: the deltas are the same bits.
 static uint32_t d3_walk_group(uint32_t n) {
     return (uint64_t)n < (uint64_t)GR_WAVE * 4096 ? GR_WAVE / 2 : GR_WAVE;
 }

@@ -557,6 +557,78 @@ def bench_d3(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_ba2000(args):
+    """The reference's CPU-runnable configuration: preferential attachment, 2000 nodes, m 5,
+    seed 0, starts U[-100,100]^2, theta 1.2, FP64 with the reference's node-order global sums,
+    100 iterations (the whole layout). Layouts are timed on fresh sessions (graph upload and
+    the iteration's graph capture included), then 100 more iterations of one resident session;
+    the device's 100-iteration positions compared bit for bit with the oracle's, whose run of
+    the same 100 iterations on one core is the CPU rate."""
+    from graph_render_amd import synth
+    import graph_render_amd as G
+    w = synth.workload("ba2000")
+    n, iters = w["n"], w["iterations"]
+    topo = G.Topology(n, w["source"], w["target"])
+    params = G.Fa2Params(dim=2, seed=w["seed"], max_iter=iters)
+    ext = G.Fa2Ext(theta=w["theta"], precision="fp64", start=w["start"])
+    layouts, final = [], None
+    for r in range(max(args.warmup, 1) + max(args.steps // 10, 5)):
+        t0 = time.perf_counter()
+        with G.Session(topo, params, ext) as s:
+            done = s.step(iters)
+            final = s.positions()
+        el = time.perf_counter() - t0
+        assert done == iters, (done, iters)
+        if r >= max(args.warmup, 1):
+            layouts.append(el)
+    layout_s = float(np.median(layouts))
+    with G.Session(topo, params, ext) as s:
+        s.step(iters)
+        t0 = time.perf_counter()
+        more = s.step(iters)
+        steady = time.perf_counter() - t0
+        walk_ms = s.time("walk", 20)
+        ncells = s.tree(bounds=False)["ncells"]
+    wb = walk_bytes(n, ncells, "fp64")
+    achieved = wb / (walk_ms * 1e-3) / 1e9
+    print(f"ba2000: layout {layout_s * 1e3:.2f} ms, walk {walk_ms * 1e3:.1f} us, {ncells} cells",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": "FA2-BH layouts/s @BA(2000, m 5), 100 iterations, FP64 node-order sums",
+        "value": round(1.0 / layout_s, 3), "unit": "layouts/s", "n_gpus": 1,
+        "steps": len(layouts), "warmup": max(args.warmup, 1),
+        "ms_per_step": round(layout_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "preferential attachment, 2000 nodes, m 5, seed 0, starts "
+                               "U[-100,100]^2 (seed 0), theta 1.2, FP64, 100 iterations",
+                   "nodes": n, "raw_edges": int(len(w["source"]))},
+        "iters_per_s_in_layout": round(iters / layout_s, 1),
+        "steady_iters_per_s": round(more / steady, 1),
+        "walk_ms": round(walk_ms, 5), "tree_cells": ncells,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": None, "kernel": "k_walk<double,2>", "algorithmic_bytes": wb},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        info = cpu_info()
+        core = pin_one_core()
+        t0 = time.perf_counter()
+        ref = O.Fa2(n, w["source"], w["target"], dim=2, seed=w["seed"], start=w["start"],
+                    weights=None, tree=True, theta2=w["theta"] ** 2)
+        ran = ref.run(iters)
+        el = time.perf_counter() - t0
+        line["bit_exact_vs_oracle"] = bool(ran == iters and np.array_equal(
+            final.view(np.uint64), ref.positions().view(np.uint64)))
+        line["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "layouts/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "the whole 100-iteration layout of the same graph and "
+                                          "start (oracle/fa2_oracle.c, gcc -O2, one thread "
+                                          f"pinned to core {core}, graph setup included)",
+                                **info}
+    print(json.dumps(line), flush=True)
+
+
 def bench_d3sim(args):
     """layout.force.barnes_hut, the frozen d3 force simulation, on the device: the
     reference's gate model at 100k nodes (FORCE_CEILING, registry/force.rs:31), 112 ticks of
@@ -808,7 +880,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="time only the FP32 headline (for profiler runs)")
     ap.add_argument("--workload", choices=["sbm1m", "batch", "precision", "adjoint",
-                                           "autotune", "d3", "forcesim", "d3sim", "pmsim"],
+                                           "autotune", "d3", "forcesim", "d3sim", "pmsim",
+                                           "ba2000"],
                     default="sbm1m")
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args()
@@ -827,6 +900,9 @@ def main():
         return
     if args.workload == "autotune":
         bench_autotune(args)
+        return
+    if args.workload == "ba2000":
+        bench_ba2000(args)
         return
     if args.workload == "d3":
         bench_d3(args)

@@ -138,6 +138,7 @@ struct FaDev {
     // walk workgroups, radix-sort keys per thread (a tile is 256 of them)
     uint32_t walk_block = 256;
     int walk_xcd = 1;
+    uint32_t walk_group = 64;  // tree positions per walk wave (set per launch from n)
     uint32_t sort_items = 8;
 
     hipStream_t stream = nullptr;

@@ -1376,10 +1376,13 @@ __global__ __launch_bounds__(WB) void k_walk(const FaDev d) {
     const GrState *st = d.st;
     if (halted(d)) return;
     const uint32_t n = d.n;
-    const uint32_t lb = d.walk_xcd ? gr_xcd_block(blockIdx.x, (n + WB - 1) / WB) : blockIdx.x;
-    const uint32_t p = lb * WB + threadIdx.x;
-    if (__ballot(p < n) == 0) return;
-    const bool live = p < n;
+    // g consecutive tree positions per wave (64, or fewer on a small graph: more waves)
+    const uint32_t g = d.walk_group, per = (WB / GR_WAVE) * g;
+    const uint32_t lb = d.walk_xcd ? gr_xcd_block(blockIdx.x, (n + per - 1) / per) : blockIdx.x;
+    const uint32_t wl = threadIdx.x % GR_WAVE;
+    const uint32_t p = lb * per + (threadIdx.x / GR_WAVE) * g + wl;
+    const bool live = wl < g && p < n;
+    if (__ballot(live) == 0) return;
     const uint32_t pc = live ? p : n - 1;
     const WalkCell<T, D> *__restrict__ wc = (const WalkCell<T, D> *)d.wc;
     const T *__restrict__ sm = (const T *)d.sm;
@@ -2364,9 +2367,19 @@ static void tree_impl(FaDev &d, bool com = true) {
     }
 }
 
+// Tree positions per walk wave: 64 from 16,384 bodies up; a smaller graph's walk, a few
+// dozen latency-bound waves, gets narrower waves (at least 8 bodies, the other lanes idle)
+// and more of them. A lane's terms do not depend on the lanes beside it: the same bits.
+static uint32_t walk_group(uint32_t n) {
+    uint32_t g = GR_WAVE;
+    while (g > 8 && (uint64_t)n < (uint64_t)g * 256) g >>= 1;
+    return g;
+}
+
 template <typename T, int D, bool KAHAN, bool COUNT, int WB>
 static void walk_launch(FaDev &d) {
-    const uint32_t nb = gr_blocks(d.n, WB);
+    d.walk_group = walk_group(d.n);
+    const uint32_t nb = gr_blocks(d.n, (WB / GR_WAVE) * d.walk_group);
     const dim3 grid(d.walk_xcd ? gr_xcd_grid(nb) : nb);
     const bool buf = (uint64_t)(d.cap + 1) * sizeof(WalkCell<T, D>) < (1ull << 31);
     const bool one = d.gidn == nullptr;
